@@ -1,0 +1,158 @@
+/*
+ * sdft_hip.h -- C ABI of the MI355X (gfx950) stochastic-DFT filter library.
+ *
+ * Drop-in boundary for PINY's sDFT Chebyshev filter path.  Every entry point
+ * takes plain pointers and sizes; arrays are 0-based (the reference's
+ * 1-based `cmalloc(n)-1` arrays are passed as `&arr[1]`).  Orbital blocks use
+ * the reference's storage: separate real / imaginary FP64 arrays, state-major,
+ * `ncoef` coefficients per state with G=0 stored last (entry ncoef-1), Gamma
+ * symmetry c(-G) = c(G)*.  The real-space grid is the FFTW (a,b,c) row-major
+ * grid with c fastest.
+ *
+ * Every function returns 0 on success and a negative SDFT_E* code on error;
+ * sdft_hip_last_error() returns a message for the last error on the calling
+ * thread.  Calls on one context are not thread-safe; use one context per
+ * host thread (the reference's one MPI rank per state group).
+ */
+#ifndef SDFT_HIP_H
+#define SDFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDFT_OK 0
+#define SDFT_EINVAL (-1)   /* bad argument / unsupported shape */
+#define SDFT_EHIP (-2)     /* HIP runtime error */
+#define SDFT_ENOMEM (-3)   /* device allocation failed */
+#define SDFT_ESTATE (-4)   /* call out of order (e.g. no v_KS set) */
+
+#define SDFT_PREC_FP64 0   /* plain form: FP64 throughout */
+#define SDFT_PREC_FP32 1   /* precision form: FP32 FFT / v.psi / KB GEMM (MFMA),
+                              FP64 recurrence, accumulators and reductions */
+
+#define SDFT_RHO_FFTW 0    /* density in FFTW order (a,b,c), c fastest */
+#define SDFT_RHO_GENERIC 1 /* density in the generic PINY order, a fastest */
+
+typedef struct sdft_ctx sdft_ctx;     /* one system: grid, G-set, v_KS, KB table */
+typedef struct sdft_block sdft_block; /* a device-resident block of orbitals */
+
+/* Grid and plane-wave set.  Replaces the fields of PARA_FFT_PKG3D that the
+ * filter reads (include/typ_defs/typedefs_gen.h:383-516: nkf1..3, ncoef,
+ * mapFFTWFilter, mapConFFTWFilter built at mathlib/fft_create_package.c:237-248)
+ * and CPEWALD::ak2Kinetic (energy/cp/cp_energy_eext.c:2259-2304). */
+typedef struct sdft_grid_desc {
+  int n1, n2, n3;            /* FFT grid (nkf1, nkf2, nkf3); cubic n1=n2=n3
+                                in {24,32,36,48,64,96,128,192} */
+  int ncoef;                 /* coefficients per state incl. G=0 (last) */
+  const int *map;            /* [ncoef] 0-based grid index of +G (mapFFTWFilter) */
+  const int *map_conj;       /* [ncoef] 0-based grid index of -G (mapConFFTWFilter) */
+  const double *ak2_kinetic; /* [ncoef] |G|^2 used by the kinetic term */
+} sdft_grid_desc;
+
+/* Create a context on the current HIP device.  Replaces the FFTW plan and map
+ * set-up of fft_create_package.c:1627-1689 for the filter's grid. */
+int sdft_hip_init(const sdft_grid_desc *desc, sdft_ctx **out);
+int sdft_hip_finalize(sdft_ctx *ctx);
+const char *sdft_hip_last_error(void);
+
+/* Kohn-Sham potential v_KS(r), n1*n2*n3 doubles in FFTW order (the
+ * cpscr_rho.v_ks_up array after the transpose of
+ * energy/cp/cp_energy_ee_rho.c:1944-1970). */
+int sdft_hip_set_vks(sdft_ctx *ctx, const double *v_ks);
+
+/* Kleinman-Bylander projectors on the coefficient set: Q_p(G) = proj_re +
+ * i proj_im ([nproj][ncoef], G=0 last) and E_p (KB energy / volume).  The
+ * nonlocal energy of a state is sum_p E_p Z_p^2 with Z_p = sum_{all G} c(G)
+ * Q_p(G).  Replaces the per-call tables of the G-space KB path
+ * (stodft/reduced-energy/cp-energy-eext-stodft.c:701-1168, getnl_fcoef
+ * energy/cp/cp_energy_eext_nonloc.c:1669+).  nproj = 0 disables the term. */
+int sdft_hip_set_projectors(sdft_ctx *ctx, int nproj, const double *proj_re,
+                            const double *proj_im, const double *proj_energy);
+
+/* Raw coefficient force f = -dE/dc* of H applied to nstate orbitals
+ * (-4 Hc for G != 0, -2 Hc at G = 0): calcCoefForceWrapSCF,
+ * stodft/reduced-energy/energy-wrapper-scf.c:1066-1197 (zero f, G-space KB,
+ * pair-packed FFT v.psi, kinetic; coefForceCalcHybridSCF
+ * cp-energy-ee-rho-stodft.c:38-355). */
+int sdft_hip_hpsi(sdft_ctx *ctx, int nstate, const double *cre, const double *cim,
+                  double *fre, double *fim, int prec);
+
+/* Chebyshev expansion coefficients c_0..c_{L-1} of sqrt(f_FD) (fun_flag 2) or
+ * f_FD (fun_flag 1) on [emin, emax]: calcChebyCoeff, stodft/coeff.c:1093-1223
+ * (M = roundFFT(32 L) cosine grid, coeff.c:1556-1600). FP64 on the device. */
+int sdft_hip_cheby_coeffs(sdft_ctx *ctx, int L, double emin, double emax, double mu,
+                          double beta, int fun_flag, double *coeffs);
+
+/* Chebyshev filter acc = sum_{k<L} c_k T_k(H^) chi over nstate orbitals with
+ * H^ = scale (H - emean): filterChebyPolyHerm, stodft/filter.c:318-706 with
+ * normHCheby stodft/normh.c:182-344.  cre/cim hold chi on entry and
+ * T_{L-1} chi on return (the reference's side effect on cre_up/cim_up).
+ * moments (nullable, L doubles) receives sum_s <chi_s|T_k chi_s> as the
+ * storeChebyMomentsFlag branch (filter.c:458-510,572-594). */
+int sdft_hip_cheby_filter(sdft_ctx *ctx, int nstate, int L, const double *coeffs,
+                          double scale, double emean, double *cre, double *cim,
+                          double *out_re, double *out_im, double *moments, int prec);
+
+/* Chebyshev moments mu_0..mu_L (L+1 doubles, summed over the nstate
+ * orbitals, not normalised): calcChebyMoments,
+ * stodft/calc-chempot-chebyshev.c:596-913 (L/2 H applications, doubling
+ * identities).  cre/cim are not modified. */
+int sdft_hip_cheby_moments(sdft_ctx *ctx, int nstate, int L, double scale, double emean,
+                           const double *cre, const double *cim, double *moments, int prec);
+
+/* Chemical potential from the moments: calcChemPotCheby + calcNumElecCheby,
+ * stodft/calc-chempot-chebyshev.c:30-315,918-972.  moments are divided by
+ * nsto; N(mu) = occ sum_{k<L} c_k(mu) mu_k; bracket [mu_init -+ gap/2],
+ * widen, one interpolation step, bisection until |N - ne| <= tol.  The
+ * search runs in FP64 on the device. */
+int sdft_hip_chem_pot(sdft_ctx *ctx, int L, double emin, double emax, double beta,
+                      const double *moments, double nsto, double ne_true, double mu_init,
+                      double gap, double occ, double tol, double *mu_out, double *ne_out);
+
+/* Stochastic density rho(r) = occ_over_nsto * sum_s psi_s(r)^2 (psi from the
+ * unnormalised G->r transform): rhoCalcRealStoHybrid stodft/density.c:43-274
+ * and the scale of calcRhoStoHybridCheby :1478.  rho: n1*n2*n3 doubles in
+ * the requested layout (SDFT_RHO_FFTW or SDFT_RHO_GENERIC); accumulate != 0
+ * adds into rho instead of overwriting. */
+int sdft_hip_density(sdft_ctx *ctx, int nstate, const double *re, const double *im,
+                     double occ_over_nsto, double *rho, int layout, int accumulate);
+
+/* Kinetic and nonlocal energies of nstate orbitals (calcEnergyChemPot,
+ * stodft/calc-energy.c:133-191): eke = occ/2 sum 2|G|^2|c|^2 / nsto,
+ * enl = occ sum_s sum_p E_p Z_p^2 / nsto.  FP64 reductions on the device. */
+int sdft_hip_energies(sdft_ctx *ctx, int nstate, const double *re, const double *im,
+                      double occ, double nsto, double *eke, double *enl);
+
+/* ---- device-resident blocks (orbitals stay in HBM across calls) ---------- */
+
+/* Allocate device storage for nstate orbitals (T_k, T_{k-1}, accumulator,
+ * noise copy) in the library's internal coefficient order. */
+int sdft_hip_block_create(sdft_ctx *ctx, int nstate, int prec, sdft_block **out);
+int sdft_hip_block_destroy(sdft_block *blk);
+/* Host -> device copy of chi (reference order, FP64 re/im arrays). */
+int sdft_hip_block_set(sdft_block *blk, const double *cre, const double *cim);
+/* Filter on resident data (no host transfers); moments nullable (L doubles). */
+int sdft_hip_block_filter(sdft_block *blk, int L, const double *coeffs, double scale,
+                          double emean, double *moments);
+/* Device -> host copy: which = 0 filtered accumulator, 1 T_{L-1} (current). */
+int sdft_hip_block_get(sdft_block *blk, int which, double *re, double *im);
+
+/* Per-kernel timing of the last block_filter call when profiling is on:
+ * names[i] / ms[i] / launches[i] for up to max entries; returns the count. */
+int sdft_hip_set_profiling(sdft_ctx *ctx, int on);
+int sdft_hip_kernel_times(sdft_ctx *ctx, int max, const char **names, double *ms,
+                          int *launches);
+
+/* ---- self-test -------------------------------------------------------------
+ * Runs the library's LDS FFT on nlines interleaved complex lines of length n
+ * (dir +1: e^{+2 pi i jk/n} as FFTW_BACKWARD, -1: forward), FP64 (prec 0) or
+ * FP32 (prec 1) arithmetic; in/out are 2*n*nlines doubles.  No reference
+ * counterpart: a diagnostic for the transform the filter path uses in place
+ * of mathlib/fft_package.c's FFTW calls. */
+int sdft_hip_selftest_fft(int n, int nlines, int dir, int prec, const double *in, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDFT_HIP_H */

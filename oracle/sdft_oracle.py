@@ -1,0 +1,458 @@
+"""CPU FP64 oracle for PINY's stochastic-DFT Chebyshev filter path.
+
+TEST INFRASTRUCTURE ONLY.  Only `tests/`, `__graft_entry__.smoke()` and the
+`cpu_baseline` leg of `bench.py` may import this module, and only as the
+checker (or as the timed CPU baseline).  The product path never calls it.
+
+Each function restates one piece of the reference algorithm (cited by
+file:line under the reference tree) in plain numpy, FP64 throughout, with
+orbitals held as complex arrays (nstate, ncoef) whose last entry is G=0
+(real).  The plane-wave set, grid maps, potentials and projector tables are
+inputs (built by the package's system builder); the oracle restates what the
+filter path does with them.
+
+Parity status of each function (how it is pinned; see tests/):
+  pack_pair / unpack_pair_sub / hpsi_raw   : brute-force O(N^2) DFT on tiny
+      grids, plane-wave eigenvector closed form (H c = (|G|^2/2 + v0) c),
+      Hermiticity of H
+  nl_apply_gspace                           : Hermiticity, gradient of the NL
+      energy by finite differences, projector table vs real-space quadrature
+  norm_h_cheby / filter_cheby               : scalar recurrence on exact
+      eigenvectors, dense-H spectral oracle (sqrt f(H) chi by eigh)
+  cheby_coeffs / round_fft                  : direct cosine sums, fit-error
+      criterion of testChebyCoeff, roundFFT closed cases
+  cheby_moments / num_elec_cheby / chem_pot_cheby : doubling identities vs
+      direct <chi|T_k|chi>, eigenvalue trace on dense H, monotone N(mu)
+  density_real                              : sum(rho)/N = occ <chi|chi>/Nsto
+      identity and the brute-force real-space sum
+  energy_ke / energy_nl / nl_forces         : closed forms and FD derivatives
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+try:  # optional multi-threaded FFT for the timed CPU baseline
+    import scipy.fft as _sfft
+except Exception:  # pragma: no cover
+    _sfft = None
+
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(n: int) -> None:
+    """Threads used by the oracle's 3D FFTs (scipy.fft); 1 = numpy."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = max(1, int(n))
+
+
+def _ifftn_unnorm(z):
+    """FFTW_BACKWARD (e^{+iGr}, unnormalised) as planned at
+    mathlib/fft_create_package.c:1685-1687."""
+    if _sfft is not None and _FFT_WORKERS > 1:
+        return _sfft.ifftn(z, norm="forward", workers=_FFT_WORKERS)
+    return np.fft.ifftn(z) * z.size
+
+
+def _fftn(z):
+    """FFTW_FORWARD (e^{-iGr}, unnormalised)."""
+    if _sfft is not None and _FFT_WORKERS > 1:
+        return _sfft.fftn(z, workers=_FFT_WORKERS)
+    return np.fft.fftn(z)
+
+
+# ---------------------------------------------------------------------------
+# Pack / FFT / unpack  (mathlib/fft_package.c)
+# ---------------------------------------------------------------------------
+
+def pack_pair(c1, c2, mp, mpc, shape):
+    """dble_pack_coef_fftw3d_filter, fft_package.c:3195-3254: zero the grid,
+    Z[map(G)] = c1(G) + i c2(G), Z[mapC(G)] = c1(G)* + i c2(G)*,
+    Z[0] = c1re(0) + i c2re(0)."""
+    N = int(np.prod(shape))
+    z = np.zeros(N, dtype=np.complex128)
+    a, b = c1[:-1], c2[:-1]
+    z[mp[:-1]] = (a.real - b.imag) + 1j * (a.imag + b.real)
+    z[mpc[:-1]] = (a.real + b.imag) + 1j * (-a.imag + b.real)
+    z[0] = c1[-1].real + 1j * c2[-1].real
+    return z.reshape(shape)
+
+
+def to_r(z):
+    """para_fft_gen3d_fwd_to_r_fftw3d_filter, fft_package.c:3333-3401."""
+    return _ifftn_unnorm(z)
+
+
+def to_g(z):
+    """para_fft_gen3d_bck_to_g_fftw3d_filter, fft_package.c:3407-3504."""
+    return _fftn(z)
+
+
+def unpack_pair_sub(f1, f2, F, mp, mpc):
+    """dble_upack_coef_sum_fftw3d_filter, fft_package.c:3513-3599 (FFTW
+    branch :3573-3595): with F the forward-FFT output scaled by 1/N,
+      f1re -= 2(F(G)+F(-G))re   f1im -= 2(F(G)-F(-G))im
+      f2re -= 2(F(G)+F(-G))im   f2im -= 2(-F(G)+F(-G))re
+    and at G=0 f1re -= 2 F(0)re, f2re -= 2 F(0)im."""
+    Fl = F.ravel() / F.size
+    t = Fl[mp[:-1]]
+    tc = Fl[mpc[:-1]]
+    f1[:-1] -= 2.0 * ((t.real + tc.real) + 1j * (t.imag - tc.imag))
+    f2[:-1] -= 2.0 * ((t.imag + tc.imag) + 1j * (-t.real + tc.real))
+    f1[-1] -= 2.0 * Fl[0].real
+    f2[-1] -= 2.0 * Fl[0].imag
+
+
+# ---------------------------------------------------------------------------
+# Nonlocal Kleinman-Bylander (G-space)
+# ---------------------------------------------------------------------------
+
+def nl_projections(C, proj):
+    """Z[s, p] = sum_{all G} c_s(G) Q_p(G) = c(0)Q(0) + 2 sum_half Re(c Q).
+
+    The G-space KB matrix of controlEwdNonlocFilter / getNlmatFilter
+    (stodft/reduced-energy/cp-energy-eext-stodft.c:865-946,1105-1168) with the
+    structure factor, radial transform and Ylm folded into the table Q."""
+    if proj.shape[0] == 0:
+        return np.zeros((C.shape[0], 0))
+    Zh = 2.0 * (C[:, :-1].real @ proj[:, :-1].real.T - C[:, :-1].imag @ proj[:, :-1].imag.T)
+    Z0 = np.outer(C[:, -1].real, proj[:, -1].real)
+    return Zh + Z0
+
+
+def nl_apply_gspace(C, proj, energy):
+    """Raw NL force f = -d<psi|V_NL|psi>/dc* with the reference's scaling
+    (-4 H c for G != 0, -2 H c at G = 0; energy-wrapper-scf.c:1072-1076,
+    getnl_fcoef energy/cp/cp_energy_eext_nonloc.c:1669+):
+      f(G) = -4 sum_p E_p Z_p conj(Q_p(G))  (G != 0),  f(0) = -2 sum_p E_p Z_p Q_p(0)."""
+    f = np.zeros_like(C)
+    if proj.shape[0] == 0:
+        return f
+    Z = nl_projections(C, proj) * energy[None, :]
+    f[:, :-1] = -4.0 * (Z @ np.conj(proj[:, :-1]))
+    f[:, -1] = -2.0 * (Z @ proj[:, -1].real)
+    return f
+
+
+def nl_energy_per_state(C, proj, energy):
+    """<psi_s|V_NL|psi_s> = sum_p E_p Z_p^2 (sumNlPot, cp-energy-eext-stodft.c:650-696)."""
+    if proj.shape[0] == 0:
+        return np.zeros(C.shape[0])
+    Z = nl_projections(C, proj)
+    return (Z * Z * energy[None, :]).sum(axis=1)
+
+
+# ---------------------------------------------------------------------------
+# H.psi (raw force)
+# ---------------------------------------------------------------------------
+
+def hpsi_raw(C, sysm, with_nl=True):
+    """calcCoefForceWrapSCF (stodft/reduced-energy/energy-wrapper-scf.c:1066-1197)
+    -> calcNonLocalPseudoScf (G-space KB) + coefForceCalcHybridSCF
+    (cp-energy-ee-rho-stodft.c:38-355): per orbital pair pack -> G->r FFT ->
+    v.psi (cp_vpsi_threads, cp_energy_ee_rho.c:2817-2845) -> r->G FFT ->
+    unpack-subtract; an odd last orbital is paired with zero (identical math
+    to the single pack); kinetic f -= 2|G|^2 c (G != 0) and f_im(0) = 0."""
+    nstate, ncoef = C.shape
+    shape = tuple(sysm.n)
+    v = sysm.vks.reshape(shape)
+    f = np.zeros_like(C)
+    if with_nl and sysm.nproj > 0:
+        f += nl_apply_gspace(C, sysm.proj, sysm.proj_energy)
+    zero = np.zeros(ncoef, dtype=np.complex128)
+    for s in range(0, nstate, 2):
+        c1 = C[s]
+        c2 = C[s + 1] if s + 1 < nstate else zero
+        z = to_r(pack_pair(c1, c2, sysm.map, sysm.map_conj, shape))
+        z *= v
+        F = to_g(z)
+        f1 = f[s]
+        f2 = f[s + 1] if s + 1 < nstate else zero.copy()
+        unpack_pair_sub(f1, f2, F, sysm.map, sysm.map_conj)
+    f[:, :-1] -= 2.0 * sysm.ak2[None, :-1] * C[:, :-1]
+    f[:, -1] = f[:, -1].real
+    return f
+
+
+def hpsi(C, sysm, with_nl=True):
+    """Plain H c from the raw force: H c = -f/4 (G != 0), -f/2 (G = 0)."""
+    f = hpsi_raw(C, sysm, with_nl)
+    h = -0.25 * f
+    h[:, -1] = -0.5 * f[:, -1]
+    return h
+
+
+def gamma_dot(a, b):
+    """Gamma-point inner product 2 sum_{G!=0} Re(a* b) + a0 b0 (coded in the
+    reference as 2 ddot over all entries minus a0 b0,
+    calc-chempot-chebyshev.c:725-729)."""
+    return 2.0 * (np.sum(a.real * b.real, axis=-1) + np.sum(a.imag * b.imag, axis=-1)) \
+        - a[..., -1].real * b[..., -1].real
+
+
+# ---------------------------------------------------------------------------
+# Chebyshev recurrence and filter
+# ---------------------------------------------------------------------------
+
+def norm_h_cheby(c, f, t_prev, k, scale, emean):
+    """normHCheby recurrence part, stodft/normh.c:259-338 (in place on f):
+    k == 1: f = s1 f + p c (G != 0), f_re(0) = s2 f_re(0) + p c_re(0)
+    k > 1 : f = 2 s1 f + 2 p c - t_prev, likewise at G=0 with 2 s2
+    with s1 = -scale/4, s2 = -scale/2, p = -scale*emean; f_im(0) is left as
+    the raw force has it (0)."""
+    s1 = -scale * 0.25
+    s2 = -scale * 0.5
+    p = -scale * emean
+    if k == 1:
+        f[:, :-1] = s1 * f[:, :-1] + p * c[:, :-1]
+        f[:, -1] = (s2 * f[:, -1].real + p * c[:, -1].real) + 1j * f[:, -1].imag
+    else:
+        f[:, :-1] = 2.0 * s1 * f[:, :-1] + 2.0 * p * c[:, :-1] - t_prev[:, :-1]
+        f[:, -1] = (2.0 * s2 * f[:, -1].real + 2.0 * p * c[:, -1].real - t_prev[:, -1].real) \
+            + 1j * f[:, -1].imag
+    return f
+
+
+def filter_cheby(chi, coeffs, sysm, scale, emean, with_nl=True, moments=False):
+    """filterChebyPolyHerm, stodft/filter.c:318-706 (numChemPot = 1):
+    acc = c_0 chi; for k = 1..L-1: T_k by normHCheby, acc += c_k T_k.
+    Returns (acc, T_{L-1}) -- on return the reference leaves cre_up = T_{L-1}.
+    With moments=True also returns the optional moment accumulation of
+    filter.c:458-510,572-594 (sum_s <chi_s|T_k chi_s>, k < L)."""
+    L = len(coeffs)
+    acc = coeffs[0] * chi
+    c = chi.copy()
+    t_prev = chi.copy()
+    mom = np.zeros(L)
+    if moments:
+        mom[0] = gamma_dot(chi, chi).sum()
+    for k in range(1, L):
+        f = hpsi_raw(c, sysm, with_nl)
+        norm_h_cheby(c, f, t_prev, k, scale, emean)
+        if k > 1:
+            t_prev = c
+        c = f
+        if moments:
+            mom[k] = gamma_dot(chi, c).sum()
+        acc = acc + coeffs[k] * c
+    if moments:
+        return acc, c, mom
+    return acc, c
+
+
+# ---------------------------------------------------------------------------
+# Chebyshev coefficients (stodft/coeff.c)
+# ---------------------------------------------------------------------------
+
+def round_fft(n_in: int) -> int:
+    """roundFFT, coeff.c:1556-1600: nearest 2^n 3^m to n_in by the
+    reference's search (2^nmax first, then 2^i 3^(m+1) with m = floor of the
+    log ratio, strict improvement only)."""
+    ln2, ln3 = math.log(2.0), math.log(3.0)
+    lnn = math.log(n_in)
+    nmax = int(lnn / ln2) + 1
+    best = abs(2.0 ** nmax - n_in)
+    n, m = nmax, 0
+    for i in range(nmax):
+        x = max((lnn - i * ln2) / ln3, 0.0)
+        mnow = int(x) + 1
+        diff = abs(2.0 ** i * 3.0 ** mnow - n_in)
+        if diff < best:
+            m, n, best = mnow, i, diff
+    return (2 ** n) * (3 ** m)
+
+
+def fermi(x, mu, beta):
+    """fermiExpReal, mathlib/mathlib.c:781-785."""
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(beta * (x - mu)))
+
+
+def cheby_coeffs(L, emin, emax, mu, beta, fun_flag=2, grid=None):
+    """calcChebyCoeff, coeff.c:1093-1223 for funFlag 1 (f) and 2 (sqrt f):
+    x_j = (dE/2) cos(2 pi j/M) + E_mean, M = roundFFT(32 L);
+    c_k = 2 Re(DFT_k)/M (k >= 1), c_0 = Re(DFT_0)/M."""
+    M = grid if grid is not None else round_fft(32 * L)
+    dE = (emax - emin) * 0.5
+    em = 0.5 * (emax + emin)
+    j = np.arange(M)
+    x = dE * np.cos(2.0 * math.pi * j / M) + em
+    fv = fermi(x, mu, beta)
+    if fun_flag == 2:
+        fv = np.sqrt(fv)
+    X = np.fft.fft(fv)
+    c = 2.0 * X[:L].real / M
+    c[0] = X[0].real / M
+    return c
+
+
+def cheby_eval(coeffs, y):
+    """sum_k c_k T_k(y) by the three-term recurrence (testChebyCoeff,
+    coeff.c:1319-1460)."""
+    t0 = np.ones_like(y)
+    out = coeffs[0] * t0
+    if len(coeffs) == 1:
+        return out
+    t1 = y.copy()
+    out = out + coeffs[1] * t1
+    for k in range(2, len(coeffs)):
+        t2 = 2.0 * y * t1 - t0
+        out = out + coeffs[k] * t2
+        t0, t1 = t1, t2
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Moments and chemical potential (stodft/calc-chempot-chebyshev.c)
+# ---------------------------------------------------------------------------
+
+def cheby_moments(chi, sysm, L, scale, emean, with_nl=True):
+    """calcChebyMoments, calc-chempot-chebyshev.c:596-913 (summed over states,
+    not yet normalised): mu_0 = <chi|chi>, mu_1 = <chi|T_1>, then for
+    k = 2..L/2 one normHCheby per step and
+      mu_k = <chi|T_k>, mu_{2k-1} = 2<T_k|T_{k-1}> - mu_1,
+      mu_{2k} = 2<T_k|T_k> - mu_0
+    (the direct value of mu_k, written last, wins for k < L/2+1).
+    Returns mu_0..mu_L."""
+    nmom = L // 2 + 1 if L % 2 == 0 else (L + 1) // 2
+    mom = np.zeros(L + 1)
+    mom[0] = gamma_dot(chi, chi).sum()
+    c = chi.copy()
+    f = hpsi_raw(c, sysm, with_nl)
+    norm_h_cheby(c, f, chi, 1, scale, emean)
+    mom[1] = gamma_dot(chi, f).sum()
+    t_prev = chi.copy()
+    c = f
+    for k in range(2, nmom):
+        f = hpsi_raw(c, sysm, with_nl)
+        norm_h_cheby(c, f, t_prev, k, scale, emean)
+        mom[k] = gamma_dot(chi, f).sum()
+        mom[2 * k - 1] = 2.0 * gamma_dot(f, c).sum() - mom[1]
+        mom[2 * k] = 2.0 * gamma_dot(f, f).sum() - mom[0]
+        t_prev = c
+        c = f
+    return mom
+
+
+def num_elec_cheby(mu, moments_norm, L, emin, emax, beta, occ=2.0):
+    """calcNumElecCheby, calc-chempot-chebyshev.c:918-972:
+    N(mu) = occ sum_{k<L} c_k(mu; f) mu_k."""
+    c = cheby_coeffs(L, emin, emax, mu, beta, fun_flag=1)
+    return occ * float(np.dot(c, moments_norm[:L]))
+
+
+def chem_pot_cheby(moments, nsto, L, emin, emax, beta, ne_true, mu_init, gap,
+                   occ=2.0, tol_rel=1e-11, tol_abs=None, max_iter=200):
+    """calcChemPotCheby, calc-chempot-chebyshev.c:30-315: normalise moments by
+    the stochastic orbital count; bracket [mu0 -+ gap/2], widen by gap until
+    it brackets N_e; one linear-interpolation step; then bisection until
+    |N(mu) - N_e| <= tol (1e-11 N_e in the reference)."""
+    mnorm = moments / float(nsto)
+    tol = tol_abs if tol_abs is not None else tol_rel * ne_true
+
+    def N(mu):
+        return num_elec_cheby(mu, mnorm, L, emin, emax, beta, occ)
+
+    lo, hi = mu_init - 0.5 * gap, mu_init + 0.5 * gap
+    nlo, nhi = N(lo), N(hi)
+    it = 0
+    while nhi < ne_true and it < max_iter:
+        lo, hi = hi, hi + gap
+        nlo, nhi = N(lo), N(hi)
+        it += 1
+    while nlo > ne_true and it < max_iter:
+        hi, lo = lo, lo - gap
+        nlo, nhi = N(lo), N(hi)
+        it += 1
+    mu = (ne_true - nlo) * (hi - lo) / (nhi - nlo) + lo
+    nn = N(mu)
+    it = 0
+    while abs(nn - ne_true) > tol and it < max_iter:
+        if nn > ne_true:
+            hi, nhi = mu, nn
+        if nn < ne_true:
+            lo, nlo = mu, nn
+        mu = 0.5 * (lo + hi)
+        nn = N(mu)
+        it += 1
+    return mu, nn
+
+
+# ---------------------------------------------------------------------------
+# Density and energies
+# ---------------------------------------------------------------------------
+
+def density_real(C, sysm, occ_over_nsto):
+    """rhoCalcRealStoHybrid (stodft/density.c:43-274) + the occ/N_sto scale of
+    calcRhoStoHybridCheby (:1478): rho(r) = occ/N_sto sum_s psi_s(r)^2 with
+    psi from the pair-packed G->r FFT (sum_rho, fft_package.c:2502-2516).
+    Returned in the FFTW (a, b, c; c fastest) grid order."""
+    nstate, ncoef = C.shape
+    shape = tuple(sysm.n)
+    rho = np.zeros(shape)
+    zero = np.zeros(ncoef, dtype=np.complex128)
+    for s in range(0, nstate, 2):
+        c1 = C[s]
+        c2 = C[s + 1] if s + 1 < nstate else zero
+        z = to_r(pack_pair(c1, c2, sysm.map, sysm.map_conj, shape))
+        rho += z.real ** 2 + z.imag ** 2
+    return rho.ravel() * occ_over_nsto
+
+
+def energy_ke(C, ak2, occ, nsto):
+    """calc-energy.c:133-141,190: E_kin = occ/2 sum_s sum_{G!=0} 2|G|^2 |c|^2 / N_sto."""
+    e = 2.0 * (ak2[None, :-1] * (C[:, :-1].real ** 2 + C[:, :-1].imag ** 2)).sum()
+    return occ * 0.5 * e / nsto
+
+
+def energy_nl(C, proj, energy, occ, nsto):
+    """calc-energy.c:160-191: E_NL = occ sum_s <psi_s|V_NL|psi_s> / N_sto."""
+    return occ * nl_energy_per_state(C, proj, energy).sum() / nsto
+
+
+def nl_forces(C, sysm, occ, nsto):
+    """Hellmann-Feynman NL force on each atom (calc-nuclei-force.c:302-489,
+    G-space branch): F_a = -dE_NL/dR_a with Q_p(G) = q_p(G) e^{iG.R_a}, so
+    dZ_p/dR_a = -2 sum_half G Im(c(G) Q_p(G)); E_NL = occ/Nsto sum E_p Z^2."""
+    proj, E = sysm.proj, sysm.proj_energy
+    Z = nl_projections(C, proj)                     # (s, p)
+    cq = C[:, None, :-1] * proj[None, :, :-1]       # (s, p, g)
+    dZ = -2.0 * np.einsum("spg,gx->spx", cq.imag, sysm.gvec[:-1])
+    contrib = 2.0 * (E[None, :, None] * Z[:, :, None] * dZ).sum(axis=0)  # (p, 3)
+    F = np.zeros((sysm.natm, 3))
+    np.add.at(F, sysm.proj_atom, -contrib)
+    return F * occ / nsto
+
+
+# ---------------------------------------------------------------------------
+# Dense references for tiny systems
+# ---------------------------------------------------------------------------
+
+def gamma_real_basis_dim(ncoef):
+    return 2 * ncoef - 1
+
+
+def to_real_vec(C):
+    """Orthonormal real coordinates of Gamma coefficient vectors:
+    x = [sqrt2 Re c(G!=0), sqrt2 Im c(G!=0), c(0)], so x.y = gamma_dot."""
+    r2 = math.sqrt(2.0)
+    return np.concatenate([r2 * C[..., :-1].real, r2 * C[..., :-1].imag, C[..., -1:].real], axis=-1)
+
+
+def from_real_vec(x, ncoef):
+    r2 = math.sqrt(2.0)
+    m = ncoef - 1
+    c = (x[..., :m] + 1j * x[..., m:2 * m]) / r2
+    return np.concatenate([c, x[..., 2 * m:2 * m + 1] + 0j], axis=-1)
+
+
+def dense_hamiltonian(sysm, with_nl=True):
+    """H in the orthonormal real Gamma basis, built column by column from
+    hpsi (the FAST_FILTER idea of filter.c:1134-1526: exact spectral form)."""
+    nc = sysm.ncoef
+    dim = gamma_real_basis_dim(nc)
+    E = np.eye(dim)
+    C = from_real_vec(E, nc)
+    H = to_real_vec(hpsi(C, sysm, with_nl))
+    return H.T

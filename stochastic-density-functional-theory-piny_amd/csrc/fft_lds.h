@@ -1,0 +1,242 @@
+// fft_lds.h -- mixed-radix Stockham FFT on lines held in LDS (gfx950).
+//
+// A workgroup transforms `nlines` lines of compile-time length N that sit in
+// LDS (line l starts at buf + l*ld, elements contiguous).  Each pass with
+// radix R has N/R butterflies per line; the workgroup's threads first load
+// every butterfly's R inputs into registers, barrier, then twiddle, apply the
+// radix-R DFT and store in autosorted (Stockham) order, so the transform is
+// in place in LDS without a ping-pong buffer.  DIR = +1 computes
+// sum_k x_k e^{+2 pi i jk/N} (FFTW_BACKWARD, G->r), DIR = -1 the forward
+// transform; neither is normalised.
+//
+// Radix schedule: powers of two in radix-8 passes (one or two radix-4 passes
+// for the remainder), then radix-3 passes.  Supported N = 2^a 3^b.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SDFT_HD __host__ __device__
+#define SDFT_DEV __device__
+#else
+#define SDFT_HD
+#define SDFT_DEV
+#endif
+
+namespace sdft {
+
+template <typename T>
+struct Cx {
+  T x, y;
+};
+
+template <typename T>
+SDFT_HD inline Cx<T> cx(T a, T b) {
+  Cx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+SDFT_HD inline Cx<T> operator+(Cx<T> a, Cx<T> b) { return cx(a.x + b.x, a.y + b.y); }
+template <typename T>
+SDFT_HD inline Cx<T> operator-(Cx<T> a, Cx<T> b) { return cx(a.x - b.x, a.y - b.y); }
+template <typename T>
+SDFT_HD inline Cx<T> cmul(Cx<T> a, Cx<T> b) {
+  return cx(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+SDFT_HD inline Cx<T> cmulc(Cx<T> a, Cx<T> b) {
+  return cx(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+SDFT_HD inline Cx<T> cscale(Cx<T> a, T s) { return cx(a.x * s, a.y * s); }
+// multiply by DIR*i
+template <typename T, int DIR>
+SDFT_HD inline Cx<T> mul_di(Cx<T> a) {
+  return DIR > 0 ? cx(-a.y, a.x) : cx(a.y, -a.x);
+}
+
+// ---- radix schedule ------------------------------------------------------
+constexpr int exp2_of(int n) { int e = 0; while (n % 2 == 0) { n /= 2; ++e; } return e; }
+constexpr int exp3_of(int n) { int e = 0; while (n % 3 == 0) { n /= 3; ++e; } return e; }
+constexpr bool fft_size_ok(int n) {
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  return m == 1 && n > 1;
+}
+constexpr int n_pow2_passes(int N) {
+  int e = exp2_of(N);
+  if (e == 0) return 0;
+  if (e % 3 == 0) return e / 3;
+  if (e % 3 == 1) return e == 1 ? 1 : (e - 4) / 3 + 2;
+  return e / 3 + 1;
+}
+constexpr int n_passes(int N) { return n_pow2_passes(N) + exp3_of(N); }
+constexpr int radix_at(int N, int i) {
+  int e = exp2_of(N);
+  int p2 = n_pow2_passes(N);
+  if (i >= p2) return 3;
+  if (e % 3 == 0) return 8;
+  if (e % 3 == 1) {
+    if (e == 1) return 2;
+    return i < (e - 4) / 3 ? 8 : 4;
+  }
+  return i < e / 3 ? 8 : 4;
+}
+constexpr int ns_before(int N, int i) {
+  int ns = 1;
+  for (int k = 0; k < i; ++k) ns *= radix_at(N, k);
+  return ns;
+}
+constexpr int min_radix(int N) {
+  int r = 8;
+  for (int i = 0; i < n_passes(N); ++i) r = radix_at(N, i) < r ? radix_at(N, i) : r;
+  return r;
+}
+// Lines a workgroup of NT threads transforms per call: enough that the
+// smallest-radix pass gives each thread at most two butterflies.
+constexpr int fft_max_lines(int N, int NT) {
+  int m = 2 * NT * min_radix(N) / N;
+  return m < 1 ? 1 : m;
+}
+
+// ---- small DFTs (in registers), sign DIR -------------------------------
+template <typename T, int R, int DIR>
+struct Dft;
+
+template <typename T, int DIR>
+struct Dft<T, 2, DIR> {
+  SDFT_HD static inline void run(Cx<T>* v) {
+    Cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+
+template <typename T, int DIR>
+struct Dft<T, 3, DIR> {
+  SDFT_HD static inline void run(Cx<T>* v) {
+    const T c = T(-0.5);
+    const T s = T(DIR) * T(0.86602540378443864676372317075294);
+    Cx<T> t1 = v[1] + v[2];
+    Cx<T> t2 = v[1] - v[2];
+    Cx<T> m = cx(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    // i*s*t2
+    Cx<T> is = cx(-s * t2.y, s * t2.x);
+    v[0] = v[0] + t1;
+    v[1] = m + is;
+    v[2] = m - is;
+  }
+};
+
+template <typename T, int DIR>
+struct Dft<T, 4, DIR> {
+  SDFT_HD static inline void run(Cx<T>* v) {
+    Cx<T> a = v[0] + v[2];
+    Cx<T> b = v[0] - v[2];
+    Cx<T> c = v[1] + v[3];
+    Cx<T> d = mul_di<T, DIR>(v[1] - v[3]);
+    v[0] = a + c;
+    v[1] = b + d;
+    v[2] = a - c;
+    v[3] = b - d;
+  }
+};
+
+template <typename T, int DIR>
+struct Dft<T, 8, DIR> {
+  SDFT_HD static inline void run(Cx<T>* v) {
+    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<T, 4, DIR>::run(e);
+    Dft<T, 4, DIR>::run(o);
+    const T h = T(0.70710678118654752440084436210485);
+    // w^1 = (1 + DIR i)/sqrt2, w^2 = DIR i, w^3 = (-1 + DIR i)/sqrt2
+    Cx<T> o1 = o[1], o2 = o[2], o3 = o[3];
+    Cx<T> w1o = cx(h * (o1.x - T(DIR) * o1.y), h * (o1.y + T(DIR) * o1.x));
+    Cx<T> w2o = mul_di<T, DIR>(o2);
+    Cx<T> w3o = cx(h * (-o3.x - T(DIR) * o3.y), h * (-o3.y + T(DIR) * o3.x));
+    v[0] = e[0] + o[0];
+    v[4] = e[0] - o[0];
+    v[1] = e[1] + w1o;
+    v[5] = e[1] - w1o;
+    v[2] = e[2] + w2o;
+    v[6] = e[2] - w2o;
+    v[3] = e[3] + w3o;
+    v[7] = e[3] - w3o;
+  }
+};
+
+// ---- one Stockham pass over lines in LDS ---------------------------------
+// tw[m] = exp(+2 pi i m / N), m < N (the table for the line length).
+template <typename T, int N, int DIR, int PASS, int MAXLINES, int NT>
+SDFT_DEV inline void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+  constexpr int R = radix_at(N, PASS);
+  constexpr int NB = N / R;
+  constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
+  constexpr int NS = ns_before(N, PASS);
+  constexpr int STEP = N / (NS * R);
+  const int total = nlines * NB;
+  Cx<T> v[MAXB][R];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = idx / NB;
+      const int j = idx - line * NB;
+      const Cx<T>* src = buf + line * ld + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = src[r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = idx / NB;
+      const int j = idx - line * NB;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const Cx<T> w = tw[r * k * STEP];
+          v[b][r] = DIR > 0 ? cmul(v[b][r], w) : cmulc(v[b][r], w);
+        }
+      }
+      Dft<T, R, DIR>::run(v[b]);
+      Cx<T>* dst = buf + line * ld + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * NS] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// All passes of a length-N transform over nlines <= MAXLINES lines (the
+// pass count is a compile-time constant; the if-constexpr chain unrolls it).
+template <typename T, int N, int DIR, int MAXLINES, int NT>
+SDFT_DEV inline void fft_lines(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+  static_assert(fft_size_ok(N), "FFT length must be 2^a 3^b");
+  constexpr int P = n_passes(N);
+  static_assert(P >= 1 && P <= 6, "unsupported pass count");
+  fft_pass<T, N, DIR, 0, MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  if constexpr (P > 1) fft_pass<T, N, DIR, (P > 1 ? 1 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  if constexpr (P > 2) fft_pass<T, N, DIR, (P > 2 ? 2 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  if constexpr (P > 3) fft_pass<T, N, DIR, (P > 3 ? 3 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  if constexpr (P > 4) fft_pass<T, N, DIR, (P > 4 ? 4 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  if constexpr (P > 5) fft_pass<T, N, DIR, (P > 5 ? 5 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
+}
+
+// Any number of lines: batches of MAXLINES.
+template <typename T, int N, int DIR, int MAXLINES, int NT>
+SDFT_DEV inline void fft_lines_batched(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+  for (int l0 = 0; l0 < nlines; l0 += MAXLINES) {
+    const int nl = nlines - l0 < MAXLINES ? nlines - l0 : MAXLINES;
+    fft_lines<T, N, DIR, MAXLINES, NT>(buf + l0 * ld, nl, ld, tw, tid);
+  }
+}
+
+}  // namespace sdft

@@ -1,0 +1,957 @@
+// sdft_api.hip -- the C ABI (include/sdft_hip.h): context set-up, host<->device
+// transfers and the orchestration of the filter path on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sdft_hip.h"
+#include "sdft_internal.h"
+
+struct sdft_ctx {
+  sdft::Ctx c;
+};
+struct sdft_block {
+  sdft::Block b;
+};
+
+namespace sdft {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  g_err = buf;
+  return SDFT_EHIP;
+}
+
+// roundFFT, stodft/coeff.c:1556-1600: closest 2^n 3^m to n_in by the
+// reference's search order.
+int round_fft(int n_in) {
+  const double ln2 = std::log(2.0), ln3 = std::log(3.0);
+  const double lnn = std::log((double)n_in);
+  const int nmax = (int)(lnn / ln2) + 1;
+  double best = std::fabs(std::pow(2.0, nmax) - n_in);
+  int n = nmax, m = 0;
+  for (int i = 0; i < nmax; ++i) {
+    const double x = std::fmax((lnn - i * ln2) / ln3, 0.0);
+    const int mnow = (int)x + 1;
+    const double diff = std::fabs(std::pow(2.0, i) * std::pow(3.0, mnow) - n_in);
+    if (diff < best) {
+      m = mnow;
+      n = i;
+      best = diff;
+    }
+  }
+  int out = 1;
+  for (int i = 0; i < n; ++i) out *= 2;
+  for (int i = 0; i < m; ++i) out *= 3;
+  return out;
+}
+
+template <typename P>
+static int dalloc(P** p, size_t bytes) {
+  void* v = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&v, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return SDFT_ENOMEM;
+  }
+  *p = reinterpret_cast<P*>(v);
+  return 0;
+}
+
+template <typename V>
+static int upload(V** dst, const std::vector<V>& src) {
+  int rc = dalloc(dst, sizeof(V) * src.size());
+  if (rc) return rc;
+  SDFT_HIP_CHECK(hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int signed_k(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// Build the column / chunk / entry tables from the caller's maps.
+static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
+  Geometry& g = c.g;
+  if (d->n1 != d->n2 || d->n1 != d->n3) {
+    set_error("only cubic grids n1 = n2 = n3 are supported");
+    return SDFT_EINVAL;
+  }
+  const int n = d->n1;
+  if (chunk_cols_for(n) == 0) {
+    set_error("grid size must be one of 24,32,36,48,64,96,128,192");
+    return SDFT_EINVAL;
+  }
+  if (d->ncoef < 2 || !d->map || !d->map_conj || !d->ak2_kinetic) {
+    set_error("ncoef < 2 or null map / ak2 pointer");
+    return SDFT_EINVAL;
+  }
+  g.n = n;
+  g.N = (long long)n * n * n;
+  g.ncoef = d->ncoef;
+  g.ncp = (g.ncoef + 63) / 64 * 64;
+  const int nc = g.ncoef;
+  if (d->map[nc - 1] != 0) {
+    set_error("the last coefficient must be G=0 (map[ncoef-1] == 0)");
+    return SDFT_EINVAL;
+  }
+  std::vector<int> ka(nc), kb(nc), kc(nc);
+  int K = 0;
+  for (int i = 0; i < nc; ++i) {
+    const long long m = d->map[i];
+    if (m < 0 || m >= g.N) {
+      set_error("map index out of the grid");
+      return SDFT_EINVAL;
+    }
+    const int ia = (int)(m / ((long long)n * n)), ib = (int)((m / n) % n), ic = (int)(m % n);
+    ka[i] = signed_k(ia, n);
+    kb[i] = signed_k(ib, n);
+    kc[i] = signed_k(ic, n);
+    if (i < nc - 1) {
+      if (ka[i] == 0 && kb[i] == 0 && kc[i] == 0) {
+        set_error("G=0 appears before the last coefficient");
+        return SDFT_EINVAL;
+      }
+      const long long mc = d->map_conj[i];
+      const long long want = ((long long)((-ka[i] + n) % n) * n + (-kb[i] + n) % n) * n +
+                             (-kc[i] + n) % n;
+      if (mc != want) {
+        set_error("map_conj is not the index of -G");
+        return SDFT_EINVAL;
+      }
+    }
+    K = std::max(K, std::max(std::abs(ka[i]), std::max(std::abs(kb[i]), std::abs(kc[i]))));
+  }
+  if (2 * K + 1 > plane_rows_max_for(n) || 2 * K >= n) {
+    set_error("plane-wave sphere too large for the grid (need 2K+1 <= 3n/8+1)");
+    return SDFT_EINVAL;
+  }
+  g.K = K;
+  g.M = 2 * K + 1;
+
+  // columns (kb, kc) of the full sphere
+  std::map<std::pair<int, int>, std::vector<std::tuple<int, int, int>>> cols;  // -> (ka, i, flag)
+  for (int i = 0; i < nc - 1; ++i) {
+    cols[{kb[i], kc[i]}].push_back(std::make_tuple(ka[i], i, 0));
+    cols[{-kb[i], -kc[i]}].push_back(std::make_tuple(-ka[i], i, 1));
+  }
+  cols[{0, 0}].push_back(std::make_tuple(0, nc - 1, 2));
+  // column order: pairs (key, -key) with key positive, then (0,0) last
+  std::vector<std::pair<int, int>> order;
+  for (auto& kv : cols) {
+    const int b = kv.first.first, cc = kv.first.second;
+    const bool pos = (b > 0) || (b == 0 && cc > 0);
+    if (!pos) continue;
+    order.push_back(kv.first);
+    if (!cols.count({-b, -cc})) {
+      set_error("sphere is not inversion symmetric");
+      return SDFT_EINVAL;
+    }
+    order.push_back({-b, -cc});
+  }
+  order.push_back({0, 0});
+  g.ncol = (int)order.size();
+  g.ncolp = (g.ncol + 3) / 4 * 4;
+  std::map<std::pair<int, int>, int> colid;
+  for (int dd = 0; dd < g.ncol; ++dd) colid[order[dd]] = dd;
+
+  // chunks: column pairs packed up to CH columns; (0,0) joins the last chunk
+  const int CH = chunk_cols_for(n);
+  std::vector<int> chunk_col{0};
+  int cur = 0;
+  for (int dd = 0; dd + 1 < g.ncol; dd += 2) {
+    if (cur + 2 > CH) {
+      chunk_col.push_back(dd);
+      cur = 0;
+    }
+    cur += 2;
+  }
+  if (cur + 1 > CH) {
+    chunk_col.push_back(g.ncol - 1);
+    cur = 0;
+  }
+  chunk_col.push_back(g.ncol);
+  g.nchunk = (int)chunk_col.size() - 1;
+  g.chunk_max = CH;
+
+  // internal coefficient order: chunk by chunk, column by column, +G entries
+  // sorted by ka; G=0 last
+  g.perm.assign(nc, -1);
+  g.iperm.assign(nc, -1);
+  std::vector<int> chunk_coef{0};
+  int j = 0;
+  for (int ch = 0; ch < g.nchunk; ++ch) {
+    for (int dd = chunk_col[ch]; dd < chunk_col[ch + 1]; ++dd) {
+      auto& ents = cols[order[dd]];
+      std::sort(ents.begin(), ents.end());
+      for (auto& e : ents) {
+        if (std::get<2>(e) != 0) continue;
+        const int i = std::get<1>(e);
+        g.perm[j] = i;
+        g.iperm[i] = j;
+        ++j;
+      }
+    }
+    if (ch == g.nchunk - 1) {
+      g.perm[j] = nc - 1;
+      g.iperm[nc - 1] = j;
+      ++j;
+    }
+    chunk_coef.push_back(j);
+  }
+  if (j != nc) {
+    set_error("internal error: coefficient permutation incomplete");
+    return SDFT_EINVAL;
+  }
+  // entries per column (for the pack), coefficient positions (for the unpack)
+  std::vector<int> ent_off{0}, ent_j, ent_b, col_pos(g.ncol);
+  std::vector<int> coef_plus(nc), coef_minus(nc);
+  const int LD = n + 1;
+  for (int dd = 0; dd < g.ncol; ++dd) {
+    const int b = order[dd].first, cc = order[dd].second;
+    col_pos[dd] = (b + K) * LD + ((cc % n) + n) % n;
+    for (auto& e : cols[order[dd]]) {
+      const int a = std::get<0>(e), i = std::get<1>(e), flag = std::get<2>(e);
+      const int ia = ((a % n) + n) % n;
+      ent_j.push_back(g.iperm[i]);
+      ent_b.push_back((dd << 10) | (flag << 8) | ia);
+      if (flag == 0 || flag == 2) coef_plus[g.iperm[i]] = (dd << 8) | ia;
+      if (flag == 1 || flag == 2) coef_minus[g.iperm[i]] = (dd << 8) | ia;
+    }
+    ent_off.push_back((int)ent_j.size());
+  }
+  g.ak2.assign(g.ncp, 0.0);
+  for (int jj = 0; jj < nc; ++jj) g.ak2[jj] = d->ak2_kinetic[g.perm[jj]];
+
+  std::vector<double> tw64(2 * n);
+  std::vector<float> tw32(2 * n);
+  for (int m = 0; m < n; ++m) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m /
+                            (long double)n;
+    tw64[2 * m] = (double)cosl(ang);
+    tw64[2 * m + 1] = (double)sinl(ang);
+    tw32[2 * m] = (float)cosl(ang);
+    tw32[2 * m + 1] = (float)sinl(ang);
+  }
+  int rc;
+  if ((rc = upload(&c.t.col_pos, col_pos))) return rc;
+  if ((rc = upload(&c.t.ent_off, ent_off))) return rc;
+  if ((rc = upload(&c.t.ent_j, ent_j))) return rc;
+  if ((rc = upload(&c.t.ent_b, ent_b))) return rc;
+  if ((rc = upload(&c.t.chunk_col, chunk_col))) return rc;
+  if ((rc = upload(&c.t.chunk_coef, chunk_coef))) return rc;
+  if ((rc = upload(&c.t.coef_plus, coef_plus))) return rc;
+  if ((rc = upload(&c.t.coef_minus, coef_minus))) return rc;
+  if ((rc = upload(&c.t.ak2, g.ak2))) return rc;
+  double* t64 = nullptr;
+  float* t32 = nullptr;
+  if ((rc = upload(&t64, tw64))) return rc;
+  if ((rc = upload(&t32, tw32))) return rc;
+  c.t.tw64 = t64;
+  c.t.tw32 = t32;
+  return 0;
+}
+
+static void free_ctx(Ctx& c) {
+  void* ptrs[] = {c.t.col_pos,  c.t.ent_off,    c.t.ent_j,     c.t.ent_b,      c.t.chunk_col,
+                  c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
+                  c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
+                  c.E,          c.W,            c.zpart,       c.Y,            c.Z,
+                  c.red};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+// ---- profiling ------------------------------------------------------------
+struct ProfRec {
+  int cls;
+  hipEvent_t a, b;
+};
+static thread_local std::vector<ProfRec> g_prof;
+static thread_local std::vector<hipEvent_t> g_evpool;
+
+static hipEvent_t ev_get() {
+  if (!g_evpool.empty()) {
+    hipEvent_t e = g_evpool.back();
+    g_evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+static int prof_class(Ctx& c, const char* name) {
+  for (size_t i = 0; i < c.prof_names.size(); ++i)
+    if (c.prof_names[i] == name) return (int)i;
+  c.prof_names.push_back(name);
+  c.prof_ms.push_back(0.0);
+  c.prof_launch.push_back(0);
+  return (int)c.prof_names.size() - 1;
+}
+
+struct ProfScope {
+  Ctx& c;
+  int cls = -1;
+  hipEvent_t a{}, b{};
+  ProfScope(Ctx& cc, const char* name) : c(cc) {
+    if (!c.profiling) return;
+    cls = prof_class(c, name);
+    a = ev_get();
+    b = ev_get();
+    (void)hipEventRecord(a, c.stream);
+  }
+  ~ProfScope() {
+    if (cls < 0) return;
+    (void)hipEventRecord(b, c.stream);
+    g_prof.push_back({cls, a, b});
+  }
+};
+
+static void prof_reset(Ctx& c) {
+  for (size_t i = 0; i < c.prof_ms.size(); ++i) {
+    c.prof_ms[i] = 0.0;
+    c.prof_launch[i] = 0;
+  }
+}
+
+static int prof_collect(Ctx& c) {
+  if (g_prof.empty()) return 0;
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (auto& r : g_prof) {
+    float ms = 0.f;
+    SDFT_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    c.prof_ms[r.cls] += ms;
+    c.prof_launch[r.cls] += 1;
+    g_evpool.push_back(r.a);
+    g_evpool.push_back(r.b);
+  }
+  g_prof.clear();
+  return 0;
+}
+
+// ---- scratch --------------------------------------------------------------
+static int ensure_W(Ctx& c, size_t bytes) {
+  if (c.W_bytes >= bytes) return 0;
+  if (c.W) SDFT_HIP_CHECK(hipFree(c.W));
+  c.W = nullptr;
+  c.W_bytes = 0;
+  int rc = dalloc(&c.W, bytes);
+  if (rc) return rc;
+  c.W_bytes = bytes;
+  return 0;
+}
+
+// ---- block helpers -------------------------------------------------------
+static int block_alloc(Ctx& c, int nstate, int prec, Block& b) {
+  b.ctx = &c;
+  b.nstate = nstate;
+  b.nstate_pad = nstate + (nstate & 1);
+  b.prec = prec;
+  const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
+  int rc;
+  if ((rc = dalloc(&b.X0, bytes))) return rc;
+  if ((rc = dalloc(&b.X1, bytes))) return rc;
+  if ((rc = dalloc(&b.acc, bytes))) return rc;
+  if ((rc = dalloc(&b.chi, bytes))) return rc;
+  if ((rc = dalloc(&b.fnl, bytes))) return rc;
+  SDFT_HIP_CHECK(hipMemsetAsync(b.X0, 0, bytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.X1, 0, bytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.acc, 0, bytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.chi, 0, bytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.fnl, 0, bytes, c.stream));
+  b.cur = 0;
+  return 0;
+}
+
+static void block_free(Block& b) {
+  void* ptrs[] = {b.X0, b.X1, b.acc, b.chi, b.fnl, b.mom_part};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  b = Block();
+}
+
+// host (reference order, re/im arrays) -> device internal layout
+static int block_put(Block& b, double* dst, const double* re, const double* im) {
+  Ctx& c = *b.ctx;
+  const int nc = c.g.ncoef, ncp = c.g.ncp;
+  std::vector<double> h((size_t)b.nstate_pad * 2 * ncp, 0.0);
+  for (int s = 0; s < b.nstate; ++s) {
+    double* row = h.data() + (size_t)s * 2 * ncp;
+    const double* r = re + (size_t)s * nc;
+    const double* i = im + (size_t)s * nc;
+    for (int j = 0; j < nc; ++j) {
+      row[j] = r[c.g.perm[j]];
+      row[ncp + j] = i[c.g.perm[j]];
+    }
+  }
+  SDFT_HIP_CHECK(hipMemcpyAsync(dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int block_fetch(Block& b, const double* src, double* re, double* im) {
+  Ctx& c = *b.ctx;
+  const int nc = c.g.ncoef, ncp = c.g.ncp;
+  std::vector<double> h((size_t)b.nstate_pad * 2 * ncp);
+  SDFT_HIP_CHECK(hipMemcpyAsync(h.data(), src, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (int s = 0; s < b.nstate; ++s) {
+    const double* row = h.data() + (size_t)s * 2 * ncp;
+    double* r = re + (size_t)s * nc;
+    double* i = im + (size_t)s * nc;
+    for (int j = 0; j < nc; ++j) {
+      r[c.g.perm[j]] = row[j];
+      i[c.g.perm[j]] = row[ncp + j];
+    }
+  }
+  return 0;
+}
+
+// One application of the H sandwich to `cur`; `mode` selects the epilogue.
+static int apply_step(Block& b, int mode, const double* cur, double* out, const StepParams& sp,
+                      double* mom_part) {
+  Ctx& c = *b.ctx;
+  const int npairs = b.nstate_pad / 2;
+  int rc;
+  if ((rc = ensure_W(c, w_bytes(c, npairs, b.prec)))) return rc;
+  const double* fnl = nullptr;
+  if (c.nproj > 0) {
+    {
+      ProfScope ps(c, "kb_project");
+      if ((rc = launch_nl_project(c, b.prec, cur, b.nstate_pad))) return rc;
+    }
+    {
+      ProfScope ps(c, "kb_force");
+      if ((rc = launch_nl_force(c, b.prec, b.nstate_pad, b.fnl))) return rc;
+    }
+    fnl = b.fnl;
+  }
+  {
+    ProfScope ps(c, "expand_a");
+    if ((rc = launch_expand(c, b.prec, cur, npairs, c.W))) return rc;
+  }
+  {
+    ProfScope ps(c, "plane_bc_vpsi");
+    if ((rc = launch_plane(c, b.prec, c.W, npairs))) return rc;
+  }
+  {
+    ProfScope ps(c, "contract_a");
+    if ((rc = launch_contract(c, b.prec, mode, c.W, npairs, cur, out, b.acc, fnl, b.chi, sp,
+                              mom_part)))
+      return rc;
+  }
+  return 0;
+}
+
+static int check_vks(Ctx& c) {
+  if (!c.have_vks) {
+    set_error("v_KS not set (call sdft_hip_set_vks first)");
+    return SDFT_ESTATE;
+  }
+  return 0;
+}
+
+static int ensure_mom(Block& b, int nsteps) {
+  Ctx& c = *b.ctx;
+  const size_t nwg = (size_t)c.g.nchunk * (b.nstate_pad / 2);
+  const size_t need = sizeof(double) * 3 * nwg * (size_t)nsteps;
+  if (b.mom_part_bytes >= need) return 0;
+  if (b.mom_part) SDFT_HIP_CHECK(hipFree(b.mom_part));
+  b.mom_part = nullptr;
+  b.mom_part_bytes = 0;
+  int rc = dalloc(&b.mom_part, need);
+  if (rc) return rc;
+  b.mom_part_bytes = need;
+  return 0;
+}
+
+__global__ void k_scale_copy(const double* __restrict__ x, double a, double* __restrict__ y,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+// The Chebyshev filter on resident data (filter.c:318-706).
+static int block_filter_impl(Block& b, int L, const double* coeffs, double scale, double emean,
+                             double* moments) {
+  Ctx& c = *b.ctx;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  if (L < 1) {
+    set_error("polynomial length must be >= 1");
+    return SDFT_EINVAL;
+  }
+  if (moments && L < 2) {
+    set_error("moments need L >= 2");
+    return SDFT_EINVAL;
+  }
+  double* X[2] = {b.X0, b.X1};
+  double* T0 = X[b.cur];
+  double* T1 = X[1 - b.cur];
+  const size_t n = (size_t)b.nstate_pad * 2 * c.g.ncp;
+  if (L == 1) {
+    hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, T0,
+                       coeffs[0], b.acc, n);
+    SDFT_HIP_CHECK(hipGetLastError());
+    SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return 0;
+  }
+  if (moments && (rc = ensure_mom(b, L))) return rc;
+  prof_reset(c);
+  double* bufs[2] = {T0, T1};
+  for (int k = 1; k < L; ++k) {
+    StepParams sp;
+    const double m = (k == 1) ? 1.0 : 2.0;
+    sp.s1 = -scale * 0.25 * m;
+    sp.s2 = -scale * 0.5 * m;
+    sp.p = -scale * emean * m;
+    sp.ck = coeffs[k];
+    sp.c0 = coeffs[0];
+    sp.first = (k == 1);
+    sp.step = k;
+    double* cur = bufs[(k - 1) & 1];
+    double* out = bufs[k & 1];
+    if ((rc = apply_step(b, kModeFilter, cur, out, sp, moments ? b.mom_part : nullptr))) return rc;
+  }
+  // T_{L-1} sits in bufs[(L-1) & 1]
+  b.cur = (bufs[(L - 1) & 1] == b.X0) ? 0 : 1;
+  if ((rc = prof_collect(c))) return rc;
+  if (moments) {
+    const int nwg = c.g.nchunk * (b.nstate_pad / 2);
+    std::vector<double> red((size_t)3 * L);
+    if ((rc = moments_reduce(c, b.mom_part, L, nwg, red.data()))) return rc;
+    moments[0] = red[3 * 1 + 1];
+    for (int k = 1; k < L; ++k) moments[k] = red[3 * k + 0];
+  }
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+}  // namespace sdft
+
+using namespace sdft;
+
+extern "C" {
+
+const char* sdft_hip_last_error(void) { return g_err.c_str(); }
+
+int sdft_hip_init(const sdft_grid_desc* desc, sdft_ctx** out) {
+  if (!desc || !out) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  *out = nullptr;
+  sdft_ctx* h = new sdft_ctx();
+  Ctx& c = h->c;
+  hipError_t e = hipGetDevice(&c.device);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_fail(e, "hipGetDevice", __FILE__, __LINE__);
+  }
+  e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
+  }
+  int rc = build_geometry(c, desc);
+  if (rc) {
+    free_ctx(c);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int sdft_hip_finalize(sdft_ctx* h) {
+  if (!h) return 0;
+  (void)hipStreamSynchronize(h->c.stream);
+  free_ctx(h->c);
+  delete h;
+  return 0;
+}
+
+int sdft_hip_set_vks(sdft_ctx* h, const double* v) {
+  if (!h || !v) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  const size_t N = (size_t)c.g.N;
+  for (size_t i = 0; i < N; ++i)
+    if (!std::isfinite(v[i])) {
+      set_error("v_KS contains a non-finite value");
+      return SDFT_EINVAL;
+    }
+  int rc;
+  if (!c.vks64 && (rc = dalloc(&c.vks64, sizeof(double) * N))) return rc;
+  if (!c.vks32 && (rc = dalloc(&c.vks32, sizeof(float) * N))) return rc;
+  std::vector<float> v32(N);
+  for (size_t i = 0; i < N; ++i) v32[i] = (float)v[i];
+  SDFT_HIP_CHECK(hipMemcpy(c.vks64, v, sizeof(double) * N, hipMemcpyHostToDevice));
+  SDFT_HIP_CHECK(hipMemcpy(c.vks32, v32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  c.have_vks = true;
+  return 0;
+}
+
+int sdft_hip_set_projectors(sdft_ctx* h, int nproj, const double* pre, const double* pim,
+                            const double* pe) {
+  if (!h || nproj < 0 || (nproj > 0 && (!pre || !pim || !pe))) {
+    set_error("bad projector arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  if (c.A64) (void)hipFree(c.A64);
+  if (c.A32) (void)hipFree(c.A32);
+  if (c.E) (void)hipFree(c.E);
+  c.A64 = nullptr;
+  c.A32 = nullptr;
+  c.E = nullptr;
+  c.nproj = 0;
+  if (nproj == 0) return 0;
+  const int nc = c.g.ncoef, ncp = c.g.ncp;
+  const size_t K = 2 * (size_t)ncp;
+  std::vector<double> A((size_t)nproj * K, 0.0);
+  std::vector<float> A32((size_t)nproj * K, 0.f);
+  for (int p = 0; p < nproj; ++p) {
+    for (int j = 0; j < nc; ++j) {
+      const int i = c.g.perm[j];
+      const bool g0 = (j == nc - 1);
+      const double w = g0 ? 1.0 : 2.0;
+      const double qr = pre[(size_t)p * nc + i], qi = pim[(size_t)p * nc + i];
+      A[(size_t)p * K + j] = w * qr;
+      A[(size_t)p * K + ncp + j] = g0 ? 0.0 : -w * qi;
+    }
+  }
+  for (size_t i = 0; i < A.size(); ++i) A32[i] = (float)A[i];
+  int rc;
+  if ((rc = upload(&c.A64, A))) return rc;
+  if ((rc = upload(&c.A32, A32))) return rc;
+  std::vector<double> E(pe, pe + nproj);
+  if ((rc = upload(&c.E, E))) return rc;
+  c.nproj = nproj;
+  return 0;
+}
+
+int sdft_hip_hpsi(sdft_ctx* h, int nstate, const double* cre, const double* cim, double* fre,
+                  double* fim, int prec) {
+  if (!h || nstate < 1 || !cre || !cim || !fre || !fim || (prec != 0 && prec != 1)) {
+    set_error("bad hpsi arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  Block b;
+  if ((rc = block_alloc(c, nstate, prec, b))) {
+    block_free(b);
+    return rc;
+  }
+  StepParams sp{};
+  if (!(rc = block_put(b, b.X0, cre, cim)) && !(rc = apply_step(b, kModeForce, b.X0, b.X1, sp,
+                                                                  nullptr)))
+    rc = block_fetch(b, b.X1, fre, fim);
+  if (!rc) rc = prof_collect(c);
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_cheby_coeffs(sdft_ctx* h, int L, double emin, double emax, double mu, double beta,
+                          int fun_flag, double* coeffs) {
+  if (!h || L < 1 || !coeffs || !(emax > emin) || (fun_flag != 1 && fun_flag != 2)) {
+    set_error("bad coefficient arguments");
+    return SDFT_EINVAL;
+  }
+  return cheby_coeffs_device(h->c, L, emin, emax, mu, beta, fun_flag, coeffs);
+}
+
+int sdft_hip_cheby_filter(sdft_ctx* h, int nstate, int L, const double* coeffs, double scale,
+                          double emean, double* cre, double* cim, double* ore, double* oim,
+                          double* moments, int prec) {
+  if (!h || nstate < 1 || L < 1 || !coeffs || !cre || !cim || !ore || !oim ||
+      (prec != 0 && prec != 1)) {
+    set_error("bad filter arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  Block b;
+  int rc = block_alloc(c, nstate, prec, b);
+  if (!rc) rc = block_put(b, b.X0, cre, cim);
+  if (!rc && moments) {
+    const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
+    SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
+  }
+  if (!rc) rc = block_filter_impl(b, L, coeffs, scale, emean, moments);
+  if (!rc) rc = block_fetch(b, b.acc, ore, oim);
+  if (!rc) rc = block_fetch(b, b.cur ? b.X1 : b.X0, cre, cim);
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_cheby_moments(sdft_ctx* h, int nstate, int L, double scale, double emean,
+                           const double* cre, const double* cim, double* moments, int prec) {
+  if (!h || nstate < 1 || L < 2 || !cre || !cim || !moments || (prec != 0 && prec != 1)) {
+    set_error("bad moment arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  Block b;
+  rc = block_alloc(c, nstate, prec, b);
+  if (!rc) rc = block_put(b, b.X0, cre, cim);
+  const int nmom = (L % 2 == 0) ? L / 2 + 1 : (L + 1) / 2;
+  if (!rc) rc = ensure_mom(b, nmom);
+  if (!rc) {
+    const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
+    SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
+    prof_reset(c);
+    double* bufs[2] = {b.X0, b.X1};
+    for (int k = 1; k < nmom && !rc; ++k) {
+      StepParams sp;
+      const double m = (k == 1) ? 1.0 : 2.0;
+      sp.s1 = -scale * 0.25 * m;
+      sp.s2 = -scale * 0.5 * m;
+      sp.p = -scale * emean * m;
+      sp.ck = 0.0;
+      sp.c0 = 0.0;
+      sp.first = (k == 1);
+      sp.step = k;
+      rc = apply_step(b, kModeMoments, bufs[(k - 1) & 1], bufs[k & 1], sp, b.mom_part);
+    }
+    if (!rc) rc = prof_collect(c);
+  }
+  if (!rc) {
+    const int nwg = c.g.nchunk * (b.nstate_pad / 2);
+    std::vector<double> red((size_t)3 * nmom);
+    rc = moments_reduce(c, b.mom_part, nmom, nwg, red.data());
+    if (!rc) {
+      // calc-chempot-chebyshev.c:722-897 write order (direct mu_k wins)
+      for (int k = 0; k <= L; ++k) moments[k] = 0.0;
+      moments[0] = red[3 * 1 + 1];
+      moments[1] = red[3 * 1 + 0];
+      for (int k = 2; k < nmom; ++k) {
+        moments[k] = red[3 * k + 0];
+        moments[2 * k - 1] = 2.0 * red[3 * k + 1] - moments[1];
+        moments[2 * k] = 2.0 * red[3 * k + 2] - moments[0];
+      }
+    }
+  }
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_chem_pot(sdft_ctx* h, int L, double emin, double emax, double beta,
+                      const double* moments, double nsto, double ne_true, double mu_init,
+                      double gap, double occ, double tol, double* mu_out, double* ne_out) {
+  if (!h || L < 1 || !moments || !mu_out || !ne_out || !(emax > emin) || !(nsto > 0) ||
+      !(gap > 0) || !(tol > 0)) {
+    set_error("bad chemical-potential arguments");
+    return SDFT_EINVAL;
+  }
+  return chem_pot_device(h->c, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ, tol,
+                         mu_out, ne_out);
+}
+
+int sdft_hip_density(sdft_ctx* h, int nstate, const double* re, const double* im,
+                     double occ_over_nsto, double* rho, int layout, int accumulate) {
+  if (!h || nstate < 1 || !re || !im || !rho || (layout != 0 && layout != 1)) {
+    set_error("bad density arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  Block b;
+  int rc = block_alloc(c, nstate, 0, b);
+  if (!rc) rc = block_put(b, b.X0, re, im);
+  const int npairs = b.nstate_pad / 2;
+  const int ngroups = std::max(1, std::min(npairs, 2048 / c.g.n));
+  double* part = nullptr;
+  double* drho = nullptr;
+  if (!rc) rc = ensure_W(c, w_bytes(c, npairs, 0));
+  if (!rc) rc = dalloc(&part, sizeof(double) * (size_t)ngroups * c.g.N);
+  if (!rc) rc = dalloc(&drho, sizeof(double) * (size_t)c.g.N);
+  if (!rc && accumulate) {
+    hipError_t e = hipMemcpy(drho, rho, sizeof(double) * c.g.N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy rho", __FILE__, __LINE__);
+  }
+  if (!rc) rc = launch_expand(c, 0, b.X0, npairs, c.W);
+  if (!rc) rc = launch_density(c, c.W, npairs, part, ngroups);
+  if (!rc) rc = launch_rho_finish(c, part, ngroups, occ_over_nsto, layout, accumulate, drho);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(rho, drho, sizeof(double) * c.g.N, hipMemcpyDeviceToHost,
+                                  c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) rc = hip_fail(e, "density download", __FILE__, __LINE__);
+  }
+  if (part) (void)hipFree(part);
+  if (drho) (void)hipFree(drho);
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_energies(sdft_ctx* h, int nstate, const double* re, const double* im, double occ,
+                      double nsto, double* eke, double* enl) {
+  if (!h || nstate < 1 || !re || !im || !eke || !enl || !(nsto > 0)) {
+    set_error("bad energy arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  Block b;
+  int rc = block_alloc(c, nstate, 0, b);
+  if (!rc) rc = block_put(b, b.X0, re, im);
+  double ke = 0.0, nl = 0.0;
+  if (!rc) rc = energy_ke_device(c, b.X0, nstate, &ke);
+  if (!rc && c.nproj > 0) {
+    rc = launch_nl_project(c, 0, b.X0, b.nstate_pad);
+    if (!rc) rc = energy_nl_device(c, b.nstate_pad, nstate, &nl);
+  }
+  if (!rc) {
+    *eke = occ * 0.5 * ke / nsto;
+    *enl = occ * nl / nsto;
+  }
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_block_create(sdft_ctx* h, int nstate, int prec, sdft_block** out) {
+  if (!h || nstate < 1 || !out || (prec != 0 && prec != 1)) {
+    set_error("bad block arguments");
+    return SDFT_EINVAL;
+  }
+  sdft_block* bb = new sdft_block();
+  int rc = block_alloc(h->c, nstate, prec, bb->b);
+  if (rc) {
+    block_free(bb->b);
+    delete bb;
+    return rc;
+  }
+  *out = bb;
+  return 0;
+}
+
+int sdft_hip_block_destroy(sdft_block* bb) {
+  if (!bb) return 0;
+  if (bb->b.ctx) (void)hipStreamSynchronize(bb->b.ctx->stream);
+  block_free(bb->b);
+  delete bb;
+  return 0;
+}
+
+int sdft_hip_block_set(sdft_block* bb, const double* cre, const double* cim) {
+  if (!bb || !cre || !cim) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  Block& b = bb->b;
+  b.cur = 0;
+  int rc = block_put(b, b.X0, cre, cim);
+  if (rc) return rc;
+  const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * b.ctx->g.ncp;
+  SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+  return 0;
+}
+
+int sdft_hip_block_filter(sdft_block* bb, int L, const double* coeffs, double scale, double emean,
+                          double* moments) {
+  if (!bb || !coeffs) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  Block& b = bb->b;
+  // every call filters chi (the reference re-filters the same noise)
+  const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * b.ctx->g.ncp;
+  b.cur = 0;
+  SDFT_HIP_CHECK(hipMemcpyAsync(b.X0, b.chi, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+  return block_filter_impl(b, L, coeffs, scale, emean, moments);
+}
+
+int sdft_hip_block_get(sdft_block* bb, int which, double* re, double* im) {
+  if (!bb || !re || !im || (which != 0 && which != 1)) {
+    set_error("bad block_get arguments");
+    return SDFT_EINVAL;
+  }
+  Block& b = bb->b;
+  const double* src = which == 0 ? b.acc : (b.cur ? b.X1 : b.X0);
+  return block_fetch(b, src, re, im);
+}
+
+int sdft_hip_set_profiling(sdft_ctx* h, int on) {
+  if (!h) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  h->c.profiling = on != 0;
+  return 0;
+}
+
+int sdft_hip_kernel_times(sdft_ctx* h, int max, const char** names, double* ms, int* launches) {
+  if (!h) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  const int n = (int)c.prof_names.size();
+  for (int i = 0; i < n && i < max; ++i) {
+    if (names) names[i] = c.prof_names[i].c_str();
+    if (ms) ms[i] = c.prof_ms[i];
+    if (launches) launches[i] = c.prof_launch[i];
+  }
+  return n;
+}
+
+int sdft_hip_selftest_fft(int n, int nlines, int dir, int prec, const double* in, double* out) {
+  if (nlines < 1 || !in || !out || (dir != 1 && dir != -1) || (prec != 0 && prec != 1) ||
+      chunk_cols_for(n) == 0) {
+    set_error("bad selftest arguments");
+    return SDFT_EINVAL;
+  }
+  std::vector<double> tw64(2 * n);
+  std::vector<float> tw32(2 * n);
+  for (int m = 0; m < n; ++m) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * m / (long double)n;
+    tw64[2 * m] = (double)cosl(ang);
+    tw64[2 * m + 1] = (double)sinl(ang);
+    tw32[2 * m] = (float)cosl(ang);
+    tw32[2 * m + 1] = (float)sinl(ang);
+  }
+  const size_t bytes = sizeof(double) * 2 * (size_t)n * nlines;
+  double *din = nullptr, *dout = nullptr;
+  void* dtw = nullptr;
+  int rc = dalloc(&din, bytes);
+  if (!rc) rc = dalloc(&dout, bytes);
+  if (!rc) rc = dalloc(&dtw, sizeof(double) * 2 * n);
+  if (!rc) {
+    hipError_t e = hipMemcpy(din, in, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(dtw, prec == 0 ? (const void*)tw64.data() : (const void*)tw32.data(),
+                    prec == 0 ? sizeof(double) * 2 * n : sizeof(float) * 2 * n,
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "selftest upload", __FILE__, __LINE__);
+  }
+  if (!rc) rc = selftest_fft_dispatch(n, nlines, dir, prec, din, dout, dtw);
+  if (!rc) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "selftest run", __FILE__, __LINE__);
+  }
+  if (din) (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  if (dtw) (void)hipFree(dtw);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// sdft_cheby.hip -- FP64 scalar side of the Chebyshev path, on the device.
+//
+//  * expansion coefficients of f_FD or sqrt(f_FD) (calcChebyCoeff,
+//    stodft/coeff.c:1093-1223): c_k = (2/M) sum_j g(x_j) cos(2 pi jk/M),
+//    c_0 = (1/M) sum_j g(x_j), x_j = (dE/2) cos(2 pi j/M) + E_mean,
+//    M = roundFFT(32 L) (coeff.c:1556-1600).  The reference takes the real
+//    part of an M-point complex DFT; here the same cosine sums are evaluated
+//    directly, one workgroup per k.
+//  * electron count and chemical potential (calcNumElecCheby /
+//    calcChemPotCheby, stodft/calc-chempot-chebyshev.c:30-315,918-972):
+//    N(mu) = occ sum_{k<L} c_k(mu) m_k = (occ/M) sum_j f(x_j; mu) h_j with
+//    h_j = m_0 + 2 sum_{k=1}^{L-1} m_k cos(2 pi jk/M) computed once, so each
+//    N(mu) is one M-point reduction; the bracket / interpolation / bisection
+//    sequence of the reference runs in a single workgroup.
+//  * per-step moment partial reduction and the kinetic / nonlocal energy
+//    sums (stodft/calc-energy.c:133-191).
+#include <cmath>
+#include <vector>
+
+#include "sdft_internal.h"
+
+namespace sdft {
+
+int round_fft(int n_in);
+
+__device__ inline double fermi_dev(double x, double mu, double beta) {
+  return 1.0 / (1.0 + exp(beta * (x - mu)));
+}
+
+__global__ void k_costab(double* __restrict__ tab, int M) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) tab[j] = cospi(2.0 * (double)j / (double)M);
+}
+
+__global__ void k_cheby_fvals(const double* __restrict__ costab, int M, double dE, double em,
+                              double mu, double beta, int fun_flag, double* __restrict__ g) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const double x = dE * costab[j] + em;
+  double f = fermi_dev(x, mu, beta);
+  if (fun_flag == 2) f = sqrt(f);
+  g[j] = f;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) s += sh[w];
+  return s;
+}
+
+// one workgroup per k
+__global__ void k_cheby_project(const double* __restrict__ g, const double* __restrict__ costab,
+                                int M, double* __restrict__ c) {
+  __shared__ double sh[16];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    const long long idx = ((long long)j * k) % M;
+    s += g[j] * costab[idx];
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) c[k] = (k == 0 ? 1.0 : 2.0) * s / (double)M;
+}
+
+// h_j = m_0 + 2 sum_{k=1}^{L-1} m_k cos(2 pi jk/M)
+__global__ void k_cheby_h(const double* __restrict__ m, int L, const double* __restrict__ costab,
+                          int M, double* __restrict__ h) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  double s = 0.0;
+  int idx = 0;
+  for (int k = 1; k < L; ++k) {
+    idx += j;
+    if (idx >= M) idx -= M;
+    s += m[k] * costab[idx];
+  }
+  h[j] = m[0] + 2.0 * s;
+}
+
+__device__ double num_elec_dev(const double* costab, const double* h, int M, double dE,
+                               double em, double mu, double beta, double occ, double* sh) {
+  double s = 0.0;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    const double x = dE * costab[j] + em;
+    s += fermi_dev(x, mu, beta) * h[j];
+  }
+  s = block_sum(s, sh);
+  return occ * s / (double)M;
+}
+
+// Bracket + interpolation + bisection of calcChemPotCheby, one workgroup.
+__global__ void k_chem_pot(const double* __restrict__ costab, const double* __restrict__ h, int M,
+                           double dE, double em, double beta, double occ, double ne_true,
+                           double mu_init, double gap, double tol, int max_iter,
+                           double* __restrict__ out) {
+  __shared__ double sh[16];
+  double lo = mu_init - 0.5 * gap, hi = mu_init + 0.5 * gap;
+  double nlo = num_elec_dev(costab, h, M, dE, em, lo, beta, occ, sh);
+  double nhi = num_elec_dev(costab, h, M, dE, em, hi, beta, occ, sh);
+  int it = 0;
+  while (nhi < ne_true && it < max_iter) {
+    lo = hi;
+    hi += gap;
+    nlo = num_elec_dev(costab, h, M, dE, em, lo, beta, occ, sh);
+    nhi = num_elec_dev(costab, h, M, dE, em, hi, beta, occ, sh);
+    ++it;
+  }
+  while (nlo > ne_true && it < max_iter) {
+    hi = lo;
+    lo -= gap;
+    nlo = num_elec_dev(costab, h, M, dE, em, lo, beta, occ, sh);
+    nhi = num_elec_dev(costab, h, M, dE, em, hi, beta, occ, sh);
+    ++it;
+  }
+  double mu = (ne_true - nlo) * (hi - lo) / (nhi - nlo) + lo;
+  double nn = num_elec_dev(costab, h, M, dE, em, mu, beta, occ, sh);
+  int ib = 0;
+  while (fabs(nn - ne_true) > tol && ib < max_iter) {
+    if (nn > ne_true) {
+      hi = mu;
+      nhi = nn;
+    }
+    if (nn < ne_true) {
+      lo = mu;
+      nlo = nn;
+    }
+    mu = 0.5 * (lo + hi);
+    nn = num_elec_dev(costab, h, M, dE, em, mu, beta, occ, sh);
+    ++ib;
+  }
+  if (threadIdx.x == 0) {
+    out[0] = mu;
+    out[1] = nn;
+    out[2] = (double)(it + ib);
+  }
+}
+
+// out[step*3 + i] = sum_w part[(step*nwg + w)*3 + i]
+__global__ void k_mom_reduce(const double* __restrict__ part, int nwg, double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int step = blockIdx.x;
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += blockDim.x) s += part[((size_t)step * nwg + w) * 3 + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[step * 3 + i] = s;
+  }
+}
+
+// per state: sum_{j < ncoef-1} 2 ak2 (re^2 + im^2)
+__global__ void k_eke(const double* __restrict__ X, int ldx, int ncp, int ncoef,
+                      const double* __restrict__ ak2, double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int s = blockIdx.x;
+  const double* x = X + (size_t)s * ldx;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < ncoef - 1; j += blockDim.x)
+    acc += 2.0 * ak2[j] * (x[j] * x[j] + x[ncp + j] * x[ncp + j]);
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[s] = acc;
+}
+
+// sum_{p, s < nstate} E_p Z[p][s]^2
+__global__ void k_enl(const double* __restrict__ Z, const double* __restrict__ E, int P, int ns,
+                      int nstate, double* __restrict__ out) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < P * ns; i += blockDim.x) {
+    const int p = i / ns, s = i - p * ns;
+    if (s < nstate) acc += E[p] * Z[i] * Z[i];
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
+// ==========================================================================
+
+static int scratch(Ctx& c, size_t need) {
+  if (c.red_bytes >= need) return 0;
+  if (c.red) SDFT_HIP_CHECK(hipFree(c.red));
+  c.red = nullptr;
+  c.red_bytes = 0;
+  if (hipMalloc(&c.red, need) != hipSuccess) {
+    set_error("device allocation failed (reduction scratch)");
+    return -3;
+  }
+  c.red_bytes = need;
+  return 0;
+}
+
+int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,
+                        int fun_flag, double* coeffs_host) {
+  const int M = round_fft(32 * L);
+  int rc = scratch(c, sizeof(double) * ((size_t)2 * M + L));
+  if (rc) return rc;
+  double* costab = c.red;
+  double* g = costab + M;
+  double* cdev = g + M;
+  const double dE = (emax - emin) * 0.5, em = 0.5 * (emax + emin);
+  const int nb = (M + 255) / 256;
+  hipLaunchKernelGGL(k_costab, dim3(nb), dim3(256), 0, c.stream, costab, M);
+  hipLaunchKernelGGL(k_cheby_fvals, dim3(nb), dim3(256), 0, c.stream, costab, M, dE, em, mu, beta,
+                     fun_flag, g);
+  hipLaunchKernelGGL(k_cheby_project, dim3(L), dim3(256), 0, c.stream, g, costab, M, cdev);
+  SDFT_HIP_CHECK(hipGetLastError());
+  SDFT_HIP_CHECK(hipMemcpyAsync(coeffs_host, cdev, sizeof(double) * L, hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta, const double* moments,
+                    double nsto, double ne_true, double mu_init, double gap, double occ,
+                    double tol, double* mu_out, double* ne_out) {
+  const int M = round_fft(32 * L);
+  int rc = scratch(c, sizeof(double) * ((size_t)2 * M + L + 4));
+  if (rc) return rc;
+  double* costab = c.red;
+  double* h = costab + M;
+  double* m = h + M;
+  double* out = m + L;
+  std::vector<double> mn(L);
+  for (int k = 0; k < L; ++k) mn[k] = moments[k] / nsto;
+  SDFT_HIP_CHECK(hipMemcpyAsync(m, mn.data(), sizeof(double) * L, hipMemcpyHostToDevice, c.stream));
+  const int nb = (M + 255) / 256;
+  hipLaunchKernelGGL(k_costab, dim3(nb), dim3(256), 0, c.stream, costab, M);
+  hipLaunchKernelGGL(k_cheby_h, dim3(nb), dim3(256), 0, c.stream, m, L, costab, M, h);
+  const double dE = (emax - emin) * 0.5, em = 0.5 * (emax + emin);
+  hipLaunchKernelGGL(k_chem_pot, dim3(1), dim3(1024), 0, c.stream, costab, h, M, dE, em, beta, occ,
+                     ne_true, mu_init, gap, tol, 400, out);
+  SDFT_HIP_CHECK(hipGetLastError());
+  double res[3];
+  SDFT_HIP_CHECK(hipMemcpyAsync(res, out, sizeof(res), hipMemcpyDeviceToHost, c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  *mu_out = res[0];
+  *ne_out = res[1];
+  return 0;
+}
+
+int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3) {
+  int rc = scratch(c, sizeof(double) * (size_t)3 * nsteps);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mom_reduce, dim3(nsteps), dim3(256), 0, c.stream, mom_part, nwg, c.red);
+  SDFT_HIP_CHECK(hipGetLastError());
+  SDFT_HIP_CHECK(hipMemcpyAsync(out3, c.red, sizeof(double) * 3 * nsteps, hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int energy_ke_device(Ctx& c, const double* X, int nstate, double* out) {
+  int rc = scratch(c, sizeof(double) * (size_t)nstate);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_eke, dim3(nstate), dim3(256), 0, c.stream, X, 2 * c.g.ncp, c.g.ncp,
+                     c.g.ncoef, c.t.ak2, c.red);
+  SDFT_HIP_CHECK(hipGetLastError());
+  std::vector<double> h(nstate);
+  SDFT_HIP_CHECK(hipMemcpyAsync(h.data(), c.red, sizeof(double) * nstate, hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (int i = 0; i < nstate; ++i) s += h[i];
+  *out = s;
+  return 0;
+}
+
+int energy_nl_device(Ctx& c, int ns, int nstate, double* out) {
+  int rc = scratch(c, sizeof(double));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_enl, dim3(1), dim3(1024), 0, c.stream, c.Z, c.E, c.nproj, ns, nstate, c.red);
+  SDFT_HIP_CHECK(hipGetLastError());
+  SDFT_HIP_CHECK(hipMemcpyAsync(out, c.red, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+}  // namespace sdft

@@ -1,0 +1,156 @@
+// sdft_internal.h -- device context and kernel launchers shared by the HIP
+// translation units of libsdft_hip.
+//
+// Internal data layout in HBM (all device arrays owned by the context/block):
+//  * coefficient order: the caller's half-sphere entries are permuted so that
+//    coefficients whose +G lie in the same column chunk are contiguous
+//    (columns run along the a axis and are indexed by (kb, kc)); G=0 stays
+//    last.  The permutation is applied on upload and undone on download.
+//  * an orbital block X holds nstate_pad (even) states, each 2*ncp doubles:
+//    real parts [0, ncoef), zero pad to ncp, imaginary parts [ncp, ncp+ncoef),
+//    zero pad.  As a matrix it is (2 ncp) x nstate, column-major: the
+//    operand of the Kleinman-Bylander GEMMs.
+//  * W: partially transformed pair grids, [pair][ra][ncolp] complex (T),
+//    i.e. for every a-plane the values of all sphere columns (kb, kc).  It is
+//    the only intermediate of the FFT sandwich; no full n^3 grid is stored.
+//  * v_KS in FFTW order [ra][rb][rc], FP64 and FP32 copies.
+//  * KB table A[p][2 ncp]: A = w Re Q | -w Im Q with w = 2 (G != 0), 1 (G=0),
+//    so that Z = A X and the NL force is -2 A^T diag(E) Z.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace sdft {
+
+constexpr int kThreads = 256;       // threads per workgroup of the FFT kernels
+
+// Error reporting (thread-local message, negative status codes).
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define SDFT_HIP_CHECK(expr)                                               \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) return ::sdft::hip_fail(e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+struct Geometry {
+  int n = 0;          // cubic grid extent
+  long long N = 0;    // n^3
+  int ncoef = 0;      // coefficients incl. G=0
+  int ncp = 0;        // padded coefficient stride (multiple of 64)
+  int K = 0;          // max |k| over the sphere (all axes)
+  int M = 0;          // rows of the (kb) plane buffer: 2K+1
+  int ncol = 0;       // sphere columns (kb, kc)
+  int ncolp = 0;      // padded column stride of W
+  int nchunk = 0;     // column chunks (column-pair aligned)
+  int chunk_max = 0;  // max columns in a chunk
+  // host copies
+  std::vector<int> perm;        // internal j -> caller index
+  std::vector<int> iperm;       // caller index -> internal j
+  std::vector<double> ak2;      // internal order, ncp
+};
+
+struct DeviceTables {
+  int* col_pos = nullptr;     // [ncol] LDS offset (kb+K)*(n+1) + kc mod n in the plane buffer
+  int* ent_off = nullptr;     // [ncol+1] entry offsets per column
+  int* ent_j = nullptr;       // [nent] internal coefficient index
+  int* ent_b = nullptr;       // [nent] (d << 10) | (flag << 8) | ia ; flag 0 +G, 1 -G, 2 G=0
+  int* chunk_col = nullptr;   // [nchunk+1]
+  int* chunk_coef = nullptr;  // [nchunk+1]
+  int* coef_plus = nullptr;   // [ncoef] (d << 8) | ia of +G (internal order)
+  int* coef_minus = nullptr;  // [ncoef] (d << 8) | ia of -G
+  double* ak2 = nullptr;      // [ncp] internal order
+  void* tw64 = nullptr;       // [n] double2 exp(+2 pi i m/n)
+  void* tw32 = nullptr;       // [n] float2
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geometry g;
+  DeviceTables t;
+  double* vks64 = nullptr;   // [N]
+  float* vks32 = nullptr;    // [N]
+  bool have_vks = false;
+  int nproj = 0;
+  double* A64 = nullptr;     // [nproj][2 ncp]
+  float* A32 = nullptr;      // [nproj][2 ncp]
+  double* E = nullptr;       // [nproj]
+  // scratch (grown on demand)
+  void* W = nullptr;
+  size_t W_bytes = 0;
+  double* zpart = nullptr;   // split-K partials of Z
+  size_t zpart_bytes = 0;
+  double* Y = nullptr;       // [nproj][nstate_pad] = -2 E Z
+  double* Z = nullptr;       // [nproj][nstate_pad]
+  size_t Y_bytes = 0;
+  double* red = nullptr;     // reduction scratch
+  size_t red_bytes = 0;
+  // profiling
+  bool profiling = false;
+  std::vector<std::string> prof_names;
+  std::vector<double> prof_ms;
+  std::vector<int> prof_launch;
+};
+
+struct Block {
+  Ctx* ctx = nullptr;
+  int nstate = 0;
+  int nstate_pad = 0;
+  int prec = 0;
+  double* X0 = nullptr;    // T ping
+  double* X1 = nullptr;    // T pong
+  double* acc = nullptr;   // filtered accumulator
+  double* chi = nullptr;   // copy of the input noise (moments)
+  double* fnl = nullptr;   // NL raw force of the current step
+  double* mom_part = nullptr;  // per-step moment partials
+  size_t mom_part_bytes = 0;
+  int cur = 0;             // which of X0/X1 holds the current T
+};
+
+// ---- recurrence parameters passed to the contract kernel ----------------
+struct StepParams {
+  double s1, s2, p;   // -scale/4, -scale/2, -scale*emean (x2 after step 1)
+  double ck;          // coefficient of the produced T_k
+  double c0;          // coefficient of T_0 (step 1 initialises acc)
+  int first;          // 1 on the first step (T_1 = s1 f + p T_0)
+  int step;           // step index (moment partial row)
+};
+
+enum ContractMode { kModeForce = 0, kModeFilter = 1, kModeMoments = 2 };
+
+// ---- launchers (sdft_kernels.hip) ----------------------------------------
+size_t w_bytes(const Ctx& c, int npairs, int prec);
+int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W);
+int launch_plane(Ctx& c, int prec, void* W, int npairs);
+int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
+                    const double* Xcur, double* Xout, double* acc, const double* fnl,
+                    const double* chi, const StepParams& sp, double* mom_part);
+int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups);
+int launch_rho_finish(Ctx& c, const double* rho_part, int ngroups, double scale,
+                      int layout, int accumulate, double* rho);
+int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in, double* out,
+                          const void* tw);
+int chunk_cols_for(int n);
+int plane_rows_max_for(int n);
+int round_fft(int n_in);
+
+// ---- GEMM (sdft_gemm.hip) -----------------------------------------------
+int launch_nl_project(Ctx& c, int prec, const double* X, int nstate_pad);
+int launch_nl_force(Ctx& c, int prec, int nstate_pad, double* fnl);
+
+// ---- Chebyshev scalars (sdft_cheby.hip) ---------------------------------
+int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,
+                        int fun_flag, double* coeffs_host);
+int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta,
+                    const double* moments, double nsto, double ne_true, double mu_init,
+                    double gap, double occ, double tol, double* mu_out, double* ne_out);
+int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3);
+int energy_ke_device(Ctx& c, const double* X, int nstate, double* out);
+int energy_nl_device(Ctx& c, int nstate_pad, int nstate, double* out);
+
+}  // namespace sdft

@@ -1,0 +1,567 @@
+// sdft_kernels.hip -- the G->r->G sandwich of H.psi for Gamma-packed orbital
+// pairs, as three hand-written gfx950 kernels:
+//
+//   k_expand   : per (pair, column chunk): Gamma pair pack of the sphere
+//                coefficients (dble_pack_coef_fftw3d_filter,
+//                mathlib/fft_package.c:3195-3254) straight into LDS columns
+//                along a, inverse FFT along a, write W[pair][ra][column].
+//   k_plane    : per (pair, a-plane): columns -> (kb, c) rows, inverse FFT
+//                along c, then in sub-chunks of rc: inverse FFT along b,
+//                z *= v_KS (cp_vpsi_threads, energy/cp/cp_energy_ee_rho.c:
+//                2817-2845), forward FFT along b; forward FFT along c; back
+//                to columns.  The real-space pair grid exists only in LDS.
+//   k_contract : per (pair, column-pair chunk): forward FFT along a, then per
+//                stored coefficient the unpack of F(G), F(-G)
+//                (dble_upack_coef_sum_fftw3d_filter, fft_package.c:3513-3599),
+//                the NL force from the KB GEMM, the kinetic term
+//                (cp-energy-ee-rho-stodft.c:305-320), and -- in filter mode --
+//                the Chebyshev recurrence of normHCheby (stodft/normh.c:
+//                259-338) and the accumulation of filter.c:596-612.
+//
+// Only the sphere's columns (kb, kc) are transformed along a and only the
+// sphere's kb rows along c, so an a-plane holds M x n values between the
+// first and the last kernel instead of n^2, and no n^3 grid touches HBM.
+#include "fft_lds.h"
+#include "sdft_internal.h"
+
+namespace sdft {
+
+// ---- per-size configuration ----------------------------------------------
+constexpr int largest_divisor_le(int n, int cap) {
+  for (int d = cap; d >= 1; --d)
+    if (n % d == 0) return d;
+  return 1;
+}
+
+template <int N>
+struct Cfg {
+  static constexpr int ML = fft_max_lines(N, kThreads);  // lines per FFT call
+  static constexpr int CH = ML < 32 ? ML : 32;           // columns per chunk
+  static constexpr int CB = largest_divisor_le(N, ML < 16 ? ML : 16);
+  static constexpr int LD = N + 1;                       // padded LDS line stride
+};
+
+int chunk_cols_for(int n) {
+  switch (n) {
+    case 24: return Cfg<24>::CH;
+    case 32: return Cfg<32>::CH;
+    case 36: return Cfg<36>::CH;
+    case 48: return Cfg<48>::CH;
+    case 64: return Cfg<64>::CH;
+    case 96: return Cfg<96>::CH;
+    case 128: return Cfg<128>::CH;
+    case 192: return Cfg<192>::CH;
+    default: return 0;
+  }
+}
+
+template <int N>
+constexpr int plane_rows_max() { return (3 * N) / 8 + 1; }
+
+int plane_rows_max_for(int n) { return (3 * n) / 8 + 1; }
+
+template <typename T, int N>
+size_t plane_lds_bytes(int M) {
+  return sizeof(Cx<T>) * ((size_t)M * Cfg<N>::LD + (size_t)Cfg<N>::CB * Cfg<N>::LD + N);
+}
+
+__device__ inline int wrapi(int k, int n) { return k < 0 ? k + n : k; }
+
+// ---- block reduction of 3 doubles ----------------------------------------
+__device__ inline void block_sum3(double* v, double* sh, int tid) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[i] = x;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) {
+    sh[wave * 3 + 0] = v[0];
+    sh[wave * 3 + 1] = v[1];
+    sh[wave * 3 + 2] = v[2];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      a += sh[w * 3 + 0];
+      b += sh[w * 3 + 1];
+      c += sh[w * 3 + 2];
+    }
+    v[0] = a;
+    v[1] = b;
+    v[2] = c;
+  }
+}
+
+// ---- K1: pack + inverse FFT along a --------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_expand(
+    const double* __restrict__ X, int ldx, int ncp, const int* __restrict__ chunk_col,
+    const int* __restrict__ ent_off, const int* __restrict__ ent_j, const int* __restrict__ ent_b,
+    const Cx<T>* __restrict__ twg, Cx<T>* __restrict__ W, int ncolp) {
+  constexpr int CH = Cfg<N>::CH, LD = Cfg<N>::LD, ML = Cfg<N>::ML;
+  __shared__ Cx<T> buf[CH * LD];
+  __shared__ Cx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int chunk = blockIdx.x;
+  const int q = blockIdx.y;
+  const int c0 = chunk_col[chunk];
+  const int nc = chunk_col[chunk + 1] - c0;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  for (int i = tid; i < nc * LD; i += kThreads) buf[i] = cx(T(0), T(0));
+  __syncthreads();
+  const double* x1 = X + (size_t)(2 * q) * ldx;
+  const double* x2 = x1 + ldx;
+  const int e1 = ent_off[c0 + nc];
+  for (int e = ent_off[c0] + tid; e < e1; e += kThreads) {
+    const int j = ent_j[e];
+    const int b = ent_b[e];
+    const int d = b >> 10, flag = (b >> 8) & 3, ia = b & 255;
+    const double ar = x1[j], ai = x1[ncp + j], br = x2[j], bi = x2[ncp + j];
+    double zr, zi;
+    if (flag == 0) {        // +G: c1 + i c2
+      zr = ar - bi;
+      zi = ai + br;
+    } else if (flag == 1) { // -G: c1* + i c2*
+      zr = ar + bi;
+      zi = -ai + br;
+    } else {                // G = 0: c1re + i c2re
+      zr = ar;
+      zi = br;
+    }
+    buf[(d - c0) * LD + ia] = cx(T(zr), T(zi));
+  }
+  __syncthreads();
+  fft_lines<T, N, +1, ML, kThreads>(buf, nc, LD, tw, tid);
+  Cx<T>* Wq = W + (size_t)q * N * ncolp;
+  for (int e = tid; e < nc * N; e += kThreads) {
+    const int ra = e / nc;
+    const int dl = e - ra * nc;
+    Wq[(size_t)ra * ncolp + c0 + dl] = buf[dl * LD + ra];
+  }
+}
+
+// ---- K2: one a-plane: c and b transforms around v.psi --------------------
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int ncolp, int ncol,
+                                                    const int* __restrict__ col_pos, int M,
+                                                    const T* __restrict__ v,
+                                                    const Cx<T>* __restrict__ twg) {
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = Cfg<N>::CB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* P = reinterpret_cast<Cx<T>*>(smem_raw);
+  Cx<T>* Q = P + M * LD;
+  Cx<T>* tw = Q + CB * LD;
+  const int tid = threadIdx.x;
+  const int ra = blockIdx.x;
+  const int q = blockIdx.y;
+  const int K = (M - 1) / 2;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  for (int i = tid; i < M * LD; i += kThreads) P[i] = cx(T(0), T(0));
+  __syncthreads();
+  Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+  for (int d = tid; d < ncol; d += kThreads) P[col_pos[d]] = Wp[d];
+  __syncthreads();
+  fft_lines_batched<T, N, +1, ML, kThreads>(P, M, LD, tw, tid);  // along c
+  const T* vplane = v + (size_t)ra * N * N;
+  for (int rc0 = 0; rc0 < N; rc0 += CB) {
+    for (int i = tid; i < CB * LD; i += kThreads) Q[i] = cx(T(0), T(0));
+    __syncthreads();
+    for (int e = tid; e < M * CB; e += kThreads) {
+      const int r = e / CB, l = e - r * CB;
+      Q[l * LD + wrapi(r - K, N)] = P[r * LD + rc0 + l];
+    }
+    __syncthreads();
+    fft_lines<T, N, +1, ML, kThreads>(Q, CB, LD, tw, tid);  // along b: psi(ra, rb, rc)
+    for (int e = tid; e < CB * N; e += kThreads) {
+      const int rb = e / CB, l = e - rb * CB;
+      const T vv = vplane[rb * N + rc0 + l];
+      Cx<T> z = Q[l * LD + rb];
+      Q[l * LD + rb] = cx(z.x * vv, z.y * vv);
+    }
+    __syncthreads();
+    fft_lines<T, N, -1, ML, kThreads>(Q, CB, LD, tw, tid);  // back along b
+    for (int e = tid; e < M * CB; e += kThreads) {
+      const int r = e / CB, l = e - r * CB;
+      P[r * LD + rc0 + l] = Q[l * LD + wrapi(r - K, N)];
+    }
+    __syncthreads();
+  }
+  fft_lines_batched<T, N, -1, ML, kThreads>(P, M, LD, tw, tid);  // back along c
+  for (int d = tid; d < ncol; d += kThreads) Wp[d] = P[col_pos[d]];
+}
+
+// ---- K3: forward FFT along a + unpack + force / recurrence ---------------
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_contract(
+    const Cx<T>* __restrict__ W, int ncolp, const int* __restrict__ chunk_col,
+    const int* __restrict__ chunk_coef, const int* __restrict__ coef_plus,
+    const int* __restrict__ coef_minus, const double* __restrict__ ak2,
+    const Cx<T>* __restrict__ twg, const double* __restrict__ Xcur, double* __restrict__ Xout,
+    double* __restrict__ acc, const double* __restrict__ fnl, const double* __restrict__ chi,
+    int ldx, int ncp, int ncoef, double invN, int mode, StepParams sp,
+    double* __restrict__ mom_part) {
+  constexpr int CH = Cfg<N>::CH, LD = Cfg<N>::LD, ML = Cfg<N>::ML;
+  __shared__ Cx<T> buf[CH * LD];
+  __shared__ Cx<T> tw[N];
+  __shared__ double red[3 * (kThreads / 64)];
+  const int tid = threadIdx.x;
+  const int chunk = blockIdx.x;
+  const int q = blockIdx.y;
+  const int c0 = chunk_col[chunk];
+  const int nc = chunk_col[chunk + 1] - c0;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  const Cx<T>* Wq = W + (size_t)q * N * ncolp;
+  for (int e = tid; e < nc * N; e += kThreads) {
+    const int ra = e / nc;
+    const int dl = e - ra * nc;
+    buf[dl * LD + ra] = Wq[(size_t)ra * ncolp + c0 + dl];
+  }
+  __syncthreads();
+  fft_lines<T, N, -1, ML, kThreads>(buf, nc, LD, tw, tid);
+
+  double dots[3] = {0.0, 0.0, 0.0};
+  const int j0 = chunk_coef[chunk], j1 = chunk_coef[chunk + 1];
+  for (int j = j0 + tid; j < j1; j += kThreads) {
+    const int pp = coef_plus[j], pm = coef_minus[j];
+    const Cx<T> t = buf[((pp >> 8) - c0) * LD + (pp & 255)];
+    const Cx<T> tc = buf[((pm >> 8) - c0) * LD + (pm & 255)];
+    const double tr = double(t.x) * invN, ti = double(t.y) * invN;
+    const double tcr = double(tc.x) * invN, tci = double(tc.y) * invN;
+    const bool g0 = (j == ncoef - 1);
+    double f[2][2];
+    if (!g0) {
+      f[0][0] = -2.0 * (tr + tcr);
+      f[0][1] = -2.0 * (ti - tci);
+      f[1][0] = -2.0 * (ti + tci);
+      f[1][1] = -2.0 * (-tr + tcr);
+    } else {
+      f[0][0] = -2.0 * tr;
+      f[0][1] = 0.0;
+      f[1][0] = -2.0 * ti;
+      f[1][1] = 0.0;
+    }
+    const double k2 = ak2[j];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      const size_t base = (size_t)(2 * q + o) * ldx;
+      const double cr = Xcur[base + j], ci = Xcur[base + ncp + j];
+      double fr = f[o][0], fi = f[o][1];
+      if (fnl) {
+        fr += fnl[base + j];
+        fi += fnl[base + ncp + j];
+      }
+      if (!g0) {
+        fr -= 2.0 * k2 * cr;
+        fi -= 2.0 * k2 * ci;
+      } else {
+        fi = 0.0;
+      }
+      if (mode == kModeForce) {
+        Xout[base + j] = fr;
+        Xout[base + ncp + j] = fi;
+        continue;
+      }
+      double nr, ni;
+      if (sp.first) {
+        nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
+        ni = g0 ? fi : sp.s1 * fi + sp.p * ci;
+      } else {
+        const double pr = Xout[base + j], pi = Xout[base + ncp + j];
+        nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr - pr;
+        ni = g0 ? fi : sp.s1 * fi + sp.p * ci - pi;
+      }
+      Xout[base + j] = nr;
+      Xout[base + ncp + j] = ni;
+      if (mode == kModeFilter) {
+        if (sp.first) {
+          acc[base + j] = sp.c0 * cr + sp.ck * nr;
+          acc[base + ncp + j] = sp.c0 * ci + sp.ck * ni;
+        } else {
+          acc[base + j] += sp.ck * nr;
+          acc[base + ncp + j] += sp.ck * ni;
+        }
+      }
+      if (mom_part) {
+        // Gamma dot as coded at calc-chempot-chebyshev.c:725-729:
+        // 2 sum re*re + 2 sum im*im - re(0)*re(0)
+        const double wr = g0 ? 1.0 : 2.0;
+        const double xr = chi[base + j], xi = chi[base + ncp + j];
+        dots[0] += wr * xr * nr + 2.0 * xi * ni;
+        if (sp.first)
+          dots[1] += wr * cr * cr + 2.0 * ci * ci;
+        else
+          dots[1] += wr * nr * cr + 2.0 * ni * ci;
+        dots[2] += wr * nr * nr + 2.0 * ni * ni;
+      }
+    }
+  }
+  if (mom_part) {
+    block_sum3(dots, red, tid);
+    if (tid == 0) {
+      const size_t row = (size_t)sp.step * gridDim.x * gridDim.y + (size_t)q * gridDim.x + chunk;
+      mom_part[row * 3 + 0] = dots[0];
+      mom_part[row * 3 + 1] = dots[1];
+      mom_part[row * 3 + 2] = dots[2];
+    }
+  }
+}
+
+// ---- density: |psi1|^2 + |psi2|^2 summed over the pairs of a group -------
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ W, int ncolp,
+                                                      int ncol, const int* __restrict__ col_pos,
+                                                      int M, const Cx<T>* __restrict__ twg,
+                                                      double* __restrict__ rho_part, int npairs,
+                                                      int ngroups) {
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = Cfg<N>::CB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* P = reinterpret_cast<Cx<T>*>(smem_raw);
+  Cx<T>* Q = P + M * LD;
+  Cx<T>* tw = Q + CB * LD;
+  const int tid = threadIdx.x;
+  const int ra = blockIdx.x;
+  const int g = blockIdx.y;
+  const int K = (M - 1) / 2;
+  double* rp = rho_part + ((size_t)g * N + ra) * N * N;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  bool first = true;
+  for (int q = g; q < npairs; q += ngroups) {
+    for (int i = tid; i < M * LD; i += kThreads) P[i] = cx(T(0), T(0));
+    __syncthreads();
+    const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+    for (int d = tid; d < ncol; d += kThreads) P[col_pos[d]] = Wp[d];
+    __syncthreads();
+    fft_lines_batched<T, N, +1, ML, kThreads>(P, M, LD, tw, tid);
+    for (int rc0 = 0; rc0 < N; rc0 += CB) {
+      for (int i = tid; i < CB * LD; i += kThreads) Q[i] = cx(T(0), T(0));
+      __syncthreads();
+      for (int e = tid; e < M * CB; e += kThreads) {
+        const int r = e / CB, l = e - r * CB;
+        Q[l * LD + wrapi(r - K, N)] = P[r * LD + rc0 + l];
+      }
+      __syncthreads();
+      fft_lines<T, N, +1, ML, kThreads>(Q, CB, LD, tw, tid);
+      for (int e = tid; e < CB * N; e += kThreads) {
+        const int rb = e / CB, l = e - rb * CB;
+        const Cx<T> z = Q[l * LD + rb];
+        const double s = double(z.x) * double(z.x) + double(z.y) * double(z.y);
+        const size_t idx = (size_t)rb * N + rc0 + l;
+        rp[idx] = first ? s : rp[idx] + s;
+      }
+      __syncthreads();
+    }
+    first = false;
+  }
+  if (first) {
+    for (int e = tid; e < N * N; e += kThreads) rp[e] = 0.0;
+  }
+}
+
+// rho = scale * sum_g rho_part[g] (+ rho), FFTW or generic (a fastest) order
+__global__ void k_rho_finish(const double* __restrict__ part, int ngroups, long long Ntot, int n,
+                             double scale, int layout, int accumulate, double* __restrict__ rho) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Ntot) return;
+  double s = 0.0;
+  for (int g = 0; g < ngroups; ++g) s += part[(size_t)g * Ntot + i];
+  s *= scale;
+  long long o = i;
+  if (layout == 1) {
+    const long long ia = i / ((long long)n * n);
+    const long long rem = i - ia * (long long)n * n;
+    const long long ib = rem / n, ic = rem - ib * n;
+    o = (ic * n + ib) * n + ia;
+  }
+  rho[o] = accumulate ? rho[o] + s : s;
+}
+
+// ---- FFT self-test: lines in global memory ------------------------------
+template <typename T, int N, int DIR>
+__global__ __launch_bounds__(kThreads) void k_fft_test(const double* __restrict__ in,
+                                                       double* __restrict__ out, int nlines,
+                                                       const Cx<T>* __restrict__ twg) {
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML;
+  __shared__ Cx<T> buf[ML * LD];
+  __shared__ Cx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int l0 = blockIdx.x * ML;
+  const int nl = nlines - l0 < ML ? nlines - l0 : ML;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  for (int e = tid; e < nl * N; e += kThreads) {
+    const int l = e / N, k = e - l * N;
+    const size_t g = ((size_t)(l0 + l) * N + k) * 2;
+    buf[l * LD + k] = cx(T(in[g]), T(in[g + 1]));
+  }
+  __syncthreads();
+  fft_lines<T, N, DIR, ML, kThreads>(buf, nl, LD, tw, tid);
+  for (int e = tid; e < nl * N; e += kThreads) {
+    const int l = e / N, k = e - l * N;
+    const size_t g = ((size_t)(l0 + l) * N + k) * 2;
+    out[g] = double(buf[l * LD + k].x);
+    out[g + 1] = double(buf[l * LD + k].y);
+  }
+}
+
+// ==========================================================================
+// launchers
+// ==========================================================================
+
+size_t w_bytes(const Ctx& c, int npairs, int prec) {
+  const size_t elem = prec == 0 ? sizeof(Cx<double>) : sizeof(Cx<float>);
+  return (size_t)npairs * c.g.n * c.g.ncolp * elem;
+}
+
+constexpr int SDFT_EINVAL_CODE = -1;
+
+#define SDFT_SIZE_SWITCH(n, CALL)          \
+  switch (n) {                             \
+    case 24: { constexpr int NN = 24; CALL; } break;   \
+    case 32: { constexpr int NN = 32; CALL; } break;   \
+    case 36: { constexpr int NN = 36; CALL; } break;   \
+    case 48: { constexpr int NN = 48; CALL; } break;   \
+    case 64: { constexpr int NN = 64; CALL; } break;   \
+    case 96: { constexpr int NN = 96; CALL; } break;   \
+    case 128: { constexpr int NN = 128; CALL; } break; \
+    case 192: { constexpr int NN = 192; CALL; } break; \
+    default: set_error("unsupported grid size"); return SDFT_EINVAL_CODE; \
+  }
+
+template <typename T, int N>
+static int expand_impl(Ctx& c, const double* X, int npairs, void* W, const void* tw) {
+  const Geometry& g = c.g;
+  dim3 grid(g.nchunk, npairs);
+  hipLaunchKernelGGL((k_expand<T, N>), grid, dim3(kThreads), 0, c.stream, X, 2 * g.ncp, g.ncp,
+                     c.t.chunk_col, c.t.ent_off, c.t.ent_j, c.t.ent_b,
+                     reinterpret_cast<const Cx<T>*>(tw), reinterpret_cast<Cx<T>*>(W), g.ncolp);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(c.g.n, return (expand_impl<double, NN>(c, X, npairs, W, c.t.tw64)));
+  } else {
+    SDFT_SIZE_SWITCH(c.g.n, return (expand_impl<float, NN>(c, X, npairs, W, c.t.tw32)));
+  }
+  return 0;
+}
+
+template <typename T, int N>
+static int plane_impl(Ctx& c, void* W, int npairs, const T* v, const void* tw) {
+  const Geometry& g = c.g;
+  const size_t lds = plane_lds_bytes<T, N>(g.M);
+  if (lds > 160 * 1024) {
+    set_error("plane buffer exceeds LDS for this grid in this precision");
+    return SDFT_EINVAL_CODE;
+  }
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid(N, npairs);
+  hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream,
+                     reinterpret_cast<Cx<T>*>(W), g.ncolp, g.ncol, c.t.col_pos, g.M, v,
+                     reinterpret_cast<const Cx<T>*>(tw));
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_plane(Ctx& c, int prec, void* W, int npairs) {
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<double, NN>(c, W, npairs, c.vks64, c.t.tw64)));
+  } else {
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<float, NN>(c, W, npairs, c.vks32, c.t.tw32)));
+  }
+  return 0;
+}
+
+template <typename T, int N>
+static int contract_impl(Ctx& c, int mode, const void* W, int npairs, const double* Xcur,
+                         double* Xout, double* acc, const double* fnl, const double* chi,
+                         const StepParams& sp, double* mom_part, const void* tw) {
+  const Geometry& g = c.g;
+  dim3 grid(g.nchunk, npairs);
+  const double invN = 1.0 / (double)g.N;
+  hipLaunchKernelGGL((k_contract<T, N>), grid, dim3(kThreads), 0, c.stream,
+                     reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.chunk_col, c.t.chunk_coef,
+                     c.t.coef_plus, c.t.coef_minus, c.t.ak2, reinterpret_cast<const Cx<T>*>(tw),
+                     Xcur, Xout, acc, fnl, chi, 2 * g.ncp, g.ncp, g.ncoef, invN, mode, sp,
+                     mom_part);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs, const double* Xcur,
+                    double* Xout, double* acc, const double* fnl, const double* chi,
+                    const StepParams& sp, double* mom_part) {
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(c.g.n, return (contract_impl<double, NN>(c, mode, W, npairs, Xcur, Xout, acc,
+                                                              fnl, chi, sp, mom_part, c.t.tw64)));
+  } else {
+    SDFT_SIZE_SWITCH(c.g.n, return (contract_impl<float, NN>(c, mode, W, npairs, Xcur, Xout, acc,
+                                                             fnl, chi, sp, mom_part, c.t.tw32)));
+  }
+  return 0;
+}
+
+template <typename T, int N>
+static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups,
+                        const void* tw) {
+  const Geometry& g = c.g;
+  const size_t lds = plane_lds_bytes<T, N>(g.M);
+  if (lds > 160 * 1024) {
+    set_error("plane buffer exceeds LDS for this grid in this precision");
+    return SDFT_EINVAL_CODE;
+  }
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density<T, N>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid(N, ngroups);
+  hipLaunchKernelGGL((k_density<T, N>), grid, dim3(kThreads), lds, c.stream,
+                     reinterpret_cast<const Cx<T>*>(W), g.ncolp, g.ncol, c.t.col_pos, g.M,
+                     reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups) {
+  SDFT_SIZE_SWITCH(c.g.n, return (density_impl<double, NN>(c, W, npairs, rho_part, ngroups,
+                                                           c.t.tw64)));
+  return 0;
+}
+
+int launch_rho_finish(Ctx& c, const double* rho_part, int ngroups, double scale, int layout,
+                      int accumulate, double* rho) {
+  const long long Ntot = c.g.N;
+  const int nb = (int)((Ntot + 255) / 256);
+  hipLaunchKernelGGL(k_rho_finish, dim3(nb), dim3(256), 0, c.stream, rho_part, ngroups, Ntot,
+                     c.g.n, scale, layout, accumulate, rho);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int N>
+static int fft_test_impl(int nlines, int dir, const double* in, double* out, const void* tw) {
+  constexpr int ML = Cfg<N>::ML;
+  const int nb = (nlines + ML - 1) / ML;
+  if (dir > 0)
+    hipLaunchKernelGGL((k_fft_test<T, N, 1>), dim3(nb), dim3(kThreads), 0, 0, in, out, nlines,
+                       reinterpret_cast<const Cx<T>*>(tw));
+  else
+    hipLaunchKernelGGL((k_fft_test<T, N, -1>), dim3(nb), dim3(kThreads), 0, 0, in, out, nlines,
+                       reinterpret_cast<const Cx<T>*>(tw));
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in, double* out,
+                          const void* tw) {
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(n, return (fft_test_impl<double, NN>(nlines, dir, in, out, tw)));
+  } else {
+    SDFT_SIZE_SWITCH(n, return (fft_test_impl<float, NN>(nlines, dir, in, out, tw)));
+  }
+  return 0;
+}
+
+}  // namespace sdft

@@ -1,0 +1,270 @@
+"""ctypes binding of libsdft_hip.so (include/sdft_hip.h).
+
+The library must have been built (sdft_amd.build.build()); importing this
+module never falls back to a CPU path: a missing library or a failing call
+raises.  Orbital blocks are numpy complex arrays (nstate, ncoef) with G=0
+last, split into the separate FP64 real / imaginary arrays the C ABI takes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+RHO_FFTW = 0
+RHO_GENERIC = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class SdftError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("ncoef", C.c_int),
+                ("map", _ip), ("map_conj", _ip), ("ak2_kinetic", _dp)]
+
+
+EXPORTS = [
+    "sdft_hip_init", "sdft_hip_finalize", "sdft_hip_last_error", "sdft_hip_set_vks",
+    "sdft_hip_set_projectors", "sdft_hip_hpsi", "sdft_hip_cheby_coeffs", "sdft_hip_cheby_filter",
+    "sdft_hip_cheby_moments", "sdft_hip_chem_pot", "sdft_hip_density", "sdft_hip_energies",
+    "sdft_hip_block_create", "sdft_hip_block_destroy", "sdft_hip_block_set",
+    "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
+    "sdft_hip_kernel_times", "sdft_hip_selftest_fft",
+]
+
+_LIB = None
+
+
+def load(path: str | None = None):
+    """Load libsdft_hip.so (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = path or _build.lib_path()
+    if not os.path.exists(p):
+        raise SdftError("libsdft_hip.so not built at %s: run __graft_entry__.build()" % p)
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    sig = {
+        "sdft_hip_init": ([C.POINTER(GridDesc), C.POINTER(vp)], C.c_int),
+        "sdft_hip_finalize": ([vp], C.c_int),
+        "sdft_hip_last_error": ([], C.c_char_p),
+        "sdft_hip_set_vks": ([vp, _dp], C.c_int),
+        "sdft_hip_set_projectors": ([vp, C.c_int, _dp, _dp, _dp], C.c_int),
+        "sdft_hip_hpsi": ([vp, C.c_int, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
+        "sdft_hip_cheby_coeffs": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, _dp], C.c_int),
+        "sdft_hip_cheby_filter": ([vp, C.c_int, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp,
+                                   _dp, _dp, _dp, C.c_int], C.c_int),
+        "sdft_hip_cheby_moments": ([vp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp,
+                                    C.c_int], C.c_int),
+        "sdft_hip_chem_pot": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_double,
+                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
+                               _dp], C.c_int),
+        "sdft_hip_density": ([vp, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_energies": ([vp, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp], C.c_int),
+        "sdft_hip_block_create": ([vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+        "sdft_hip_block_destroy": ([vp], C.c_int),
+        "sdft_hip_block_set": ([vp, _dp, _dp], C.c_int),
+        "sdft_hip_block_filter": ([vp, C.c_int, _dp, C.c_double, C.c_double, _dp], C.c_int),
+        "sdft_hip_block_get": ([vp, C.c_int, _dp, _dp], C.c_int),
+        "sdft_hip_set_profiling": ([vp, C.c_int], C.c_int),
+        "sdft_hip_kernel_times": ([vp, C.c_int, C.POINTER(C.c_char_p), _dp, _ip], C.c_int),
+        "sdft_hip_selftest_fft": ([C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().sdft_hip_last_error().decode(errors="replace")
+        raise SdftError("sdft_hip error %d: %s" % (rc, msg))
+
+
+def _split(Cz):
+    Cz = np.ascontiguousarray(Cz)
+    return np.ascontiguousarray(Cz.real, dtype=np.float64), np.ascontiguousarray(Cz.imag, dtype=np.float64)
+
+
+def selftest_fft(x: np.ndarray, direction: int, prec: int = PREC_FP64) -> np.ndarray:
+    """Transform the rows of complex x (nlines, n) with the library's LDS FFT."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    nl, n = x.shape
+    buf = x.view(np.float64).copy()
+    out = np.empty_like(buf)
+    _check(lib.sdft_hip_selftest_fft(n, nl, direction, prec, _ptr(buf), _ptr(out)))
+    return out.view(np.complex128).reshape(nl, n)
+
+
+class Context:
+    """One system on the current HIP device (grid, G-set, v_KS, KB table)."""
+
+    def __init__(self, sysm, with_nl: bool = True):
+        self.lib = load()
+        self.sys = sysm
+        self._map = np.ascontiguousarray(sysm.map, dtype=np.int32)
+        self._mapc = np.ascontiguousarray(sysm.map_conj, dtype=np.int32)
+        self._ak2 = np.ascontiguousarray(sysm.ak2, dtype=np.float64)
+        d = GridDesc(sysm.n[0], sysm.n[1], sysm.n[2], sysm.ncoef,
+                     self._map.ctypes.data_as(_ip), self._mapc.ctypes.data_as(_ip),
+                     self._ak2.ctypes.data_as(_dp))
+        h = C.c_void_p()
+        _check(self.lib.sdft_hip_init(C.byref(d), C.byref(h)))
+        self.h = h
+        self.set_vks(sysm.vks)
+        if with_nl and sysm.nproj > 0:
+            self.set_projectors(sysm.proj, sysm.proj_energy)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdft_hip_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_vks(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.lib.sdft_hip_set_vks(self.h, _ptr(v)))
+
+    def set_projectors(self, proj, energy):
+        pr = np.ascontiguousarray(proj.real, dtype=np.float64)
+        pi = np.ascontiguousarray(proj.imag, dtype=np.float64)
+        e = np.ascontiguousarray(energy, dtype=np.float64)
+        _check(self.lib.sdft_hip_set_projectors(self.h, pr.shape[0], _ptr(pr), _ptr(pi), _ptr(e)))
+
+    def clear_projectors(self):
+        _check(self.lib.sdft_hip_set_projectors(self.h, 0, None, None, None))
+
+    def hpsi_raw(self, Cz, prec=PREC_FP64):
+        re, im = _split(Cz)
+        fre = np.empty_like(re)
+        fim = np.empty_like(im)
+        _check(self.lib.sdft_hip_hpsi(self.h, re.shape[0], _ptr(re), _ptr(im), _ptr(fre),
+                                      _ptr(fim), prec))
+        return fre + 1j * fim
+
+    def cheby_coeffs(self, L, emin, emax, mu, beta, fun_flag=2):
+        c = np.empty(L)
+        _check(self.lib.sdft_hip_cheby_coeffs(self.h, L, emin, emax, mu, beta, fun_flag, _ptr(c)))
+        return c
+
+    def cheby_filter(self, chi, coeffs, scale, emean, prec=PREC_FP64, moments=False):
+        re, im = _split(chi)
+        ore = np.empty_like(re)
+        oim = np.empty_like(im)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        mom = np.zeros(len(coeffs)) if moments else None
+        _check(self.lib.sdft_hip_cheby_filter(self.h, re.shape[0], len(coeffs), _ptr(coeffs),
+                                              scale, emean, _ptr(re), _ptr(im), _ptr(ore),
+                                              _ptr(oim), _ptr(mom), prec))
+        out = (ore + 1j * oim, re + 1j * im)
+        return out + (mom,) if moments else out
+
+    def cheby_moments(self, chi, L, scale, emean, prec=PREC_FP64):
+        re, im = _split(chi)
+        mom = np.zeros(L + 1)
+        _check(self.lib.sdft_hip_cheby_moments(self.h, re.shape[0], L, scale, emean, _ptr(re),
+                                               _ptr(im), _ptr(mom), prec))
+        return mom
+
+    def chem_pot(self, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ=2.0,
+                 tol=None):
+        m = np.ascontiguousarray(moments, dtype=np.float64)
+        tol = tol if tol is not None else 1e-11 * ne_true
+        mu = C.c_double()
+        ne = C.c_double()
+        _check(self.lib.sdft_hip_chem_pot(self.h, L, emin, emax, beta, _ptr(m), nsto, ne_true,
+                                          mu_init, gap, occ, tol, C.byref(mu), C.byref(ne)))
+        return mu.value, ne.value
+
+    def density(self, Cz, occ_over_nsto, layout=RHO_FFTW):
+        re, im = _split(Cz)
+        rho = np.zeros(self.sys.ngrid)
+        _check(self.lib.sdft_hip_density(self.h, re.shape[0], _ptr(re), _ptr(im), occ_over_nsto,
+                                         _ptr(rho), layout, 0))
+        return rho
+
+    def energies(self, Cz, occ, nsto):
+        re, im = _split(Cz)
+        eke = C.c_double()
+        enl = C.c_double()
+        _check(self.lib.sdft_hip_energies(self.h, re.shape[0], _ptr(re), _ptr(im), occ, nsto,
+                                          C.byref(eke), C.byref(enl)))
+        return eke.value, enl.value
+
+    def set_profiling(self, on: bool):
+        _check(self.lib.sdft_hip_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        names = (C.c_char_p * 32)()
+        ms = np.zeros(32)
+        la = np.zeros(32, dtype=np.int32)
+        n = self.lib.sdft_hip_kernel_times(self.h, 32, names, _ptr(ms), la.ctypes.data_as(_ip))
+        if n < 0:
+            _check(n)
+        return {names[i].decode(): (float(ms[i]), int(la[i])) for i in range(min(n, 32))}
+
+
+class Block:
+    """A device-resident block of orbitals (sdft_hip_block_*)."""
+
+    def __init__(self, ctx: Context, nstate: int, prec: int = PREC_FP64):
+        self.ctx = ctx
+        self.nstate = nstate
+        h = C.c_void_p()
+        _check(ctx.lib.sdft_hip_block_create(ctx.h, nstate, prec, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.sdft_hip_block_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, chi):
+        re, im = _split(chi)
+        _check(self.ctx.lib.sdft_hip_block_set(self.h, _ptr(re), _ptr(im)))
+
+    def filter(self, coeffs, scale, emean, moments=False):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        mom = np.zeros(len(coeffs)) if moments else None
+        _check(self.ctx.lib.sdft_hip_block_filter(self.h, len(coeffs), _ptr(coeffs), scale, emean,
+                                                  _ptr(mom)))
+        return mom
+
+    def get(self, which=0):
+        ncoef = self.ctx.sys.ncoef
+        re = np.empty((self.nstate, ncoef))
+        im = np.empty((self.nstate, ncoef))
+        _check(self.ctx.lib.sdft_hip_block_get(self.h, which, _ptr(re), _ptr(im)))
+        return re + 1j * im

@@ -1,0 +1,100 @@
+"""The reference's filter-path operations, driven over the C ABI.
+
+Names follow stodft/ in the reference: genEnergyMax / genEnergyMin
+(calc-spectral-range.c), calcChebyCoeff (coeff.c), calcChemPotCheby
+(calc-chempot-chebyshev.c), filterChebyPolyHerm (filter.c),
+calcRhoStoHybridCheby (density.c), calcEnergyChemPot (calc-energy.c) and the
+orchestration of genStoOrbitalCheby (gen-stodft-wf.c:356-694).  All the
+arithmetic runs in libsdft_hip; this module only sequences calls.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import hip as _hip
+
+
+def spectral_bounds(sysm) -> tuple[float, float]:
+    """Rigorous bounds of the spectrum of H = T + v + V_NL on the coefficient
+    set: T in [0, max|G|^2/2], v in [min v, max v], and V_NL = B E B^T lies
+    in [lambda_min, lambda_max] of |E|^(1/2) Gram |E|^(1/2) (signed by E),
+    Gram_pq = Q_p(0)Q_q(0) + 2 Re sum_half conj(Q_p) Q_q."""
+    lo_nl = hi_nl = 0.0
+    if sysm.nproj > 0:
+        q = sysm.proj
+        gram = np.outer(q[:, -1].real, q[:, -1].real) + 2.0 * np.real(np.conj(q[:, :-1]) @ q[:, :-1].T)
+        e = sysm.proj_energy
+        s = np.sqrt(np.abs(e))
+        m = (s[:, None] * gram * s[None, :]) * np.sign(e)[None, :]
+        # eigenvalues of E Gram (similar to the symmetric form for E >= 0)
+        ev = np.linalg.eigvals(m).real
+        lo_nl, hi_nl = min(0.0, float(ev.min())), max(0.0, float(ev.max()))
+    vmin, vmax = float(sysm.vks.min()), float(sysm.vks.max())
+    return vmin + lo_nl, 0.5 * float(sysm.ak2.max()) + vmax + hi_nl
+
+
+def _hpsi(ctx, c, prec):
+    f = ctx.hpsi_raw(c, prec)
+    h = -0.25 * f
+    h[:, -1] = -0.5 * f[:, -1]
+    return h
+
+
+def _gdot(a, b):
+    return 2.0 * (np.sum(a.real * b.real) + np.sum(a.imag * b.imag)) - a[..., -1].real @ b[..., -1].real
+
+
+def gen_energy_max(ctx, niter=60, seed=7, pad=0.1, prec=_hip.PREC_FP64):
+    """genEnergyMax (calc-spectral-range.c:33-333): power iteration on H for
+    the largest eigenvalue, padded by +0.1 Ha."""
+    from . import systems
+    v = systems.noise_orbitals(ctx.sys, 1, seed)
+    v /= np.sqrt(_gdot(v, v))
+    e = 0.0
+    for _ in range(niter):
+        w = _hpsi(ctx, v, prec)
+        e = _gdot(v, w)
+        v = w / np.sqrt(_gdot(w, w))
+    return e + pad
+
+
+def gen_energy_min(ctx, emax, niter=200, seed=8, pad=0.1, prec=_hip.PREC_FP64):
+    """genEnergyMin (calc-spectral-range.c:339-566): power iteration on
+    (E_max - H), E_min = E_max - lambda, padded by -0.1 Ha."""
+    from . import systems
+    v = systems.noise_orbitals(ctx.sys, 1, seed)
+    v /= np.sqrt(_gdot(v, v))
+    lam = 0.0
+    for _ in range(niter):
+        w = emax * v - _hpsi(ctx, v, prec)
+        lam = _gdot(v, w)
+        v = w / np.sqrt(_gdot(w, w))
+    return emax - lam - pad
+
+
+@dataclasses.dataclass
+class FilterResult:
+    mu: float
+    ne: float
+    coeffs: np.ndarray
+    moments: np.ndarray
+    filtered: np.ndarray
+    emin: float
+    emax: float
+
+
+def gen_sto_orbital_cheby(ctx, chi, L, beta, ne_true, mu_init, gap, emin=None, emax=None,
+                          occ=2.0, prec=_hip.PREC_FP64):
+    """genStoOrbitalCheby (gen-stodft-wf.c:356-694) with chem_pot_opt = cheby:
+    spectral range -> moments -> mu -> sqrt(f) coefficients -> filter."""
+    if emin is None or emax is None:
+        emin, emax = spectral_bounds(ctx.sys)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    nsto = chi.shape[0]
+    mom = ctx.cheby_moments(chi, L, scale, emean, prec)
+    mu, ne = ctx.chem_pot(L, emin, emax, beta, mom, nsto, ne_true, mu_init, gap, occ)
+    coeffs = ctx.cheby_coeffs(L, emin, emax, mu, beta, 2)
+    acc, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec)
+    return FilterResult(mu, ne, coeffs, mom, acc, emin, emax)

@@ -1,0 +1,154 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle (same seeded inputs).
+
+Every call goes through the C ABI (libsdft_hip.so via ctypes).  Tolerances:
+FP64 form <= 1e-12 relative on H.psi and <= 1e-10 relative on the filtered
+orbitals / density (1e-10 relative in density); FP32
+form checked against FP64 with the bound written in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from sdft_amd import hip as H
+    H.load()
+    return H
+
+
+@pytest.fixture(scope="module")
+def si8_ctx(hip, si8_system):
+    return hip.Context(si8_system)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def bounds(sysm):
+    from sdft_amd import stodft
+    return stodft.spectral_bounds(sysm)
+
+
+@pytest.mark.parametrize("n", [24, 32, 36, 48, 64, 96, 128, 192])
+def test_lds_fft_matches_numpy(hip, n):
+    say(f"case: selftest_fft n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((37, n)) + 1j * rng.standard_normal((37, n))
+    for prec, tol in ((hip.PREC_FP64, 1e-14), (hip.PREC_FP32, 3e-6)):
+        yb = hip.selftest_fft(x, +1, prec)
+        yf = hip.selftest_fft(x, -1, prec)
+        ref_b = np.fft.ifft(x, axis=1) * n
+        ref_f = np.fft.fft(x, axis=1)
+        assert rel(yb, ref_b) < tol * np.log2(n), (n, prec, rel(yb, ref_b))
+        assert rel(yf, ref_f) < tol * np.log2(n), (n, prec, rel(yf, ref_f))
+
+
+@pytest.mark.parametrize("nstate", [1, 2, 5, 8])
+def test_hpsi_fp64_si8(hip, si8_ctx, si8_system, nstate):
+    from sdft_amd import systems
+    say(f"case: hpsi fp64 si8 nstate={nstate}")
+    chi = systems.noise_orbitals(si8_system, nstate, seed=0)
+    f_gpu = si8_ctx.hpsi_raw(chi, hip.PREC_FP64)
+    f_ref = O.hpsi_raw(chi, si8_system)
+    assert rel(f_gpu, f_ref) < 1e-12
+    assert np.all(f_gpu[:, -1].imag == 0.0)
+
+
+def test_hpsi_fp32_si8(hip, si8_ctx, si8_system):
+    from sdft_amd import systems
+    say("case: hpsi fp32 si8")
+    chi = systems.noise_orbitals(si8_system, 6, seed=3)
+    f32 = si8_ctx.hpsi_raw(chi, hip.PREC_FP32)
+    f_ref = O.hpsi_raw(chi, si8_system)
+    # FP32 FFT / v.psi / KB GEMM: relative error bounded by ~ a few 1e-6
+    assert rel(f32, f_ref) < 2e-5
+
+
+def test_hpsi_local_only(hip, si8_system):
+    from sdft_amd import systems
+    say("case: hpsi local-only si8")
+    ctx = hip.Context(si8_system, with_nl=False)
+    chi = systems.noise_orbitals(si8_system, 4, seed=5)
+    f_gpu = ctx.hpsi_raw(chi)
+    f_ref = O.hpsi_raw(chi, si8_system, with_nl=False)
+    assert rel(f_gpu, f_ref) < 1e-12
+    ctx.close()
+
+
+def test_cheby_coeffs(hip, si8_ctx):
+    say("case: cheby coeffs")
+    for L, fun in ((256, 2), (300, 1), (1024, 2)):
+        c_gpu = si8_ctx.cheby_coeffs(L, -0.9, 6.7, 0.21, 100.0, fun)
+        c_ref = O.cheby_coeffs(L, -0.9, 6.7, 0.21, 100.0, fun)
+        assert np.max(np.abs(c_gpu - c_ref)) < 1e-13
+
+
+def test_filter_fp64_si8(hip, si8_ctx, si8_system):
+    from sdft_amd import systems
+    say("case: filter fp64 si8 L=32 B=8")
+    emin, emax = bounds(si8_system)
+    L = 32
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.2, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si8_system, 8, seed=0)
+    acc_g, tl_g, mom_g = si8_ctx.cheby_filter(chi, coeffs, scale, emean, moments=True)
+    acc_r, tl_r, mom_r = O.filter_cheby(chi, coeffs, si8_system, scale, emean, moments=True)
+    assert rel(acc_g, acc_r) < 1e-10
+    assert rel(tl_g, tl_r) < 1e-10
+    assert np.max(np.abs(mom_g - mom_r)) / np.max(np.abs(mom_r)) < 1e-10
+
+
+def test_filter_fp32_si8(hip, si8_ctx, si8_system):
+    from sdft_amd import systems
+    say("case: filter fp32 si8 L=64 B=8")
+    emin, emax = bounds(si8_system)
+    L = 64
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.2, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si8_system, 8, seed=2)
+    acc_g, _ = si8_ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP32)
+    acc_r, _ = O.filter_cheby(chi, coeffs, si8_system, scale, emean)
+    assert rel(acc_g, acc_r) < 1e-4
+
+
+def test_moments_and_chem_pot(hip, si8_ctx, si8_system):
+    from sdft_amd import systems
+    say("case: moments + mu si8 L=40 B=6")
+    emin, emax = bounds(si8_system)
+    L = 40
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si8_system, 6, seed=1)
+    m_g = si8_ctx.cheby_moments(chi, L, scale, emean)
+    m_r = O.cheby_moments(chi, si8_system, L, scale, emean)
+    assert np.max(np.abs(m_g - m_r)) / np.max(np.abs(m_r)) < 1e-10
+    beta = 1.0 / 0.01
+    ne = 32.0
+    mu_r, n_r = O.chem_pot_cheby(m_r, 6, L, emin, emax, beta, ne, 0.2, 0.4)
+    mu_g, n_g = si8_ctx.chem_pot(L, emin, emax, beta, m_r, 6, ne, 0.2, 0.4)
+    assert abs(n_g - ne) <= 1e-11 * ne * 1.0001
+    assert abs(mu_g - mu_r) < 1e-9
+
+
+def test_density_and_energies(hip, si8_ctx, si8_system):
+    from sdft_amd import systems
+    say("case: density + energies si8 B=7")
+    C = systems.noise_orbitals(si8_system, 7, seed=4) * 0.3
+    rho_g = si8_ctx.density(C, 2.0 / 7)
+    rho_r = O.density_real(C, si8_system, 2.0 / 7)
+    assert rel(rho_g, rho_r) < 1e-12
+    eke, enl = si8_ctx.energies(C, 2.0, 7)
+    assert abs(eke - O.energy_ke(C, si8_system.ak2, 2.0, 7)) < 1e-12 * abs(eke)
+    enl_r = O.energy_nl(C, si8_system.proj, si8_system.proj_energy, 2.0, 7)
+    assert abs(enl - enl_r) < 1e-12 * abs(enl_r)
