@@ -85,7 +85,13 @@ def main():
     ap.add_argument("--workload", default="si64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="short run for hardware counters: an 8-term FP32 and FP64 filter of "
+                         "the workload, then a 256 MiB plain copy (counter calibration)")
     args = ap.parse_args()
+    if args.pmc_run:
+        pmc_run(args.workload)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's filter path runs on a single MI355X "
               "(orbital sharding across GPUs is a design note, not a launched mode); "
@@ -169,11 +175,15 @@ def main():
     roofline = None
     if dom_name == "plane_bc_vpsi":
         gbs = bytes_ / avg_s / 1e9
+        tfl = flops / avg_s / 1e12
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": dom_name, "avg_launch_ms": round(avg_s * 1e3, 4),
-                    "algorithmic_bytes": bytes_,
-                    "achieved_tflops": round(flops / avg_s / 1e12, 2)}
+                    "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic("k_plane<float, %d>" % wl.n, wl),
+                    "kernel": "k_plane<float,%d> (b/c transforms + v.psi)" % wl.n,
+                    "avg_launch_ms": round(avg_s * 1e3, 4),
+                    "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
+                    "achieved_tflops": round(tfl, 2),
+                    "valu_frac": round(tfl / FP64_PEAK_TFLOPS, 4)}
     else:
         roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": None, "traffic": None, "kernel": dom_name,
@@ -217,6 +227,39 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(kernel, wl):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel factors)."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_%s.json" % wl.name.split("-")[0])
+    try:
+        import json as _j
+        d = _j.load(open(path))
+        k = d["kernels"][kernel]
+        return round(k["read_bytes"] + k["write_bytes"])
+    except Exception:
+        return None
+
+
+def pmc_run(workload):
+    """Few dispatches of each hot kernel for `rocprofv3 --pmc` passes."""
+    from sdft_amd import hip, stodft, systems
+    wl = systems.WORKLOADS[workload]
+    sysm = systems.build_system(wl.name, wl.a0, wl.rep, wl.n)
+    chi = systems.noise_orbitals(sysm, wl.nstate, wl.seed)
+    ctx = hip.Context(sysm)
+    emin, emax = stodft.spectral_bounds(sysm)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    coeffs = ctx.cheby_coeffs(8, emin, emax, 0.3, 1.0 / wl.kt, 2)
+    for prec in (hip.PREC_FP32, hip.PREC_FP64):
+        blk = hip.Block(ctx, wl.nstate, prec)
+        blk.set(chi)
+        blk.filter(coeffs, scale, emean)
+        blk.close()
+        log("pmc-run: filter prec=%d done" % prec)
+    hip.selftest_copy(256 << 20, 3)
+    log("pmc-run: copy 256 MiB x3 done")
 
 
 def ctx_ncol(sysm):

@@ -152,6 +152,12 @@ int sdft_hip_kernel_times(sdft_ctx *ctx, int max, const char **names, double *ms
  * of mathlib/fft_package.c's FFTW calls. */
 int sdft_hip_selftest_fft(int n, int nlines, int dir, int prec, const double *in, double *out);
 
+/* Copies a device buffer of nbytes (rounded down to doubles) reps times with
+ * a plain streaming kernel (one read + one write per element) on the current
+ * device: the known-size reference against which hardware byte counters of
+ * the filter kernels are scaled.  No reference counterpart. */
+int sdft_hip_selftest_copy(long long nbytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -913,6 +913,30 @@ int sdft_hip_kernel_times(sdft_ctx* h, int max, const char** names, double* ms, 
   return n;
 }
 
+int sdft_hip_selftest_copy(long long nbytes, int reps) {
+  if (nbytes < 8 || reps < 1) {
+    set_error("bad copy selftest arguments");
+    return SDFT_EINVAL;
+  }
+  const size_t n = (size_t)nbytes / sizeof(double);
+  double *a = nullptr, *b = nullptr;
+  int rc = dalloc(&a, n * sizeof(double));
+  if (!rc) rc = dalloc(&b, n * sizeof(double));
+  if (!rc) {
+    hipError_t e = hipMemset(a, 0, n * sizeof(double));
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a, 1.0,
+                         b, n);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = hip_fail(e, "copy selftest", __FILE__, __LINE__);
+  }
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 int sdft_hip_selftest_fft(int n, int nlines, int dir, int prec, const double* in, double* out) {
   if (nlines < 1 || !in || !out || (dir != 1 && dir != -1) || (prec != 0 && prec != 1) ||
       chunk_cols_for(n) == 0) {

@@ -38,7 +38,7 @@ EXPORTS = [
     "sdft_hip_cheby_moments", "sdft_hip_chem_pot", "sdft_hip_density", "sdft_hip_energies",
     "sdft_hip_block_create", "sdft_hip_block_destroy", "sdft_hip_block_set",
     "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
-    "sdft_hip_kernel_times", "sdft_hip_selftest_fft",
+    "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
 ]
 
 _LIB = None
@@ -80,6 +80,7 @@ def load(path: str | None = None):
         "sdft_hip_set_profiling": ([vp, C.c_int], C.c_int),
         "sdft_hip_kernel_times": ([vp, C.c_int, C.POINTER(C.c_char_p), _dp, _ip], C.c_int),
         "sdft_hip_selftest_fft": ([C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp], C.c_int),
+        "sdft_hip_selftest_copy": ([C.c_longlong, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -115,6 +116,11 @@ def selftest_fft(x: np.ndarray, direction: int, prec: int = PREC_FP64) -> np.nda
     out = np.empty_like(buf)
     _check(lib.sdft_hip_selftest_fft(n, nl, direction, prec, _ptr(buf), _ptr(out)))
     return out.view(np.complex128).reshape(nl, n)
+
+
+def selftest_copy(nbytes: int, reps: int = 1) -> None:
+    """Run the library's plain copy kernel (byte-counter calibration)."""
+    _check(load().sdft_hip_selftest_copy(int(nbytes), int(reps)))
 
 
 class Context:
