@@ -152,3 +152,29 @@ def test_density_and_energies(hip, si8_ctx, si8_system):
     assert abs(eke - O.energy_ke(C, si8_system.ak2, 2.0, 7)) < 1e-12 * abs(eke)
     enl_r = O.energy_nl(C, si8_system.proj, si8_system.proj_energy, 2.0, 7)
     assert abs(enl - enl_r) < 1e-12 * abs(enl_r)
+
+
+def test_against_golden_fixture(hip, si8_ctx, si8_system):
+    """The HIP path against the committed oracle fixture (Si8, L=32, B=4)."""
+    import os
+    say("case: golden fixture si8 L=32 B=4")
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "si8_L32_B4.npz")))
+    chi = g["chi"]
+    emin, emax = g["emin_emax"]
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    assert rel(si8_ctx.hpsi_raw(chi), g["f"]) < 1e-12
+    acc, tl = si8_ctx.cheby_filter(chi, g["coeffs"], scale, emean)
+    assert rel(acc, g["acc"]) < 1e-10 and rel(tl, g["tl"]) < 1e-10
+    _, t4 = si8_ctx.cheby_filter(chi, np.ones(5), scale, emean)
+    assert rel(t4, g["T4"]) < 1e-11
+    m = si8_ctx.cheby_moments(chi, 32, scale, emean)
+    assert np.max(np.abs(m - g["moments"])) < 1e-10 * np.max(np.abs(g["moments"]))
+    mu, nmu = si8_ctx.chem_pot(32, emin, emax, 100.0, g["moments"], 4, 32.0, 0.3, 0.4)
+    assert abs(mu - g["mu"][0]) < 1e-9
+    coeffs = si8_ctx.cheby_coeffs(32, emin, emax, g["mu"][0], 100.0, 2)
+    assert np.max(np.abs(coeffs - g["coeffs"])) < 1e-13
+    rho = si8_ctx.density(g["acc"], 2.0 / 4)
+    assert abs(rho.sum() - g["rho_sum"][0]) < 1e-11 * g["rho_sum"][0]
+    assert rel(rho[::16], g["rho_sub"]) < 1e-11
+    eke, enl = si8_ctx.energies(g["acc"], 2.0, 4)
+    assert np.allclose([eke, enl], g["energies"], rtol=1e-12)

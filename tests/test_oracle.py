@@ -1,0 +1,384 @@
+"""Pins of the CPU oracle (oracle/sdft_oracle.py) independent of the oracle's
+own formulas: closed forms, brute-force sums on tiny grids, invariants and
+finite differences.  The reference ships no golden vectors or tests for this
+path, so these checks are what fixes the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import sdft_oracle as O
+
+
+def gdot(a, b):
+    return 2.0 * (np.sum(a[..., :-1].real * b[..., :-1].real, axis=-1)
+                  + np.sum(a[..., :-1].imag * b[..., :-1].imag, axis=-1)) \
+        + a[..., -1].real * b[..., -1].real
+
+
+# ---------------------------------------------------------------------------
+# plane-wave set and maps
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("a0,rep,ncoef", [(10.26, 1, 370), (10.26, 2, 3016), (10.26, 4, 24245),
+                                          (10.26, 6, 81662)])
+def test_sphere_sizes_match_configs(a0, rep, ncoef):
+    from sdft_amd import systems
+    L = a0 * rep
+    k = systems.half_sphere(L, 12.0)
+    assert k.shape[0] == ncoef
+    assert np.all(k[-1] == 0)
+    # exact count of the full sphere by brute force
+    b = 2 * math.pi / L
+    K = int(math.sqrt(12.0) / b) + 1
+    r = np.arange(-K, K + 1)
+    g2 = b * b * (r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
+    assert (g2 <= 12.0).sum() == 2 * (ncoef - 1) + 1
+
+
+def test_maps_are_inverse_pairs(si8_system):
+    s = si8_system
+    n = s.n[0]
+    assert s.map[-1] == 0 and s.map_conj[-1] == 0
+    assert len(set(s.map[:-1]) | set(s.map_conj[:-1])) == 2 * (s.ncoef - 1)
+    for i in range(0, s.ncoef - 1, 7):
+        ka, kb, kc = s.k[i]
+        assert s.map[i] == ((ka % n) * n + kb % n) * n + kc % n
+        assert s.map_conj[i] == ((-ka % n) * n + (-kb) % n) * n + (-kc) % n
+
+
+# ---------------------------------------------------------------------------
+# FFT conventions and pack / unpack
+# ---------------------------------------------------------------------------
+
+def test_fft_conventions_bruteforce():
+    rng = np.random.default_rng(0)
+    n = 6
+    z = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    idx = np.arange(n)
+    A, B, Cc = np.meshgrid(idx, idx, idx, indexing="ij")
+    r = np.stack([A.ravel(), B.ravel(), Cc.ravel()], 1)
+    ph = np.exp(2j * np.pi * (r @ r.T) / n)      # e^{+i k.r 2pi/n}
+    back = (ph @ z.ravel()).reshape(n, n, n)
+    fwd = (np.conj(ph) @ z.ravel()).reshape(n, n, n)
+    assert np.allclose(O.to_r(z), back, atol=1e-10)
+    assert np.allclose(O.to_g(z), fwd, atol=1e-10)
+
+
+def _psi_real_space(c, sysm):
+    """psi(r) = sum_{all G} c(G) e^{iG.r} on the grid, brute force."""
+    n = sysm.n[0]
+    idx = np.arange(n)
+    A, B, Cc = np.meshgrid(idx, idx, idx, indexing="ij")
+    r = np.stack([A.ravel(), B.ravel(), Cc.ravel()], 1) / n
+    k = sysm.k[:-1]
+    ph = np.exp(2j * np.pi * (r @ k.T))
+    psi = 2.0 * (ph * c[None, :-1]).real.sum(axis=1) + c[-1].real
+    return psi
+
+
+def test_pair_pack_gives_two_real_orbitals(tiny_system):
+    from sdft_amd import systems
+    s = tiny_system
+    c = systems.noise_orbitals(s, 2, seed=11)
+    z = O.to_r(O.pack_pair(c[0], c[1], s.map, s.map_conj, s.n)).ravel()
+    assert np.allclose(z.real, _psi_real_space(c[0], s), atol=1e-10)
+    assert np.allclose(z.imag, _psi_real_space(c[1], s), atol=1e-10)
+
+
+def test_plane_wave_eigenvector_closed_form(tiny_system):
+    """Constant v and no KB: a plane wave is an eigenvector with
+    H c = (|G|^2/2 + v0) c (G != 0) and v0 c at G = 0; this fixes the
+    -4/-2 unpack factors, the FFT signs and the kinetic term."""
+    import copy
+    s = copy.copy(tiny_system)
+    v0 = 0.37
+    s.vks = np.full(s.ngrid, v0)
+    for i in (0, 5, s.ncoef - 2, s.ncoef - 1):
+        c = np.zeros((1, s.ncoef), dtype=complex)
+        c[0, i] = 0.6 - 0.3j if i < s.ncoef - 1 else 0.8
+        h = O.hpsi(c, s, with_nl=False)
+        lam = 0.5 * s.ak2[i] + v0
+        assert np.allclose(h, lam * c, atol=1e-12)
+
+
+def test_local_potential_matrix_elements(tiny_system):
+    """(V c)(G) = sum_G' v(G - G') c(G') with v(G) = (1/N) sum_r v(r) e^{-iG.r},
+    summed explicitly over the full sphere."""
+    from sdft_amd import systems
+    s = tiny_system
+    n = s.n[0]
+    c = systems.noise_orbitals(s, 1, seed=2)[0]
+    kfull = np.concatenate([s.k[:-1], -s.k[:-1], np.zeros((1, 3), int)])
+    cfull = np.concatenate([c[:-1], np.conj(c[:-1]), [c[-1].real]])
+    vg = np.fft.fftn(s.vks.reshape(s.n)) / s.ngrid
+    out = np.zeros(s.ncoef, dtype=complex)
+    for i in range(s.ncoef):
+        d = s.k[i][None, :] - kfull
+        out[i] = (vg[d[:, 0] % n, d[:, 1] % n, d[:, 2] % n] * cfull).sum()
+    f = O.hpsi_raw(c[None, :], s, with_nl=False)[0]
+    h = -0.25 * f
+    h[-1] = -0.5 * f[-1]
+    h[:-1] -= 0.5 * s.ak2[:-1] * c[:-1]   # remove kinetic
+    assert np.allclose(h[:-1], out[:-1], atol=1e-12)
+    assert abs(h[-1].real - out[-1].real) < 1e-12
+
+
+def test_hamiltonian_is_hermitian(tiny_system):
+    H = O.dense_hamiltonian(tiny_system)
+    assert np.max(np.abs(H - H.T)) < 1e-13 * np.max(np.abs(H))
+
+
+def test_raw_force_is_minus_energy_gradient(tiny_system):
+    """f = -d<psi|H|psi>/d(re c, im c): fixes the overall -4 / -2 scaling of
+    kinetic, local and nonlocal terms together."""
+    from sdft_amd import systems
+    s = tiny_system
+    c = systems.noise_orbitals(s, 1, seed=4)
+    c[0, -1] = c[0, -1].real
+
+    def energy(cc):
+        return float(gdot(cc, O.hpsi(cc, s))[0])
+
+    f = O.hpsi_raw(c, s)[0]
+    h = 1e-6
+    for i in (0, 3, s.ncoef - 2, s.ncoef - 1):
+        for part in (1.0, 1j):
+            if i == s.ncoef - 1 and part == 1j:
+                continue
+            cp, cm = c.copy(), c.copy()
+            cp[0, i] += h * part
+            cm[0, i] -= h * part
+            g = (energy(cp) - energy(cm)) / (2 * h)
+            got = f[i].real if part == 1.0 else f[i].imag
+            assert abs(got + g) < 1e-6 * max(1.0, abs(g)), (i, part, got, g)
+
+
+def test_kb_table_matches_real_space_quadrature():
+    """Q_p(G) = int beta_p(r) e^{iG.r} d^3r for r^l e^{-r^2/2rc^2} Y_lm (unit
+    norm), by direct quadrature on a fine cube around the atom."""
+    from sdft_amd import systems
+    L = 10.26
+    pos = np.array([[1.3, 2.1, 0.7]])
+    k = np.array([[0, 0, 0], [1, 0, 0], [1, -1, 2], [0, 2, 1], [0, 0, 0]])
+    proj, E, atom, ls, ms = systems.kb_table(L, pos, k, 0.8, (2.0, 1.0))
+    h = 0.08
+    x = np.arange(-4.8, 4.8 + h / 2, h)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    rc = 0.8
+    g = k * (2 * np.pi / L)
+    for p in range(proj.shape[0]):
+        l, m = ls[p], ms[p]
+        if l == 0:
+            beta = np.exp(-r ** 2 / (2 * rc * rc)) / math.sqrt(4 * math.pi)
+        else:
+            comp = (X, Y, Z)[m]
+            beta = comp * np.exp(-r ** 2 / (2 * rc * rc)) * math.sqrt(3 / (4 * math.pi))
+        beta /= math.sqrt((beta ** 2).sum() * h ** 3)
+        for i in range(1, 4):
+            gr = g[i, 0] * (X + pos[0, 0]) + g[i, 1] * (Y + pos[0, 1]) + g[i, 2] * (Z + pos[0, 2])
+            q = (beta * np.exp(1j * gr)).sum() * h ** 3
+            assert abs(q - proj[p, i]) < 2e-6 * max(1.0, abs(q)), (p, i, q, proj[p, i])
+    assert np.allclose(E, np.where(ls == 0, 2.0, 1.0) / L ** 3, rtol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Chebyshev coefficients and recurrence
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("L,M", [(256, 8748), (1024, 34992), (2048, 69984), (3000, 98304),
+                                 (100, 3456), (37, 1296)])
+def test_round_fft(L, M):
+    """roundFFT (coeff.c:1556-1600) by hand: the only pure power of two it
+    considers is 2^nmax with nmax = floor(log2 n) + 1, and every loop candidate
+    is 2^i 3^(floor((ln n - i ln2)/ln3) + 1), i.e. carries a factor >= 3.  For
+    n = 8192 = 2^13: 2^14 is 8192 away, the candidates for i = 0..13 are
+    19683, 13122, 8748, 17496, 11664, 23328, 15552, 10368, 20736, 13824, 9216,
+    18432, 12288, 24576, the nearest being 4 * 3^7 = 8748.  The other rows
+    follow the same enumeration (32 L = 32768 -> 16 * 3^7 = 34992, 65536 ->
+    32 * 3^7 = 69984, 96000 -> 2^15 * 3 = 98304, 3200 -> 2^7 * 3^3 = 3456,
+    1184 -> 2^4 * 3^4 = 1296)."""
+    assert O.round_fft(32 * L) == M
+
+
+def test_round_fft_is_2n3m():
+    for n_in in range(50, 5000, 37):
+        m = O.round_fft(n_in)
+        while m % 2 == 0:
+            m //= 2
+        while m % 3 == 0:
+            m //= 3
+        assert m == 1
+
+
+@pytest.mark.parametrize("fun", [1, 2])
+def test_cheby_coeffs_direct_cosine_sums(fun):
+    L, emin, emax, mu, beta = 64, -1.2, 7.5, 0.3, 60.0
+    c = O.cheby_coeffs(L, emin, emax, mu, beta, fun)
+    M = O.round_fft(32 * L)
+    j = np.arange(M, dtype=np.longdouble)
+    x = (emax - emin) / 2 * np.cos(2 * np.pi * j / M) + (emax + emin) / 2
+    fv = 1 / (1 + np.exp(np.longdouble(beta) * (x - mu)))
+    if fun == 2:
+        fv = np.sqrt(fv)
+    for k in range(L):
+        ck = (fv * np.cos(2 * np.pi * j * k / M)).sum() * (2 if k else 1) / M
+        assert abs(float(ck) - c[k]) < 1e-14
+
+
+def test_cheby_fit_error_criterion():
+    """testChebyCoeff (coeff.c:1319-1460): max error of sum c_k T_k on 1000
+    points below 1e-9 once the degree resolves the Fermi edge."""
+    emin, emax, mu, beta = -1.0, 7.0, 0.2, 100.0
+    x = np.linspace(emin, emax, 1000)
+    y = (x - 0.5 * (emin + emax)) * 2 / (emax - emin)
+
+    def err(L):
+        c = O.cheby_coeffs(L, emin, emax, mu, beta, 2)
+        return np.max(np.abs(O.cheby_eval(c, y) - np.sqrt(O.fermi(x, mu, beta))))
+
+    # genChebyHermit (coeff.c:837,864-876): start at int(2 beta dE), +1000
+    L = int(2 * beta * (emax - emin))
+    e0 = err(L)
+    while err(L) >= 1e-9:
+        L += 1000
+    assert e0 > 1e-9 and L == 2600 and err(L) < 1e-9
+
+
+def test_filter_on_exact_eigenvector(tiny_system):
+    """Plane wave with constant v, no KB: the filter returns p(y) c with
+    p = sum_k c_k T_k evaluated by the scalar recurrence."""
+    import copy
+    s = copy.copy(tiny_system)
+    s.vks = np.full(s.ngrid, -0.2)
+    s.proj = np.zeros((0, s.ncoef), dtype=complex)
+    s.proj_energy = np.zeros(0)
+    emin, emax = -0.5, 2.0
+    L = 40
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.1, 30.0, 2)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    i = 7
+    c = np.zeros((1, s.ncoef), dtype=complex)
+    c[0, i] = 1.0 + 0.5j
+    acc, _ = O.filter_cheby(c, coeffs, s, scale, emean, with_nl=False)
+    lam = 0.5 * s.ak2[i] - 0.2
+    p = O.cheby_eval(coeffs, np.array([scale * (lam - emean)]))[0]
+    assert np.allclose(acc, p * c, atol=1e-12)
+
+
+def test_filter_matches_dense_spectral_form(tiny_system):
+    """The FAST_FILTER idea (filter.c:1134-1526): sqrt f(H) chi from the
+    eigen-decomposition of the dense H; agreement to the fit error."""
+    from sdft_amd import systems, stodft
+    s = tiny_system
+    H = O.dense_hamiltonian(s)
+    w, V = np.linalg.eigh(0.5 * (H + H.T))
+    emin, emax = stodft.spectral_bounds(s)
+    L, mu, beta = 600, 0.6, 40.0
+    coeffs = O.cheby_coeffs(L, emin, emax, mu, beta, 2)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, 3, seed=9)
+    acc, _ = O.filter_cheby(chi, coeffs, s, scale, emean)
+    x = O.to_real_vec(chi)
+    exact = O.from_real_vec((V @ (np.sqrt(O.fermi(w, mu, beta))[:, None] * (V.T @ x.T))).T, s.ncoef)
+    fit = np.max(np.abs(O.cheby_eval(coeffs, scale * (w - emean)) - np.sqrt(O.fermi(w, mu, beta))))
+    assert fit < 1e-8
+    assert np.max(np.abs(acc - exact)) < 50 * fit * np.max(np.abs(chi)) + 1e-10
+
+
+# ---------------------------------------------------------------------------
+# moments, electron count, chemical potential
+# ---------------------------------------------------------------------------
+
+def test_moment_doubling_matches_direct(tiny_system):
+    from sdft_amd import systems, stodft
+    s = tiny_system
+    emin, emax = stodft.spectral_bounds(s)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, 4, seed=1)
+    L = 30
+    m = O.cheby_moments(chi, s, L, scale, emean)
+    _, _, direct = O.filter_cheby(chi, np.ones(L + 1), s, scale, emean, moments=True)
+    assert np.max(np.abs(m[:L + 1] - direct[:L + 1])) < 1e-9 * np.max(np.abs(direct))
+
+
+def test_moments_trace_and_mu_on_complete_basis(tiny_system):
+    """With chi = the orthonormal basis, sum_s <chi_s|T_k|chi_s> = Tr T_k(H^)
+    exactly; then N(mu) = occ sum_i f(e_i) and the bisection finds the mu of
+    the eigenvalue sum (to the fit error)."""
+    from scipy.optimize import brentq
+
+    from sdft_amd import stodft
+    s = tiny_system
+    H = O.dense_hamiltonian(s)
+    w = np.linalg.eigvalsh(0.5 * (H + H.T))
+    emin, emax = stodft.spectral_bounds(s)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    dim = w.size
+    chi = O.from_real_vec(np.eye(dim), s.ncoef)
+    L = 600
+    m = O.cheby_moments(chi, s, L, scale, emean)
+    y = scale * (w - emean)
+    for k in (0, 1, 2, 7, 50, 101, 299, 300, 301, 599):
+        assert abs(m[k] - np.cos(k * np.arccos(np.clip(y, -1, 1))).sum()) < 1e-8 * dim
+    beta, occ = 40.0, 2.0
+    ne = 24.0
+    mu_true = brentq(lambda mu: occ * O.fermi(w, mu, beta).sum() - ne, emin, emax)
+    mu, nn = O.chem_pot_cheby(m, 1, L, emin, emax, beta, ne, 0.5, 0.3)
+    assert abs(nn - ne) <= 1e-11 * ne
+    assert abs(mu - mu_true) < 1e-6
+
+
+def test_num_elec_is_monotone(tiny_system):
+    from sdft_amd import systems, stodft
+    s = tiny_system
+    emin, emax = stodft.spectral_bounds(s)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, 8, seed=3)
+    m = O.cheby_moments(chi, s, 120, scale, emean) / 8
+    ns = np.array([O.num_elec_cheby(mu, m, 120, emin, emax, 20.0)
+                   for mu in np.linspace(emin, emax, 40)])
+    # monotone up to the Gibbs ripple of the finite expansion of f
+    assert np.all(np.diff(ns) > -1e-5 * ns.max())
+    assert ns[-1] - ns[0] > 0.9 * ns.max()
+
+
+# ---------------------------------------------------------------------------
+# density, energies, forces
+# ---------------------------------------------------------------------------
+
+def test_density_sum_rule_and_brute_force(tiny_system):
+    from sdft_amd import systems
+    s = tiny_system
+    C = systems.noise_orbitals(s, 3, seed=6)
+    rho = O.density_real(C, s, 2.0 / 3)
+    assert abs(rho.sum() / s.ngrid - 2.0 / 3 * gdot(C, C).sum()) < 1e-10 * rho.sum() / s.ngrid
+    brute = sum(_psi_real_space(C[i], s) ** 2 for i in range(3)) * 2.0 / 3
+    assert np.allclose(rho, brute, atol=1e-10)
+
+
+def test_kinetic_energy_closed_form(tiny_system):
+    s = tiny_system
+    c = np.zeros((2, s.ncoef), dtype=complex)
+    c[0, 3] = 1.0
+    c[1, 9] = 0.5j
+    e = O.energy_ke(c, s.ak2, 2.0, 2)
+    assert abs(e - (2.0 / 2 * 0.5 * 2 * (s.ak2[3] + 0.25 * s.ak2[9]))) < 1e-14
+
+
+def test_nl_forces_finite_difference():
+    from sdft_amd import systems
+    base = systems.build_system("fd", 10.26, 1, 24, g2max=3.0)
+    C = systems.noise_orbitals(base, 2, seed=5)
+    F = O.nl_forces(C, base, 2.0, 2)
+    h = 1e-4
+    for a, x in ((0, 0), (3, 2), (5, 1)):
+        e = []
+        for sgn in (1, -1):
+            pos = base.pos.copy()
+            pos[a, x] += sgn * h
+            proj, E, *_ = systems.kb_table(base.L, pos, base.k, 0.8, (2.0, 1.0))
+            e.append(O.energy_nl(C, proj, E, 2.0, 2))
+        g = (e[0] - e[1]) / (2 * h)
+        assert abs(F[a, x] + g) < 1e-6 * max(1.0, abs(g)), (a, x, F[a, x], g)
