@@ -1,0 +1,67 @@
+/*
+ * sdft_stodft.h -- host C shim with the reference's stochastic-DFT filter-path
+ * entry points (stodft/proto_stodft_local.h) over the sdft_hip C ABI.
+ *
+ * The reference functions take PINY's CP / CLASS / GENERAL_DATA structs; this
+ * shim gathers exactly the fields they read into one struct, SDFT_STODFT,
+ * keeps the reference's array conventions (1-based `cmalloc(n)-1` arrays,
+ * state-major re/im coefficient arrays with G=0 at index ncoef) and its error
+ * behaviour (print an @@@..._ERROR_@@@ banner, exit(1)).  A PINY build links
+ * libsdft_host.so and fills SDFT_STODFT from its structs.
+ */
+#ifndef SDFT_STODFT_H
+#define SDFT_STODFT_H
+
+#include "sdft_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct SDFT_STODFT {
+  sdft_ctx *ctx;            /* from sdft_hip_init + set_vks + set_projectors */
+  int prec;                 /* SDFT_PREC_FP64 / SDFT_PREC_FP32 */
+  /* CPCOEFFS_INFO / STODFTINFO / CHEBYSHEVINFO scalars */
+  int numStateUpProc;       /* nstate_up_proc */
+  int numCoeff;             /* ncoef */
+  int polynormLength;       /* Chebyshev length L */
+  int numStateStoUp;        /* N_sto (all ranks) */
+  double energyMin, energyMax, energyMean, energyDiff;
+  double scale;             /* chebyshevInfo->scale = 2 / energyDiff */
+  double beta;              /* 1 / kT */
+  double occNumber;         /* 2 for unpolarised */
+  double numElecTrue;       /* N_e */
+  double chemPotInit, gapInit;
+  /* CPCOEFFS_POS / STODFTCOEFPOS arrays (1-based, numStateUpProc*numCoeff) */
+  double *cre_up, *cim_up;
+  double *fcre_up, *fcim_up;
+  double *wfUpRe1, *wfUpIm1;   /* T_{k-1} between normHCheby calls */
+  double **stoWfUpRe, **stoWfUpIm; /* [0] = filtered orbitals */
+  double *expanCoeff;       /* [polynormLength] (numChemPot = 1), 0-based */
+  double *chebyMomentsUp;   /* [polynormLength + 1], 0-based */
+  double *chemPot;          /* [1] */
+  double *rhoUp;            /* [n1*n2*n3] density output, 0-based, FFTW order */
+} SDFT_STODFT;
+
+/* stodft/filter.c:318-706: acc = sum_k c_k T_k chi; cre_up left = T_{L-1} chi. */
+void filterChebyPolyHerm(SDFT_STODFT *st);
+/* stodft/normh.c:182-344: one H application + recurrence step iPoly on the
+ * host arrays (cpyFlag = 1 rotates cre <- T_k, wfUpRe1 <- T_{k-1}). */
+void normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag);
+/* stodft/reduced-energy/energy-wrapper-scf.c:1066-1197: fcre/fcim = raw force. */
+void calcCoefForceWrapSCF(SDFT_STODFT *st);
+/* stodft/coeff.c:1093-1223 (funFlag 1 = f, 2 = sqrt f). */
+void calcChebyCoeff(SDFT_STODFT *st, double *chemPot, double *chebyCoeffs, int funFlag);
+/* stodft/calc-chempot-chebyshev.c:596-913 (cre_up = chi, unchanged). */
+void calcChebyMoments(SDFT_STODFT *st);
+/* stodft/calc-chempot-chebyshev.c:918-972. */
+double calcNumElecCheby(SDFT_STODFT *st, double chemPot, double *chebyCoeffs);
+/* stodft/calc-chempot-chebyshev.c:30-315: moments, then chemPot[0] = mu. */
+void calcChemPotCheby(SDFT_STODFT *st);
+/* stodft/density.c:1276-1520 (real-space part): rhoUp = occ/N_sto sum psi^2. */
+void calcRhoStoHybridCheby(SDFT_STODFT *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDFT_STODFT_H */

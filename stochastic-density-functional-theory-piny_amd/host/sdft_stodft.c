@@ -1,0 +1,132 @@
+/*
+ * sdft_stodft.c -- the reference's filter-path entry points over the C ABI
+ * (declarations and field meanings in include/sdft_stodft.h).
+ */
+#include "sdft_stodft.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void sdft_die(const char *where, int rc) {
+  printf("@@@@@@@@@@@@@@@@@@@@_ERROR_@@@@@@@@@@@@@@@@@@@@\n");
+  printf("%s failed (status %d): %s\n", where, rc, sdft_hip_last_error());
+  printf("@@@@@@@@@@@@@@@@@@@@_ERROR_@@@@@@@@@@@@@@@@@@@@\n");
+  fflush(stdout);
+  exit(1);
+}
+
+#define SDFT_CALL(where, expr)          \
+  do {                                  \
+    int rc_ = (expr);                   \
+    if (rc_ != 0) sdft_die(where, rc_); \
+  } while (0)
+
+void calcCoefForceWrapSCF(SDFT_STODFT *st) {
+  SDFT_CALL("calcCoefForceWrapSCF",
+            sdft_hip_hpsi(st->ctx, st->numStateUpProc, &st->cre_up[1], &st->cim_up[1],
+                          &st->fcre_up[1], &st->fcim_up[1], st->prec));
+}
+
+/* normh.c:182-344 */
+void normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag) {
+  const int nc = st->numCoeff;
+  const int tot = st->numStateUpProc * nc;
+  double s1 = -st->scale * 0.25;
+  double s2 = -st->scale * 0.5;
+  double p = -st->scale * st->energyMean;
+  double *cre = st->cre_up, *cim = st->cim_up;
+  double *fre = st->fcre_up, *fim = st->fcim_up;
+  double *wre = st->wfUpRe1, *wim = st->wfUpIm1;
+  int is, i;
+  calcCoefForceWrapSCF(st);
+  if (iPoly == 1) {
+    for (is = 0; is < st->numStateUpProc; is++) {
+      const int off = is * nc;
+      for (i = 1; i < nc; i++) {
+        fre[off + i] = s1 * fre[off + i] + p * cre[off + i];
+        fim[off + i] = s1 * fim[off + i] + p * cim[off + i];
+      }
+      fre[off + nc] = s2 * fre[off + nc] + p * cre[off + nc];
+    }
+    if (cpyFlag == 1) {
+      memcpy(&cre[1], &fre[1], tot * sizeof(double));
+      memcpy(&cim[1], &fim[1], tot * sizeof(double));
+    }
+  } else {
+    s1 *= 2.0;
+    s2 *= 2.0;
+    p *= 2.0;
+    for (is = 0; is < st->numStateUpProc; is++) {
+      const int off = is * nc;
+      for (i = 1; i < nc; i++) {
+        fre[off + i] = s1 * fre[off + i] + p * cre[off + i] - wre[off + i];
+        fim[off + i] = s1 * fim[off + i] + p * cim[off + i] - wim[off + i];
+      }
+      fre[off + nc] = s2 * fre[off + nc] + p * cre[off + nc] - wre[off + nc];
+    }
+    if (cpyFlag == 1) {
+      memcpy(&wre[1], &cre[1], tot * sizeof(double));
+      memcpy(&wim[1], &cim[1], tot * sizeof(double));
+      memcpy(&cre[1], &fre[1], tot * sizeof(double));
+      memcpy(&cim[1], &fim[1], tot * sizeof(double));
+    }
+  }
+}
+
+/* filter.c:318-706, the whole loop on the device */
+void filterChebyPolyHerm(SDFT_STODFT *st) {
+  const int tot = st->numStateUpProc * st->numCoeff;
+  SDFT_CALL("filterChebyPolyHerm",
+            sdft_hip_cheby_filter(st->ctx, st->numStateUpProc, st->polynormLength, st->expanCoeff,
+                                  st->scale, st->energyMean, &st->cre_up[1], &st->cim_up[1],
+                                  &st->stoWfUpRe[0][1], &st->stoWfUpIm[0][1], NULL, st->prec));
+  if (st->polynormLength > 1) {
+    /* normHCheby with cpyFlag 1 leaves fcre = cre = T_{L-1} */
+    memcpy(&st->fcre_up[1], &st->cre_up[1], tot * sizeof(double));
+    memcpy(&st->fcim_up[1], &st->cim_up[1], tot * sizeof(double));
+  }
+}
+
+void calcChebyCoeff(SDFT_STODFT *st, double *chemPot, double *chebyCoeffs, int funFlag) {
+  SDFT_CALL("calcChebyCoeff",
+            sdft_hip_cheby_coeffs(st->ctx, st->polynormLength, st->energyMin, st->energyMax,
+                                  chemPot[0], st->beta, funFlag, chebyCoeffs));
+}
+
+void calcChebyMoments(SDFT_STODFT *st) {
+  SDFT_CALL("calcChebyMoments",
+            sdft_hip_cheby_moments(st->ctx, st->numStateUpProc, st->polynormLength, st->scale,
+                                   st->energyMean, &st->cre_up[1], &st->cim_up[1],
+                                   st->chebyMomentsUp, st->prec));
+}
+
+/* calc-chempot-chebyshev.c:918-972 (moments already normalised by N_sto) */
+double calcNumElecCheby(SDFT_STODFT *st, double chemPot, double *chebyCoeffs) {
+  double ne = 0.0;
+  int k;
+  calcChebyCoeff(st, &chemPot, chebyCoeffs, 1);
+  for (k = 0; k < st->polynormLength; k++) ne += chebyCoeffs[k] * st->chebyMomentsUp[k];
+  return ne * st->occNumber;
+}
+
+/* calc-chempot-chebyshev.c:30-315 */
+void calcChemPotCheby(SDFT_STODFT *st) {
+  double mu = 0.0, ne = 0.0;
+  int k;
+  calcChebyMoments(st);
+  for (k = 0; k <= st->polynormLength; k++) st->chebyMomentsUp[k] /= (double)st->numStateStoUp;
+  SDFT_CALL("calcChemPotCheby",
+            sdft_hip_chem_pot(st->ctx, st->polynormLength, st->energyMin, st->energyMax, st->beta,
+                              st->chebyMomentsUp, 1.0, st->numElecTrue, st->chemPotInit,
+                              st->gapInit, st->occNumber, 1.0e-11 * st->numElecTrue, &mu, &ne));
+  st->chemPot[0] = mu;
+  printf("Correct Chemical Potential is %.16lg\n", mu);
+}
+
+void calcRhoStoHybridCheby(SDFT_STODFT *st) {
+  SDFT_CALL("calcRhoStoHybridCheby",
+            sdft_hip_density(st->ctx, st->numStateUpProc, &st->stoWfUpRe[0][1],
+                             &st->stoWfUpIm[0][1], st->occNumber / (double)st->numStateStoUp,
+                             st->rhoUp, SDFT_RHO_FFTW, 0));
+}

@@ -43,6 +43,18 @@ def test_python_binding_covers_all_declared():
     assert sorted(hip.EXPORTS) == declared_functions()
 
 
+def test_host_shim_exports_reference_entry_points(lib_path):
+    from sdft_amd import build
+    src = open(os.path.join(REPO, "include", "sdft_stodft.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(?:void|double)\s+(\w+)\s*\(SDFT_STODFT", src)))
+    assert names == sorted(["filterChebyPolyHerm", "normHCheby", "calcCoefForceWrapSCF",
+                            "calcChebyCoeff", "calcChebyMoments", "calcNumElecCheby",
+                            "calcChemPotCheby", "calcRhoStoHybridCheby"])
+    host = ctypes.CDLL(build.host_lib_path())
+    assert all(hasattr(host, n) for n in names)
+
+
 def test_last_error_without_gpu(lib_path):
     lib = ctypes.CDLL(lib_path)
     lib.sdft_hip_last_error.restype = ctypes.c_char_p
