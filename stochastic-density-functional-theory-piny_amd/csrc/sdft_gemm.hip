@@ -9,10 +9,14 @@
 //   Z  = A X                      (projections, split-K, reduced in FP64)
 //   Y  = -2 diag(E) Z
 //   F  = A^T Y                    (raw NL force, -4/-2 scaling included)
-// Tiles: 64x64 per workgroup of 4 waves, each wave 2x2 MFMA 16x16 blocks,
-// K staged through LDS 16 at a time.  FP64 form: v_mfma_f64_16x16x4f64;
-// FP32 form: v_mfma_f32_16x16x4f32 with FP32 accumulation, results stored
-// and reduced in FP64.
+//
+// Both products run on the 16x16x4 MFMA (v_mfma_f64_16x16x4f64 for the FP64
+// form, v_mfma_f32_16x16x4f32 for the FP32 form).  The K order of an MFMA
+// accumulation is free, so lane l (row or column l%16, group g = l/16) owns
+// the 8 consecutive k = k0 + 8g .. k0 + 8g + 7 of a 32-wide K step and loads
+// them with vector loads straight into registers: no LDS staging and no
+// barriers inside the K loop.  Each wave accumulates a 32x32 tile (2x2 MFMA
+// blocks).
 #include "sdft_internal.h"
 
 namespace sdft {
@@ -44,28 +48,30 @@ struct Mfma<float> {
   __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int KS = 32;   // K per step: 4 lane groups x 8 consecutive k
+constexpr int KPL = 8;   // k per lane per step
 
-// C[m][n] (+ split offset) = alpha * sum_{k in split} A(m,k) B(k,n)
-// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; one of sam/sak and one
-// of sbk/sbn is 1 (the contiguous direction, used for coalesced tile loads).
-template <typename TA, typename TB, typename TI>
-__global__ __launch_bounds__(256) void k_gemm(int M, int Nn, int K, int kchunk,
-                                              const TA* __restrict__ A, long sam, long sak,
-                                              const TB* __restrict__ B, long sbk, long sbn,
-                                              double* __restrict__ C, long scm, long scn,
-                                              long c_split, double alpha) {
-  __shared__ TI As[BK][BM + 1];
-  __shared__ TI Bs[BK][BN + 1];
+// 8 consecutive values src[0..7] converted to TI (vector loads)
+template <typename TI, typename TS>
+__device__ inline void load8(const TS* __restrict__ src, TI* dst) {
+#pragma unroll
+  for (int t = 0; t < KPL; ++t) dst[t] = TI(src[t]);
+}
+
+// ---- Z partials: zpart[z][p][s] = sum_{k in split z} A[p][k] X[s][k] -------
+// grid (ceil(P/32), ceil(ns/32), nsplit), 4 waves per workgroup share the
+// 32x32 output tile and interleave the split's K steps; the four partial
+// tiles are summed through LDS.
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
+                                                    const double* __restrict__ X, int P, int ns,
+                                                    int K, int ksteps_per_split,
+                                                    double* __restrict__ zpart) {
   typedef typename Mfma<TI>::acc_t acc_t;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.x * BM;
-  const int n0 = blockIdx.y * BN;
-  const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(K, kbeg + kchunk);
+  __shared__ double red[4][32 * 32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p0 = blockIdx.x * 32, s0 = blockIdx.y * 32;
+  const int g = lane >> 4, r16 = lane & 15;
   acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -73,69 +79,77 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int Nn, int K, int kchunk,
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
-
-  const bool a_kfast = (sak == 1);
-  const bool b_nfast = (sbn == 1);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+  const int step0 = blockIdx.z * ksteps_per_split;
+  const int nsteps_total = K / KS;
+  const int step1 = min(nsteps_total, step0 + ksteps_per_split);
+  const TA* arow[2];
+  const double* xrow[2];
+  bool aok[2], xok[2];
 #pragma unroll
-    for (int i = 0; i < (BM * BK) / 256; ++i) {
-      const int e = tid + i * 256;
-      int m, k;
-      if (a_kfast) {
-        k = e % BK;
-        m = e / BK;
-      } else {
-        m = e % BM;
-        k = e / BM;
-      }
-      const int gm = m0 + m, gk = k0 + k;
-      TI val = TI(0);
-      if (gm < M && gk < kend) val = TI(A[(long)gm * sam + (long)gk * sak]);
-      As[k][m] = val;
+  for (int i = 0; i < 2; ++i) {
+    const int p = p0 + 16 * i + r16;
+    aok[i] = p < P;
+    arow[i] = A + (size_t)(aok[i] ? p : 0) * K + KPL * g;
+    const int s = s0 + 16 * i + r16;
+    xok[i] = s < ns;
+    xrow[i] = X + (size_t)(xok[i] ? s : 0) * K + KPL * g;
+  }
+  // software pipeline: the operands of the next K step are loaded while the
+  // MFMAs of the current one run
+  TI a0[2][KPL], b0[2][KPL], a1[2][KPL], b1[2][KPL];
+  auto load_step = [&](int st, TI (&a)[2][KPL], TI (&b)[2][KPL]) {
+    const int k0 = st * KS;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (aok[i]) load8<TI>(arow[i] + k0, a[i]);
+      else
+#pragma unroll
+        for (int t = 0; t < KPL; ++t) a[i][t] = TI(0);
+      if (xok[i]) load8<TI>(xrow[i] + k0, b[i]);
+      else
+#pragma unroll
+        for (int t = 0; t < KPL; ++t) b[i][t] = TI(0);
     }
+  };
+  auto mma_step = [&](TI (&a)[2][KPL], TI (&b)[2][KPL]) {
 #pragma unroll
-    for (int i = 0; i < (BN * BK) / 256; ++i) {
-      const int e = tid + i * 256;
-      int n, k;
-      if (b_nfast) {
-        n = e % BN;
-        k = e / BN;
-      } else {
-        k = e % BK;
-        n = e / BK;
-      }
-      const int gn = n0 + n, gk = k0 + k;
-      TI val = TI(0);
-      if (gn < Nn && gk < kend) val = TI(B[(long)gk * sbk + (long)gn * sbn]);
-      Bs[k][n] = val;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      TI a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kr][wm * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+    for (int t = 0; t < KPL; ++t)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<TI>::run(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<TI>::run(a[i][t], b[j][t], acc[i][j]);
+  };
+  int st = step0 + wave;
+  if (st < step1) load_step(st, a0, b0);
+  while (st < step1) {
+    const int nx = st + 4;
+    if (nx < step1) load_step(nx, a1, b1);
+    mma_step(a0, b0);
+    st = nx;
+    if (st >= step1) break;
+    const int nx2 = st + 4;
+    if (nx2 < step1) load_step(nx2, a0, b0);
+    mma_step(a1, b1);
+    st = nx2;
   }
-  double* Cz = C + (long)blockIdx.z * c_split;
+  // sum the four waves' tiles
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm * 32 + i * 16 + Mfma<TI>::row(lane, r);
-        const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (gm < M && gn < Nn) Cz[(long)gm * scm + (long)gn * scn] = alpha * double(acc[i][j][r]);
+        const int row = 16 * i + Mfma<TI>::row(lane, r);
+        const int col = 16 * j + r16;
+        red[wave][row * 32 + col] = double(acc[i][j][r]);
       }
+  __syncthreads();
+  double* zp = zpart + (size_t)blockIdx.z * P * ns;
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int row = e >> 5, col = e & 31;
+    const int p = p0 + row, s = s0 + col;
+    if (p < P && s < ns) zp[(size_t)p * ns + s] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+  }
 }
 
 // Z[p][s] = sum_z part[z][p][s];  Y[p][s] = -2 E_p Z[p][s]
@@ -151,17 +165,80 @@ __global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, 
   Y[i] = -2.0 * E[p] * s;
 }
 
-template <typename TA, typename TB, typename TI>
-static int gemm(Ctx& c, int M, int Nn, int K, int nsplit, const TA* A, long sam, long sak,
-                const TB* B, long sbk, long sbn, double* C, long scm, long scn, long c_split) {
-  int kchunk = (K + nsplit - 1) / nsplit;
-  kchunk = (kchunk + BK - 1) / BK * BK;
-  nsplit = (K + kchunk - 1) / kchunk;
-  dim3 grid((M + BM - 1) / BM, (Nn + BN - 1) / BN, nsplit);
-  hipLaunchKernelGGL((k_gemm<TA, TB, TI>), grid, dim3(256), 0, c.stream, M, Nn, K, kchunk, A, sam,
-                     sak, B, sbk, sbn, C, scm, scn, c_split, 1.0);
-  SDFT_HIP_CHECK(hipGetLastError());
-  return nsplit;
+// ---- F[s][k] = sum_p Y[p][s] A[p][k] ---------------------------------------
+// grid (ceil(ns/32), ceil(K2/32)): one 32 (s) x 32 (k) tile per workgroup; its
+// 4 waves take interleaved 32-wide slices of the reduction index p and their
+// partial tiles are summed through LDS.  Loads are coalesced across the 16
+// lanes of a group (consecutive s for Y, consecutive k for A).
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_force(const double* __restrict__ Y,
+                                                  const TA* __restrict__ A, int P, int ns, int K2,
+                                                  double* __restrict__ F) {
+  typedef typename Mfma<TI>::acc_t acc_t;
+  __shared__ double red[4][32 * 32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int s0 = blockIdx.x * 32;
+  const int k0 = blockIdx.y * 32;
+  const int g = lane >> 4, r16 = lane & 15;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
+  bool sok[2], kok[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    sok[i] = (s0 + 16 * i + r16) < ns;
+    kok[i] = (k0 + 16 * i + r16) < K2;
+  }
+  TI a0[2][KPL], b0[2][KPL], a1[2][KPL], b1[2][KPL];
+  auto load_step = [&](int pb, TI (&a)[2][KPL], TI (&b)[2][KPL]) {
+#pragma unroll
+    for (int t = 0; t < KPL; ++t) {
+      const int p = pb + KPL * g + t;
+      const bool pok = p < P;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i][t] = (pok && sok[i]) ? TI(Y[(size_t)p * ns + s0 + 16 * i + r16]) : TI(0);
+        b[i][t] = (pok && kok[i]) ? TI(A[(size_t)p * K2 + k0 + 16 * i + r16]) : TI(0);
+      }
+    }
+  };
+  auto mma_step = [&](TI (&a)[2][KPL], TI (&b)[2][KPL]) {
+#pragma unroll
+    for (int t = 0; t < KPL; ++t)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<TI>::run(a[i][t], b[j][t], acc[i][j]);
+  };
+  int pb = wave * KS;
+  if (pb < P) load_step(pb, a0, b0);
+  while (pb < P) {
+    const int nx = pb + 4 * KS;
+    if (nx < P) load_step(nx, a1, b1);
+    mma_step(a0, b0);
+    pb = nx;
+    if (pb >= P) break;
+    const int nx2 = pb + 4 * KS;
+    if (nx2 < P) load_step(nx2, a0, b0);
+    mma_step(a1, b1);
+    pb = nx2;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        red[wave][(16 * i + Mfma<TI>::row(lane, r)) * 32 + 16 * j + r16] = double(acc[i][j][r]);
+  __syncthreads();
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int s = s0 + (e >> 5), k = k0 + (e & 31);
+    if (s < ns && k < K2) F[(size_t)s * K2 + k] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+  }
 }
 
 static int ensure(void** p, size_t* have, size_t need) {
@@ -180,12 +257,15 @@ static int ensure(void** p, size_t* have, size_t need) {
 
 int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
   const int P = c.nproj;
-  const int K = 2 * c.g.ncp;
-  const int tiles = ((P + BM - 1) / BM) * ((ns + BN - 1) / BN);
+  const int K = 2 * c.g.ncp;  // multiple of 128
+  const int tiles = ((P + 31) / 32) * ((ns + 31) / 32);
+  const int nsteps = K / KS;
+  // about 512 workgroups, at least 16 K steps (four per wave) per split
   int nsplit = (512 + tiles - 1) / tiles;
-  const int kmax = (K + 63) / 64;
-  if (nsplit > kmax) nsplit = kmax;
+  if (nsplit > nsteps / 16) nsplit = nsteps / 16;
   if (nsplit < 1) nsplit = 1;
+  const int per = (nsteps + nsplit - 1) / nsplit;
+  nsplit = (nsteps + per - 1) / per;
   int rc = ensure(reinterpret_cast<void**>(&c.zpart), &c.zpart_bytes,
                   sizeof(double) * (size_t)nsplit * P * ns);
   if (rc) return rc;
@@ -201,16 +281,16 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
     }
     c.Y_bytes = sizeof(double) * (size_t)P * ns;
   }
-  int used;
+  dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
   if (prec == 0)
-    used = gemm<double, double, double>(c, P, ns, K, nsplit, c.A64, K, 1, X, 1, K, c.zpart, ns, 1,
-                                        (long)P * ns);
+    hipLaunchKernelGGL((k_kb_project<double, double>), grid, dim3(256), 0, c.stream, c.A64, X, P,
+                       ns, K, per, c.zpart);
   else
-    used = gemm<float, double, float>(c, P, ns, K, nsplit, c.A32, K, 1, X, 1, K, c.zpart, ns, 1,
-                                      (long)P * ns);
-  if (used < 0) return used;
+    hipLaunchKernelGGL((k_kb_project<float, float>), grid, dim3(256), 0, c.stream, c.A32, X, P, ns,
+                       K, per, c.zpart);
+  SDFT_HIP_CHECK(hipGetLastError());
   const int tot = P * ns;
-  hipLaunchKernelGGL(k_nl_reduce, dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.zpart, used,
+  hipLaunchKernelGGL(k_nl_reduce, dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
                      P, ns, c.E, c.Z, c.Y);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
@@ -219,12 +299,15 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
 int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
   const int P = c.nproj;
   const int K2 = 2 * c.g.ncp;
-  int used;
+  dim3 grid((ns + 31) / 32, (K2 + 31) / 32);
   if (prec == 0)
-    used = gemm<double, double, double>(c, ns, K2, P, 1, c.Y, 1, ns, c.A64, K2, 1, fnl, K2, 1, 0);
+    hipLaunchKernelGGL((k_kb_force<double, double>), grid, dim3(256), 0, c.stream, c.Y, c.A64, P,
+                       ns, K2, fnl);
   else
-    used = gemm<double, float, float>(c, ns, K2, P, 1, c.Y, 1, ns, c.A32, K2, 1, fnl, K2, 1, 0);
-  return used < 0 ? used : 0;
+    hipLaunchKernelGGL((k_kb_force<float, float>), grid, dim3(256), 0, c.stream, c.Y, c.A32, P, ns,
+                       K2, fnl);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 }  // namespace sdft
