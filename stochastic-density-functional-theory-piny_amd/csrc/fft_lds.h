@@ -230,6 +230,137 @@ SDFT_DEV inline void fft_lines(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, 
   if constexpr (P > 5) fft_pass<T, N, DIR, (P > 5 ? 5 : 0), MAXLINES, NT>(buf, nlines, ld, tw, tid);
 }
 
+// ---- passes with caller-supplied first loads / last stores ---------------
+// Radix order of a transform: standard (radix_at) or reversed.  A forward
+// transform in reversed order can start where an inverse transform in
+// standard order ends: the inverse's last pass writes positions j + r N/R,
+// exactly the inputs of the reversed forward's first butterfly j.
+constexpr int radix_sched(int N, int i, bool rev) {
+  return rev ? radix_at(N, n_passes(N) - 1 - i) : radix_at(N, i);
+}
+constexpr int ns_sched(int N, int i, bool rev) {
+  int ns = 1;
+  for (int k = 0; k < i; ++k) ns *= radix_sched(N, k, rev);
+  return ns;
+}
+
+// One Stockham pass over nlines <= MAXLINES lines.  Inputs come from
+// lf(line, pos), outputs go to sf(line, pos, value); a barrier separates all
+// loads from all stores, and one follows the stores, so lf and sf may address
+// the same LDS buffer.  LINE_FAST maps consecutive threads to consecutive
+// lines (same butterfly), otherwise to consecutive butterflies of a line.
+template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
+          class LF, class SF>
+SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
+  constexpr int R = radix_sched(N, PASS, REV);
+  constexpr int NB = N / R;
+  constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
+  constexpr int NS = ns_sched(N, PASS, REV);
+  constexpr int STEP = N / (NS * R);
+  const int total = nlines * NB;
+  Cx<T> v[MAXB][R];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = LINE_FAST ? idx % nlines : idx / NB;
+      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = lf(line, j + r * NB);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = LINE_FAST ? idx % nlines : idx / NB;
+      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const Cx<T> w = tw[r * k * STEP];
+          v[b][r] = DIR > 0 ? cmul(v[b][r], w) : cmulc(v[b][r], w);
+        }
+      }
+      Dft<T, R, DIR>::run(v[b]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) sf(line, base + r * NS, v[b][r]);
+    }
+  }
+  __syncthreads();
+}
+
+// Passes FIRST..LAST (inclusive) of one transform: the first loads through
+// lf0, the last stores through sf1, the ones between go through the LDS lines
+// buf + line * ld.
+template <typename T, int N, int DIR, bool REV, int FIRST, int LAST, int MAXLINES, int NT,
+          bool LINE_FAST, class LF, class SF>
+SDFT_DEV inline void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* tw, int tid,
+                                   LF& lf0, SF& sf1) {
+  auto lb = [&](int line, int pos) { return buf[line * ld + pos]; };
+  auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + pos] = x; };
+  if constexpr (FIRST == LAST) {
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lf0, sf1);
+  } else {
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lf0, sb);
+    if constexpr (FIRST + 1 < LAST) {
+      fft_passes_io<T, N, DIR, REV, FIRST + 1, LAST - 1, MAXLINES, NT, LINE_FAST>(
+          buf, ld, nlines, tw, tid, lb, sb);
+    }
+    fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lb, sf1);
+  }
+}
+
+// Last pass of an inverse transform (standard order), a real multiplier
+// vf(line, pos) at every output position, and the first pass of the forward
+// transform in reversed order, in registers.  Outputs land at j*R + r.
+template <typename T, int N, int MAXLINES, int NT, bool LINE_FAST, class LF, class VF, class SF>
+SDFT_DEV inline void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& vf,
+                                    SF& sf) {
+  constexpr int P = n_passes(N);
+  constexpr int R = radix_at(N, P - 1);
+  constexpr int NB = N / R;
+  constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
+  const int total = nlines * NB;
+  Cx<T> v[MAXB][R];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = LINE_FAST ? idx % nlines : idx / NB;
+      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = lf(line, j + r * NB);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int idx = tid + b * NT;
+    if (idx < total) {
+      const int line = LINE_FAST ? idx % nlines : idx / NB;
+      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      if (P > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * j]);
+      }
+      Dft<T, R, +1>::run(v[b]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T s = vf(line, j + r * NB);
+        v[b][r] = cx(v[b][r].x * s, v[b][r].y * s);
+      }
+      Dft<T, R, -1>::run(v[b]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) sf(line, j * R + r, v[b][r]);
+    }
+  }
+  __syncthreads();
+}
+
 // Any number of lines: batches of MAXLINES.
 template <typename T, int N, int DIR, int MAXLINES, int NT>
 SDFT_DEV inline void fft_lines_batched(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {

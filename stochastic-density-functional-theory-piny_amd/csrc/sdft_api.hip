@@ -214,11 +214,13 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
   }
   // entries per column (for the pack), coefficient positions (for the unpack)
   std::vector<int> ent_off{0}, ent_j, ent_b, col_pos(g.ncol);
+  std::vector<int> rowcol((size_t)g.M * n, -1);
   std::vector<int> coef_plus(nc), coef_minus(nc);
   const int LD = n + 1;
   for (int dd = 0; dd < g.ncol; ++dd) {
     const int b = order[dd].first, cc = order[dd].second;
     col_pos[dd] = (b + K) * LD + ((cc % n) + n) % n;
+    rowcol[(size_t)(b + K) * n + ((cc % n) + n) % n] = dd;
     for (auto& e : cols[order[dd]]) {
       const int a = std::get<0>(e), i = std::get<1>(e), flag = std::get<2>(e);
       const int ia = ((a % n) + n) % n;
@@ -244,6 +246,7 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
   }
   int rc;
   if ((rc = upload(&c.t.col_pos, col_pos))) return rc;
+  if ((rc = upload(&c.t.rowcol, rowcol))) return rc;
   if ((rc = upload(&c.t.ent_off, ent_off))) return rc;
   if ((rc = upload(&c.t.ent_j, ent_j))) return rc;
   if ((rc = upload(&c.t.ent_b, ent_b))) return rc;
@@ -262,7 +265,8 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
 }
 
 static void free_ctx(Ctx& c) {
-  void* ptrs[] = {c.t.col_pos,  c.t.ent_off,    c.t.ent_j,     c.t.ent_b,      c.t.chunk_col,
+  void* ptrs[] = {c.t.col_pos,  c.t.rowcol,     c.t.ent_off,   c.t.ent_j,      c.t.ent_b,
+                  c.t.chunk_col,
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
                   c.E,          c.W,            c.zpart,       c.Y,            c.Z,

@@ -56,6 +56,7 @@ struct Geometry {
 
 struct DeviceTables {
   int* col_pos = nullptr;     // [ncol] LDS offset (kb+K)*(n+1) + kc mod n in the plane buffer
+  int* rowcol = nullptr;      // [M][n] column d with (kb, kc mod n) = (row - K, pos), else -1
   int* ent_off = nullptr;     // [ncol+1] entry offsets per column
   int* ent_j = nullptr;       // [nent] internal coefficient index
   int* ent_b = nullptr;       // [nent] (d << 10) | (flag << 8) | ia ; flag 0 +G, 1 -G, 2 G=0

@@ -144,53 +144,91 @@ __global__ __launch_bounds__(kThreads) void k_expand(
 }
 
 // ---- K2: one a-plane: c and b transforms around v.psi --------------------
+// rowcol[row * N + pos] is the sphere column whose (kb, kc) is (row - K, pos)
+// (pos = kc mod N), or -1: the c-lines' first pass reads W through it and
+// their last pass writes W through it, so the plane is never zero-filled or
+// scattered.  The b-lines read and write only the 2K+1 rows |kb| <= K of the
+// c-transformed plane; their inverse last pass, the v_KS multiply and the
+// forward first pass run fused in registers.
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int ncolp, int ncol,
-                                                    const int* __restrict__ col_pos, int M,
+struct PlaneCfg {
+  static constexpr int ML = Cfg<N>::ML;
+  static constexpr int CB = largest_divisor_le(N, ML < (sizeof(T) == 4 ? 32 : 16)
+                                                      ? ML
+                                                      : (sizeof(T) == 4 ? 32 : 16));
+};
+
+template <typename T, int N>
+size_t plane2_lds_bytes(int M) {
+  return sizeof(Cx<T>) * ((size_t)M * Cfg<N>::LD + (size_t)PlaneCfg<T, N>::CB * Cfg<N>::LD + N);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int ncolp,
+                                                    const int* __restrict__ rowcol, int M,
                                                     const T* __restrict__ v,
                                                     const Cx<T>* __restrict__ twg) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = Cfg<N>::CB;
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
+  constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cx<T>* P = reinterpret_cast<Cx<T>*>(smem_raw);
-  Cx<T>* Q = P + M * LD;
+  Cx<T>* Pl = reinterpret_cast<Cx<T>*>(smem_raw);
+  Cx<T>* Q = Pl + M * LD;
   Cx<T>* tw = Q + CB * LD;
   const int tid = threadIdx.x;
   const int ra = blockIdx.x;
   const int q = blockIdx.y;
   const int K = (M - 1) / 2;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
-  for (int i = tid; i < M * LD; i += kThreads) P[i] = cx(T(0), T(0));
   __syncthreads();
   Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
-  for (int d = tid; d < ncol; d += kThreads) P[col_pos[d]] = Wp[d];
-  __syncthreads();
-  fft_lines_batched<T, N, +1, ML, kThreads>(P, M, LD, tw, tid);  // along c
+  const Cx<T> zero = cx(T(0), T(0));
+
+  // inverse along c: rows of the plane, W -> LDS
+  for (int r0 = 0; r0 < M; r0 += ML) {
+    const int nl = M - r0 < ML ? M - r0 : ML;
+    const int* rc = rowcol + (size_t)r0 * N;
+    auto lw = [&](int l, int pos) {
+      const int d = rc[l * N + pos];
+      return d >= 0 ? Wp[d] : zero;
+    };
+    Cx<T>* buf = Pl + r0 * LD;
+    auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LD + pos] = x; };
+    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lw, sb);
+  }
+
+  // along b, CB columns rc at a time
   const T* vplane = v + (size_t)ra * N * N;
   for (int rc0 = 0; rc0 < N; rc0 += CB) {
-    for (int i = tid; i < CB * LD; i += kThreads) Q[i] = cx(T(0), T(0));
-    __syncthreads();
-    for (int e = tid; e < M * CB; e += kThreads) {
-      const int r = e / CB, l = e - r * CB;
-      Q[l * LD + wrapi(r - K, N)] = P[r * LD + rc0 + l];
-    }
-    __syncthreads();
-    fft_lines<T, N, +1, ML, kThreads>(Q, CB, LD, tw, tid);  // along b: psi(ra, rb, rc)
-    for (int e = tid; e < CB * N; e += kThreads) {
-      const int rb = e / CB, l = e - rb * CB;
-      const T vv = vplane[rb * N + rc0 + l];
-      Cx<T> z = Q[l * LD + rb];
-      Q[l * LD + rb] = cx(z.x * vv, z.y * vv);
-    }
-    __syncthreads();
-    fft_lines<T, N, -1, ML, kThreads>(Q, CB, LD, tw, tid);  // back along b
-    for (int e = tid; e < M * CB; e += kThreads) {
-      const int r = e / CB, l = e - r * CB;
-      P[r * LD + rc0 + l] = Q[l * LD + wrapi(r - K, N)];
-    }
-    __syncthreads();
+    auto lp = [&](int l, int pos) {
+      const int kb = pos <= N / 2 ? pos : pos - N;
+      return (kb >= -K && kb <= K) ? Pl[(kb + K) * LD + rc0 + l] : zero;
+    };
+    auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
+    auto sq = [&](int l, int pos, Cx<T> x) { Q[l * LD + pos] = x; };
+    auto vf = [&](int l, int pos) { return vplane[pos * N + rc0 + l]; };
+    auto sp = [&](int l, int pos, Cx<T> x) {
+      const int kb = pos <= N / 2 ? pos : pos - N;
+      if (kb >= -K && kb <= K) Pl[(kb + K) * LD + rc0 + l] = x;
+    };
+    if constexpr (P >= 2)
+      fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true>(Q, LD, CB, tw, tid, lp, sq);
+    fft_fused_vmul<T, N, CB, kThreads, true>(CB, tw, tid, lq, vf, sq);
+    if constexpr (P >= 2)
+      fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true>(Q, LD, CB, tw, tid, lq, sp);
   }
-  fft_lines_batched<T, N, -1, ML, kThreads>(P, M, LD, tw, tid);  // back along c
-  for (int d = tid; d < ncol; d += kThreads) Wp[d] = P[col_pos[d]];
+
+  // forward along c: LDS -> W
+  for (int r0 = 0; r0 < M; r0 += ML) {
+    const int nl = M - r0 < ML ? M - r0 : ML;
+    const int* rc = rowcol + (size_t)r0 * N;
+    Cx<T>* buf = Pl + r0 * LD;
+    auto lb = [&](int l, int pos) { return buf[l * LD + pos]; };
+    auto sw = [&](int l, int pos, Cx<T> x) {
+      const int d = rc[l * N + pos];
+      if (d >= 0) Wp[d] = x;
+    };
+    fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lb, sw);
+  }
 }
 
 // ---- K3: forward FFT along a + unpack + force / recurrence ---------------
@@ -452,7 +490,7 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
 template <typename T, int N>
 static int plane_impl(Ctx& c, void* W, int npairs, const T* v, const void* tw) {
   const Geometry& g = c.g;
-  const size_t lds = plane_lds_bytes<T, N>(g.M);
+  const size_t lds = plane2_lds_bytes<T, N>(g.M);
   if (lds > 160 * 1024) {
     set_error("plane buffer exceeds LDS for this grid in this precision");
     return SDFT_EINVAL_CODE;
@@ -461,7 +499,7 @@ static int plane_impl(Ctx& c, void* W, int npairs, const T* v, const void* tw) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid(N, npairs);
   hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream,
-                     reinterpret_cast<Cx<T>*>(W), g.ncolp, g.ncol, c.t.col_pos, g.M, v,
+                     reinterpret_cast<Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M, v,
                      reinterpret_cast<const Cx<T>*>(tw));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
