@@ -161,6 +161,9 @@ def main():
     e64 = band_energy(results["fp64"]["filtered"], rho64)
     e32 = band_energy(results["fp32"]["filtered"], rho32)
     de_atom = abs(e32 - e64) / sysm.natm
+    f64 = ctx.nl_forces(results["fp64"]["filtered"], 2.0, nst)
+    f32 = ctx.nl_forces(results["fp32"]["filtered"], 2.0, nst)
+    df = float(np.max(np.abs(f32 - f64)))
     log("FP32 vs FP64: density L1/e %.3e, energy/atom %.3e Ha (E64 %.6f, Ne %.6f)" % (
         dens_l1, de_atom, e64, ne64))
 
@@ -219,7 +222,7 @@ def main():
         "fp64": {"value": round(r64["hpsi_per_s"], 1), "ms_per_step": round(r64["ms_per_step"], 3)},
         "fp32_over_fp64": round(r32["hpsi_per_s"] / r64["hpsi_per_s"], 3),
         "fp32_error": {"density_l1_per_electron": dens_l1, "energy_per_atom_ha": de_atom,
-                       "force_ha_per_bohr": None},
+                       "nl_force_ha_per_bohr": df, "nl_force_max_ha_per_bohr": float(np.abs(f64).max())},
         "survey_model_hbm_frac": round(survey_frac32, 4),
         "kernels_fp32_ms": {k: round(v[0] / v[1], 4) for k, v in r32["kernels"].items()},
         "kernels_fp64_ms": {k: round(v[0] / v[1], 4) for k, v in r64["kernels"].items()},

@@ -124,6 +124,17 @@ int sdft_hip_density(sdft_ctx *ctx, int nstate, const double *re, const double *
 int sdft_hip_energies(sdft_ctx *ctx, int nstate, const double *re, const double *im,
                       double occ, double nsto, double *eke, double *enl);
 
+/* Hellmann-Feynman nonlocal forces on the atoms (Ha/bohr) from nstate
+ * orbitals: F_a = -(occ/nsto) sum_s sum_{p in a} 2 E_p Z_ps dZ_ps/dR_a with
+ * dQ_p/dR_a = i G Q_p (calcEnergyForce's G-space NL branch,
+ * stodft/calc-nuclei-force.c:302-489, scaled as :462-478).  gvec: [ncoef][3]
+ * cartesian G (G=0 last); proj_atom: [nproj] atom of each projector;
+ * forces: [natm][3].  The projections and their derivatives are FP64 MFMA
+ * GEMMs on the device. */
+int sdft_hip_nl_forces(sdft_ctx *ctx, int nstate, const double *re, const double *im,
+                       const double *gvec, const int *proj_atom, int natm, double occ,
+                       double nsto, double *forces);
+
 /* ---- device-resident blocks (orbitals stay in HBM across calls) ---------- */
 
 /* Allocate device storage for nstate orbitals (T_k, T_{k-1}, accumulator,

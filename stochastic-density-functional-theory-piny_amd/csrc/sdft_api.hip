@@ -273,6 +273,9 @@ static void free_ctx(Ctx& c) {
                   c.red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c.ev_cur) (void)hipEventDestroy(c.ev_cur);
+  if (c.ev_nl) (void)hipEventDestroy(c.ev_nl);
+  if (c.stream_nl) (void)hipStreamDestroy(c.stream_nl);
   if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -432,15 +435,30 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
   int rc;
   if ((rc = ensure_W(c, w_bytes(c, npairs, b.prec)))) return rc;
   const double* fnl = nullptr;
+  // Without profiling the KB chain runs on its own stream, overlapping the
+  // FFT kernels; with profiling everything stays on one stream so that the
+  // per-kernel event times are not inflated by contention.
+  const bool overlap = c.nproj > 0 && !c.profiling;
   if (c.nproj > 0) {
+    hipStream_t main = c.stream;
+    if (overlap) {
+      SDFT_HIP_CHECK(hipEventRecord(c.ev_cur, c.stream));
+      SDFT_HIP_CHECK(hipStreamWaitEvent(c.stream_nl, c.ev_cur, 0));
+      c.stream = c.stream_nl;
+    }
     {
       ProfScope ps(c, "kb_project");
-      if ((rc = launch_nl_project(c, b.prec, cur, b.nstate_pad))) return rc;
+      rc = launch_nl_project(c, b.prec, cur, b.nstate_pad);
     }
-    {
+    if (!rc) {
       ProfScope ps(c, "kb_force");
-      if ((rc = launch_nl_force(c, b.prec, b.nstate_pad, b.fnl))) return rc;
+      rc = launch_nl_force(c, b.prec, b.nstate_pad, b.fnl);
     }
+    if (overlap) {
+      c.stream = main;
+      SDFT_HIP_CHECK(hipEventRecord(c.ev_nl, c.stream_nl));
+    }
+    if (rc) return rc;
     fnl = b.fnl;
   }
   {
@@ -451,6 +469,7 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
     ProfScope ps(c, "plane_bc_vpsi");
     if ((rc = launch_plane(c, b.prec, c.W, npairs))) return rc;
   }
+  if (overlap) SDFT_HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_nl, 0));
   {
     ProfScope ps(c, "contract_a");
     if ((rc = launch_contract(c, b.prec, mode, c.W, npairs, cur, out, b.acc, fnl, b.chi, sp,
@@ -566,7 +585,11 @@ int sdft_hip_init(const sdft_grid_desc* desc, sdft_ctx** out) {
     return hip_fail(e, "hipGetDevice", __FILE__, __LINE__);
   }
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream_nl, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_cur, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_nl, hipEventDisableTiming);
   if (e != hipSuccess) {
+    free_ctx(c);
     delete h;
     return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
   }
@@ -646,8 +669,62 @@ int sdft_hip_set_projectors(sdft_ctx* h, int nproj, const double* pre, const dou
   if ((rc = upload(&c.A32, A32))) return rc;
   std::vector<double> E(pe, pe + nproj);
   if ((rc = upload(&c.E, E))) return rc;
+  c.E_host = E;
   c.nproj = nproj;
   return 0;
+}
+
+int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* im,
+                       const double* gvec, const int* proj_atom, int natm, double occ,
+                       double nsto, double* forces) {
+  if (!h || nstate < 1 || !re || !im || !gvec || !proj_atom || natm < 1 || !forces ||
+      !(nsto > 0)) {
+    set_error("bad force arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  for (int a = 0; a < natm * 3; ++a) forces[a] = 0.0;
+  if (c.nproj == 0) return 0;
+  const int P = c.nproj;
+  for (int p = 0; p < P; ++p)
+    if (proj_atom[p] < 0 || proj_atom[p] >= natm) {
+      set_error("projector atom index out of range");
+      return SDFT_EINVAL;
+    }
+  Block b;
+  int rc = block_alloc(c, nstate, 0, b);
+  if (!rc) rc = block_put(b, b.X0, re, im);
+  const int ns = b.nstate_pad, ncp = c.g.ncp;
+  std::vector<double> gint((size_t)3 * ncp, 0.0);
+  for (int j = 0; j < c.g.ncoef; ++j)
+    for (int d = 0; d < 3; ++d) gint[(size_t)d * ncp + j] = gvec[(size_t)c.g.perm[j] * 3 + d];
+  double *dg = nullptr, *Ad = nullptr, *Zd = nullptr, *part = nullptr, *S = nullptr;
+  int per = 0;
+  const int nsplit = kb_project_split(c, ns, &per);
+  if (!rc) rc = upload(&dg, gint);
+  if (!rc) rc = dalloc(&Ad, sizeof(double) * (size_t)P * 2 * ncp);
+  if (!rc) rc = dalloc(&Zd, sizeof(double) * (size_t)3 * P * ns);
+  if (!rc) rc = dalloc(&part, sizeof(double) * (size_t)nsplit * P * ns);
+  if (!rc) rc = dalloc(&S, sizeof(double) * (size_t)3 * P);
+  if (!rc) rc = nl_force_sums(c, b.X0, ns, nstate, dg, Ad, Zd, part, S);
+  std::vector<double> hs((size_t)3 * P);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(hs.data(), S, sizeof(double) * 3 * P, hipMemcpyDeviceToHost,
+                                  c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) rc = hip_fail(e, "force download", __FILE__, __LINE__);
+  }
+  if (!rc) {
+    // F_a = -(occ/nsto) sum_{p in a} 2 E_p sum_s Z dZ/dR (calc-nuclei-force.c:462-478 scaling)
+    for (int p = 0; p < P; ++p)
+      for (int d = 0; d < 3; ++d)
+        forces[proj_atom[p] * 3 + d] -= 2.0 * c.E_host[p] * hs[(size_t)p * 3 + d] * occ / nsto;
+  }
+  void* ptrs[] = {dg, Ad, Zd, part, S};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  block_free(b);
+  return rc;
 }
 
 int sdft_hip_hpsi(sdft_ctx* h, int nstate, const double* cre, const double* cim, double* fre,

@@ -162,7 +162,7 @@ __global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, 
   for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * P * ns + i];
   const int p = i / ns;
   Z[i] = s;
-  Y[i] = -2.0 * E[p] * s;
+  if (Y) Y[i] = -2.0 * E[p] * s;
 }
 
 // ---- F[s][k] = sum_p Y[p][s] A[p][k] ---------------------------------------
@@ -255,17 +255,25 @@ static int ensure(void** p, size_t* have, size_t need) {
   return 0;
 }
 
-int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
+// K split of the projection GEMM: about 512 workgroups, at least 16 K steps
+// (four per wave) per split.  Returns the split count, *per = steps per split.
+int kb_project_split(const Ctx& c, int ns, int* per) {
   const int P = c.nproj;
   const int K = 2 * c.g.ncp;  // multiple of 128
   const int tiles = ((P + 31) / 32) * ((ns + 31) / 32);
   const int nsteps = K / KS;
-  // about 512 workgroups, at least 16 K steps (four per wave) per split
   int nsplit = (512 + tiles - 1) / tiles;
   if (nsplit > nsteps / 16) nsplit = nsteps / 16;
   if (nsplit < 1) nsplit = 1;
-  const int per = (nsteps + nsplit - 1) / nsplit;
-  nsplit = (nsteps + per - 1) / per;
+  *per = (nsteps + nsplit - 1) / nsplit;
+  return (nsteps + *per - 1) / *per;
+}
+
+int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
+  const int P = c.nproj;
+  const int K = 2 * c.g.ncp;
+  int per = 0;
+  const int nsplit = kb_project_split(c, ns, &per);
   int rc = ensure(reinterpret_cast<void**>(&c.zpart), &c.zpart_bytes,
                   sizeof(double) * (size_t)nsplit * P * ns);
   if (rc) return rc;
@@ -292,6 +300,71 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
   const int tot = P * ns;
   hipLaunchKernelGGL(k_nl_reduce, dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
                      P, ns, c.E, c.Z, c.Y);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Ad[p][j] = G_d(j) A[p][ncp + j], Ad[p][ncp + j] = -G_d(j) A[p][j]: the table
+// of dQ_p/dR_a = i G Q_p in the same Gamma packing (G = 0 gives zero).
+__global__ void k_kb_deriv_table(const double* __restrict__ A, const double* __restrict__ gd,
+                                 int P, int ncp, double* __restrict__ Ad) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)P * ncp) return;
+  const size_t p = i / ncp, j = i - p * ncp;
+  const double g = gd[j];
+  Ad[p * 2 * ncp + j] = g * A[p * 2 * ncp + ncp + j];
+  Ad[p * 2 * ncp + ncp + j] = -g * A[p * 2 * ncp + j];
+}
+
+// S[p*3 + d] = sum_{s < nstate} Z[p][s] Zd[d][p][s]
+__global__ void k_force_sums(const double* __restrict__ Z, const double* __restrict__ Zd, int P,
+                             int ns, int nstate, double* __restrict__ S) {
+  __shared__ double sh[3][256];
+  const int p = blockIdx.x;
+  double a[3] = {0, 0, 0};
+  for (int s = threadIdx.x; s < nstate; s += blockDim.x) {
+    const double z = Z[(size_t)p * ns + s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[d] += z * Zd[((size_t)d * P + p) * ns + s];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = a[d];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) S[p * 3 + d] = sh[d][0];
+}
+
+// dZ/dR for the three directions (FP64) and the per-projector sums; gint is
+// G in internal coefficient order [3][ncp].
+int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gint,
+                  double* Ad, double* Zd, double* part, double* S) {
+  const int P = c.nproj;
+  const int K = 2 * c.g.ncp;
+  int rc = launch_nl_project(c, 0, X, ns);  // c.Z = A X
+  if (rc) return rc;
+  int per = 0;
+  const int nsplit = kb_project_split(c, ns, &per);
+  for (int d = 0; d < 3; ++d) {
+    const size_t tot = (size_t)P * c.g.ncp;
+    hipLaunchKernelGGL(k_kb_deriv_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       c.stream, c.A64, gint + (size_t)d * c.g.ncp, P, c.g.ncp, Ad);
+    dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
+    hipLaunchKernelGGL((k_kb_project<double, double>), grid, dim3(256), 0, c.stream, Ad, X, P, ns,
+                       K, per, part);
+    // reduce the split partials into Zd[d] (the E factor is applied later)
+    const int n2 = P * ns;
+    hipLaunchKernelGGL(k_nl_reduce, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, part, nsplit,
+                       P, ns, c.E, Zd + (size_t)d * n2, (double*)nullptr);
+    SDFT_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_force_sums, dim3(P), dim3(256), 0, c.stream, c.Z, Zd, P, ns, nstate, S);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

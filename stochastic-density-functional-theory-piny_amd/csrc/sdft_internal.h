@@ -72,6 +72,12 @@ struct DeviceTables {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the KB GEMM chain of a step runs on stream_nl, concurrently with the
+  // expand / plane kernels; ev_cur orders it after the previous step, ev_nl
+  // orders the contract kernel after it
+  hipStream_t stream_nl = nullptr;
+  hipEvent_t ev_cur = nullptr;
+  hipEvent_t ev_nl = nullptr;
   Geometry g;
   DeviceTables t;
   double* vks64 = nullptr;   // [N]
@@ -81,6 +87,7 @@ struct Ctx {
   double* A64 = nullptr;     // [nproj][2 ncp]
   float* A32 = nullptr;      // [nproj][2 ncp]
   double* E = nullptr;       // [nproj]
+  std::vector<double> E_host;
   // scratch (grown on demand)
   void* W = nullptr;
   size_t W_bytes = 0;
@@ -143,6 +150,9 @@ int round_fft(int n_in);
 // ---- GEMM (sdft_gemm.hip) -----------------------------------------------
 int launch_nl_project(Ctx& c, int prec, const double* X, int nstate_pad);
 int launch_nl_force(Ctx& c, int prec, int nstate_pad, double* fnl);
+int kb_project_split(const Ctx& c, int ns, int* per);
+int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gint, double* Ad,
+                  double* Zd, double* part, double* S);
 
 // ---- Chebyshev scalars (sdft_cheby.hip) ---------------------------------
 int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,

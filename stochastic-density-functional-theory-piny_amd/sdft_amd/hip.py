@@ -39,6 +39,7 @@ EXPORTS = [
     "sdft_hip_block_create", "sdft_hip_block_destroy", "sdft_hip_block_set",
     "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
     "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
+    "sdft_hip_nl_forces",
 ]
 
 _LIB = None
@@ -81,6 +82,8 @@ def load(path: str | None = None):
         "sdft_hip_kernel_times": ([vp, C.c_int, C.POINTER(C.c_char_p), _dp, _ip], C.c_int),
         "sdft_hip_selftest_fft": ([C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp], C.c_int),
         "sdft_hip_selftest_copy": ([C.c_longlong, C.c_int], C.c_int),
+        "sdft_hip_nl_forces": ([vp, C.c_int, _dp, _dp, _dp, _ip, C.c_int, C.c_double, C.c_double,
+                                _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -222,6 +225,17 @@ class Context:
         _check(self.lib.sdft_hip_energies(self.h, re.shape[0], _ptr(re), _ptr(im), occ, nsto,
                                           C.byref(eke), C.byref(enl)))
         return eke.value, enl.value
+
+    def nl_forces(self, Cz, occ, nsto):
+        """Hellmann-Feynman nonlocal forces (natm, 3) of the orbitals Cz."""
+        s = self.sys
+        re, im = _split(Cz)
+        g = np.ascontiguousarray(s.gvec, dtype=np.float64)
+        pa = np.ascontiguousarray(s.proj_atom, dtype=np.int32)
+        out = np.zeros((s.natm, 3))
+        _check(self.lib.sdft_hip_nl_forces(self.h, re.shape[0], _ptr(re), _ptr(im), _ptr(g),
+                                           pa.ctypes.data_as(_ip), s.natm, occ, nsto, _ptr(out)))
+        return out
 
     def set_profiling(self, on: bool):
         _check(self.lib.sdft_hip_set_profiling(self.h, 1 if on else 0))

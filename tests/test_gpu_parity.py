@@ -178,3 +178,18 @@ def test_against_golden_fixture(hip, si8_ctx, si8_system):
     assert rel(rho[::16], g["rho_sub"]) < 1e-11
     eke, enl = si8_ctx.energies(g["acc"], 2.0, 4)
     assert np.allclose([eke, enl], g["energies"], rtol=1e-12)
+
+
+def test_nl_forces_fp64(hip, si8_ctx, si8_system):
+    """HF nonlocal forces on the GPU vs the oracle (itself pinned by finite
+    differences of E_NL in tests/test_oracle.py)."""
+    from sdft_amd import systems
+    say("case: NL forces si8 B=6")
+    C = systems.noise_orbitals(si8_system, 6, seed=8) * 0.2
+    F = si8_ctx.nl_forces(C, 2.0, 6)
+    F_ref = O.nl_forces(C, si8_system, 2.0, 6)
+    assert np.max(np.abs(F - F_ref)) < 1e-10 * np.max(np.abs(F_ref))
+    # translation invariance is not expected (local + NL); the sum over
+    # projectors of one atom is what the oracle pins.  Also an odd state count:
+    F5 = si8_ctx.nl_forces(C[:5], 2.0, 5)
+    assert np.max(np.abs(F5 - O.nl_forces(C[:5], si8_system, 2.0, 5))) < 1e-10 * np.max(np.abs(F_ref))
