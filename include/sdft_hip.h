@@ -135,6 +135,16 @@ int sdft_hip_nl_forces(sdft_ctx *ctx, int nstate, const double *re, const double
                        const double *gvec, const int *proj_atom, int natm, double occ,
                        double nsto, double *forces);
 
+/* Stochastic noise orbitals on the device: genNoiseOrbitalReal,
+ * stodft/gen-noise.c:162-374 -- real-space values +-sqrt(N) (:186-195)
+ * transformed to the coefficient set, c(G) = (1/N) sum_r chi(r) e^{-iG.r},
+ * G=0 real.  The sign of grid point r (FFTW linear index) of orbital
+ * first + s is the top bit of splitmix64(seed << 44 | (first + s) << 23 | r)
+ * (a counter-based stream replacing the reference's gaussran2 seeding;
+ * seed < 2^20, orbitals < 2^21, n^3 < 2^23).  Output in the reference order. */
+int sdft_hip_noise(sdft_ctx *ctx, int nstate, long long seed, long long first, double *cre,
+                   double *cim);
+
 /* ---- device-resident blocks (orbitals stay in HBM across calls) ---------- */
 
 /* Allocate device storage for nstate orbitals (T_k, T_{k-1}, accumulator,
@@ -143,6 +153,8 @@ int sdft_hip_block_create(sdft_ctx *ctx, int nstate, int prec, sdft_block **out)
 int sdft_hip_block_destroy(sdft_block *blk);
 /* Host -> device copy of chi (reference order, FP64 re/im arrays). */
 int sdft_hip_block_set(sdft_block *blk, const double *cre, const double *cim);
+/* chi generated on the device (as sdft_hip_noise), no host transfer. */
+int sdft_hip_block_noise(sdft_block *blk, long long seed, long long first);
 /* Filter on resident data (no host transfers); moments nullable (L doubles). */
 int sdft_hip_block_filter(sdft_block *blk, int L, const double *coeffs, double scale,
                           double emean, double *moments);

@@ -507,6 +507,28 @@ __global__ void k_scale_copy(const double* __restrict__ x, double a, double* __r
   if (i < n) y[i] = a * x[i];
 }
 
+// genNoiseOrbitalReal on the device: block X0 and chi receive the noise.
+static int block_noise_impl(Block& b, long long seed, long long first) {
+  Ctx& c = *b.ctx;
+  if (seed < 0 || seed >= (1LL << 20) || first < 0 || first + b.nstate_pad > (1LL << 21) ||
+      c.g.N > (1LL << 23)) {
+    set_error("noise key out of range (seed < 2^20, orbitals < 2^21, grid < 2^23)");
+    return SDFT_EINVAL;
+  }
+  const int npairs = b.nstate_pad / 2;
+  int rc = ensure_W(c, w_bytes(c, npairs, 0));
+  if (!rc) rc = launch_noise_plane(c, c.W, npairs, (unsigned long long)seed, (int)first, b.nstate);
+  StepParams sp{};
+  if (!rc)
+    rc = launch_contract(c, 0, kModeNoise, c.W, npairs, b.X0, b.X0, b.chi, nullptr, nullptr, sp,
+                         nullptr);
+  if (!rc) {
+    b.cur = 0;
+    SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  return rc;
+}
+
 // The Chebyshev filter on resident data (filter.c:318-706).
 static int block_filter_impl(Block& b, int L, const double* coeffs, double scale, double emean,
                              double* moments) {
@@ -944,6 +966,28 @@ int sdft_hip_block_set(sdft_block* bb, const double* cre, const double* cim) {
   SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
   SDFT_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
   return 0;
+}
+
+int sdft_hip_noise(sdft_ctx* h, int nstate, long long seed, long long first, double* cre,
+                   double* cim) {
+  if (!h || nstate < 1 || !cre || !cim) {
+    set_error("bad noise arguments");
+    return SDFT_EINVAL;
+  }
+  Block b;
+  int rc = block_alloc(h->c, nstate, 0, b);
+  if (!rc) rc = block_noise_impl(b, seed, first);
+  if (!rc) rc = block_fetch(b, b.X0, cre, cim);
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_block_noise(sdft_block* bb, long long seed, long long first) {
+  if (!bb) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  return block_noise_impl(bb->b, seed, first);
 }
 
 int sdft_hip_block_filter(sdft_block* bb, int L, const double* coeffs, double scale, double emean,

@@ -129,7 +129,9 @@ struct StepParams {
   int step;           // step index (moment partial row)
 };
 
-enum ContractMode { kModeForce = 0, kModeFilter = 1, kModeMoments = 2 };
+// kModeNoise writes the transformed noise to Xout and to acc (the caller
+// passes the block's chi buffer as acc).
+enum ContractMode { kModeForce = 0, kModeFilter = 1, kModeMoments = 2, kModeNoise = 3 };
 
 // ---- launchers (sdft_kernels.hip) ----------------------------------------
 size_t w_bytes(const Ctx& c, int npairs, int prec);
@@ -139,6 +141,8 @@ int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
                     const double* Xcur, double* Xout, double* acc, const double* fnl,
                     const double* chi, const StepParams& sp, double* mom_part);
 int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups);
+int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
+                       int nstate);
 int launch_rho_finish(Ctx& c, const double* rho_part, int ngroups, double scale,
                       int layout, int accumulate, double* rho);
 int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in, double* out,

@@ -231,6 +231,84 @@ __global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int n
   }
 }
 
+// ---- noise: real-space Rademacher -> sphere columns ----------------------
+// genNoiseOrbitalReal (stodft/gen-noise.c:162-374): real-space values
+// +-sqrt(N) (ranValue = sqrt(nfft2), :186-195), transformed to G space.  The
+// sign of grid point r of orbital o is the top bit of splitmix64(seed << 44 |
+// o << 23 | r) (r the FFTW linear index), the same counter-based stream as the
+// host system builder.  Two orbitals share one complex field z = chi1 + i chi2.
+__device__ inline unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ inline double rademacher(unsigned long long seed, unsigned long long orb,
+                                    unsigned long long r) {
+  const unsigned long long h = splitmix64((seed << 44) | (orb << 23) | r);
+  return (h >> 63) ? -1.0 : 1.0;
+}
+
+template <int N>
+size_t noise_lds_bytes(int M) {
+  return sizeof(Cx<double>) * ((size_t)Cfg<N>::ML * Cfg<N>::LD + (size_t)M * Cfg<N>::LD + N);
+}
+
+// per (pair, a-plane): forward FFT along c of the generated rows (keeping
+// |kc| <= K), then along b (keeping |kb| <= K), into W's sphere columns.
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict__ W, int ncolp,
+                                                          const int* __restrict__ rowcol, int M,
+                                                          const Cx<double>* __restrict__ twg,
+                                                          unsigned long long seed, int first,
+                                                          int nstate) {
+  typedef double T;
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML;
+  constexpr int P = n_passes(N);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* Rb = reinterpret_cast<Cx<T>*>(smem_raw);  // ML rows (rb) of the plane
+  Cx<T>* Bk = Rb + ML * LD;                        // M lines (kc) along b
+  Cx<T>* tw = Bk + M * LD;
+  const int tid = threadIdx.x;
+  const int ra = blockIdx.x;
+  const int q = blockIdx.y;
+  const int K = (M - 1) / 2;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  __syncthreads();
+  const int o1 = 2 * q, o2 = 2 * q + 1;
+  const double amp = sqrt((double)N * N * N);
+  const bool has2 = o2 < nstate;
+  for (int rb0 = 0; rb0 < N; rb0 += ML) {
+    const int nl = N - rb0 < ML ? N - rb0 : ML;
+    auto gen = [&](int l, int pos) {
+      const unsigned long long r = ((unsigned long long)ra * N + rb0 + l) * N + pos;
+      const double x1 = amp * rademacher(seed, (unsigned long long)(first + o1), r);
+      const double x2 = has2 ? amp * rademacher(seed, (unsigned long long)(first + o2), r) : 0.0;
+      return cx(x1, x2);
+    };
+    auto sk = [&](int l, int pos, Cx<T> x) {
+      const int kc = pos <= N / 2 ? pos : pos - N;
+      if (kc >= -K && kc <= K) Bk[(kc + K) * LD + rb0 + l] = x;
+    };
+    fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(Rb, LD, nl, tw, tid, gen, sk);
+  }
+  Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+  for (int c0 = 0; c0 < M; c0 += ML) {
+    const int nl = M - c0 < ML ? M - c0 : ML;
+    Cx<T>* buf = Bk + c0 * LD;
+    auto lb = [&](int l, int pos) { return buf[l * LD + pos]; };
+    auto sw = [&](int l, int pos, Cx<T> x) {
+      const int kb = pos <= N / 2 ? pos : pos - N;
+      if (kb < -K || kb > K) return;
+      const int kc = c0 + l - K;
+      const int d = rowcol[(kb + K) * N + (kc < 0 ? kc + N : kc)];
+      if (d >= 0) Wp[d] = x;
+    };
+    fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lb, sw);
+  }
+}
+
 // ---- K3: forward FFT along a + unpack + force / recurrence ---------------
 template <typename T, int N>
 __global__ __launch_bounds__(kThreads) void k_contract(
@@ -282,6 +360,20 @@ __global__ __launch_bounds__(kThreads) void k_contract(
       f[1][1] = 0.0;
     }
     const double k2 = ak2[j];
+    if (mode == kModeNoise) {
+      // c(G) = (1/N) sum_r chi(r) e^{-iG.r} of the two real fields: minus a
+      // quarter (G != 0) / half (G = 0) of the unpack
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const size_t base = (size_t)(2 * q + o) * ldx;
+        const double s = g0 ? -0.5 : -0.25;
+        Xout[base + j] = s * f[o][0];
+        Xout[base + ncp + j] = s * f[o][1];
+        acc[base + j] = s * f[o][0];
+        acc[base + ncp + j] = s * f[o][1];
+      }
+      continue;
+    }
 #pragma unroll
     for (int o = 0; o < 2; ++o) {
       const size_t base = (size_t)(2 * q + o) * ldx;
@@ -565,6 +657,31 @@ static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int
 int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups) {
   SDFT_SIZE_SWITCH(c.g.n, return (density_impl<double, NN>(c, W, npairs, rho_part, ngroups,
                                                            c.t.tw64)));
+  return 0;
+}
+
+template <int N>
+static int noise_impl(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
+                      int nstate) {
+  const Geometry& g = c.g;
+  const size_t lds = noise_lds_bytes<N>(g.M);
+  if (lds > 160 * 1024) {
+    set_error("noise plane buffer exceeds LDS for this grid");
+    return SDFT_EINVAL_CODE;
+  }
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_noise_plane<N>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid(N, npairs);
+  hipLaunchKernelGGL((k_noise_plane<N>), grid, dim3(kThreads), lds, c.stream,
+                     reinterpret_cast<Cx<double>*>(W), g.ncolp, c.t.rowcol, g.M,
+                     reinterpret_cast<const Cx<double>*>(c.t.tw64), seed, first, nstate);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
+                       int nstate) {
+  SDFT_SIZE_SWITCH(c.g.n, return (noise_impl<NN>(c, W, npairs, seed, first, nstate)));
   return 0;
 }
 

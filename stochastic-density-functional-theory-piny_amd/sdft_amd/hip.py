@@ -39,7 +39,7 @@ EXPORTS = [
     "sdft_hip_block_create", "sdft_hip_block_destroy", "sdft_hip_block_set",
     "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
     "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
-    "sdft_hip_nl_forces",
+    "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
 ]
 
 _LIB = None
@@ -84,6 +84,8 @@ def load(path: str | None = None):
         "sdft_hip_selftest_copy": ([C.c_longlong, C.c_int], C.c_int),
         "sdft_hip_nl_forces": ([vp, C.c_int, _dp, _dp, _dp, _ip, C.c_int, C.c_double, C.c_double,
                                 _dp], C.c_int),
+        "sdft_hip_noise": ([vp, C.c_int, C.c_longlong, C.c_longlong, _dp, _dp], C.c_int),
+        "sdft_hip_block_noise": ([vp, C.c_longlong, C.c_longlong], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -226,6 +228,14 @@ class Context:
                                           C.byref(eke), C.byref(enl)))
         return eke.value, enl.value
 
+    def noise(self, nstate, seed, first=0):
+        """Stochastic orbitals generated on the device (genNoiseOrbitalReal)."""
+        nc = self.sys.ncoef
+        re = np.empty((nstate, nc))
+        im = np.empty((nstate, nc))
+        _check(self.lib.sdft_hip_noise(self.h, nstate, seed, first, _ptr(re), _ptr(im)))
+        return re + 1j * im
+
     def nl_forces(self, Cz, occ, nsto):
         """Hellmann-Feynman nonlocal forces (natm, 3) of the orbitals Cz."""
         s = self.sys
@@ -274,6 +284,9 @@ class Block:
     def set(self, chi):
         re, im = _split(chi)
         _check(self.ctx.lib.sdft_hip_block_set(self.h, _ptr(re), _ptr(im)))
+
+    def noise(self, seed, first=0):
+        _check(self.ctx.lib.sdft_hip_block_noise(self.h, seed, first))
 
     def filter(self, coeffs, scale, emean, moments=False):
         coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)

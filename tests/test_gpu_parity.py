@@ -193,3 +193,23 @@ def test_nl_forces_fp64(hip, si8_ctx, si8_system):
     # projectors of one atom is what the oracle pins.  Also an odd state count:
     F5 = si8_ctx.nl_forces(C[:5], 2.0, 5)
     assert np.max(np.abs(F5 - O.nl_forces(C[:5], si8_system, 2.0, 5))) < 1e-10 * np.max(np.abs(F_ref))
+
+
+@pytest.mark.parametrize("name,n,g2,nst,seed,first", [("si8", 32, 12.0, 5, 0, 0),
+                                                      ("tiny", 24, 3.0, 4, 7, 11)])
+def test_device_noise_matches_host_stream(hip, name, n, g2, nst, seed, first):
+    """genNoiseOrbitalReal on the device reproduces the host builder's
+    counter-based Rademacher noise transformed by numpy's FFT."""
+    from sdft_amd import systems
+    say(f"case: device noise {name} nstate={nst} seed={seed}")
+    s = systems.build_system(name, 10.26, 1, n, g2max=g2)
+    ctx = hip.Context(s)
+    got = ctx.noise(nst, seed, first)
+    ref = systems.noise_orbitals(s, nst, seed, first)
+    assert rel(got, ref) < 1e-12
+    # the resident block filters its own noise
+    blk = hip.Block(ctx, nst, hip.PREC_FP64)
+    blk.noise(seed, first)
+    assert rel(blk.get(1), ref) < 1e-12
+    blk.close()
+    ctx.close()
