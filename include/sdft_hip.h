@@ -161,6 +161,40 @@ int sdft_hip_block_filter(sdft_block *blk, int L, const double *coeffs, double s
 /* Device -> host copy: which = 0 filtered accumulator, 1 T_{L-1} (current). */
 int sdft_hip_block_get(sdft_block *blk, int which, double *re, double *im);
 
+/* ---- fragment batches --------------------------------------------------------
+ * fragScf (stodft/fragment/frag-scf.c:37-349, per-fragment loop :211-248)
+ * solves the fragments one after another, each a complete system with its own
+ * grid (interface-frag/init-coord-hmat-fft.c, reinitFFT.c; FFTW3D forced at
+ * parse-frag.c:211-213).  A batch takes nfrag resident blocks (one per
+ * fragment, each on its own context, all of one precision, any mix of the
+ * supported grid sizes) and runs the same Chebyshev filter on all of them
+ * together: one launch per kernel and grid size per Chebyshev step, the steps
+ * k >= 2 replayed from a captured HIP graph.  The blocks must outlive the
+ * batch and are not to be used through other calls while a batch call runs. */
+typedef struct sdft_batch sdft_batch;
+int sdft_hip_batch_create(int nfrag, sdft_block *const *blocks, sdft_batch **out);
+int sdft_hip_batch_destroy(sdft_batch *bat);
+/* filterChebyPolyHerm (stodft/filter.c:318-706) on every fragment: block f's
+ * noise copy chi is filtered with coeffs[f*L .. f*L+L) and its own scale /
+ * emean (scale[f] = 2/(E_max - E_min), emean[f] = (E_min + E_max)/2); the
+ * result is in each block's accumulator (sdft_hip_block_get, which = 0).
+ * use_graph = 0 launches every step directly (same kernels and order).
+ * L >= 2. */
+int sdft_hip_batch_filter(sdft_batch *bat, int L, const double *coeffs, const double *scale,
+                          const double *emean, int use_graph);
+/* Autotuner: times `steps` Chebyshev steps (after one warm-up step) of each
+ * candidate launch configuration -- the split of the KB projection's
+ * reduction (1, 2, 4 or 8 slices) and whether the KB products overlap the FFT
+ * kernels on a second stream -- on the batch's current data and keeps the
+ * fastest.  Writes the chosen split and overlap flag and the per-candidate
+ * step times (ms; ncand entries, at most max_cand) when the pointers are
+ * non-null.  Returns the number of candidates timed.  Leaves the blocks'
+ * T_k buffers modified (the accumulators are untouched). */
+int sdft_hip_batch_autotune(sdft_batch *bat, int steps, int *split_out, int *overlap_out,
+                            int max_cand, double *cand_ms);
+/* Set the launch configuration by hand (split in {1,2,4,8}, overlap 0/1). */
+int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap);
+
 /* Per-kernel timing of the last block_filter call when profiling is on:
  * names[i] / ms[i] / launches[i] for up to max entries; returns the count. */
 int sdft_hip_set_profiling(sdft_ctx *ctx, int on);

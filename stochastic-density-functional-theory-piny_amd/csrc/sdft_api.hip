@@ -12,12 +12,6 @@
 #include "../../include/sdft_hip.h"
 #include "sdft_internal.h"
 
-struct sdft_ctx {
-  sdft::Ctx c;
-};
-struct sdft_block {
-  sdft::Block b;
-};
 
 namespace sdft {
 
@@ -131,8 +125,8 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
     }
     K = std::max(K, std::max(std::abs(ka[i]), std::max(std::abs(kb[i]), std::abs(kc[i]))));
   }
-  if (2 * K + 1 > plane_rows_max_for(n) || 2 * K >= n) {
-    set_error("plane-wave sphere too large for the grid (need 2K+1 <= 3n/8+1)");
+  if (2 * K >= n) {
+    set_error("plane-wave sphere too large for the grid (need 2K+1 <= n)");
     return SDFT_EINVAL;
   }
   g.K = K;

@@ -63,14 +63,14 @@ __device__ inline void load8(const TS* __restrict__ src, TI* dst) {
 // 32x32 output tile and interleave the split's K steps; the four partial
 // tiles are summed through LDS.
 template <typename TA, typename TI>
-__global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
-                                                    const double* __restrict__ X, int P, int ns,
-                                                    int K, int ksteps_per_split,
-                                                    double* __restrict__ zpart) {
+__device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
+                                                const double* __restrict__ X, int P, int ns,
+                                                int K, int ksteps_per_split,
+                                                double* __restrict__ zpart, int bx, int by, int bz,
+                                                double (*red)[32 * 32]) {
   typedef typename Mfma<TI>::acc_t acc_t;
-  __shared__ double red[4][32 * 32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p0 = blockIdx.x * 32, s0 = blockIdx.y * 32;
+  const int p0 = bx * 32, s0 = by * 32;
   const int g = lane >> 4, r16 = lane & 15;
   acc_t acc[2][2];
 #pragma unroll
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
-  const int step0 = blockIdx.z * ksteps_per_split;
+  const int step0 = bz * ksteps_per_split;
   const int nsteps_total = K / KS;
   const int step1 = min(nsteps_total, step0 + ksteps_per_split);
   const TA* arow[2];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
         red[wave][row * 32 + col] = double(acc[i][j][r]);
       }
   __syncthreads();
-  double* zp = zpart + (size_t)blockIdx.z * P * ns;
+  double* zp = zpart + (size_t)bz * P * ns;
   for (int e = tid; e < 32 * 32; e += 256) {
     const int row = e >> 5, col = e & 31;
     const int p = p0 + row, s = s0 + col;
@@ -152,11 +152,37 @@ __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
   }
 }
 
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
+                                                    const double* __restrict__ X, int P, int ns,
+                                                    int K, int ksteps_per_split,
+                                                    double* __restrict__ zpart) {
+  __shared__ double red[4][32 * 32];
+  kb_project_tile<TA, TI>(A, X, P, ns, K, ksteps_per_split, zpart, blockIdx.x, blockIdx.y,
+                          blockIdx.z, red);
+}
+
+// One launch over the fragments of a batch: workgroup w belongs to fragment
+// f with prefix[f] <= w < prefix[f+1]; its tile is (local % tp, local / tp %
+// ts, local / (tp ts)).
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_project_batch(const KbProjectArgs* __restrict__ args,
+                                                          const int* __restrict__ prefix,
+                                                          int nfrag) {
+  __shared__ double red[4][32 * 32];
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const KbProjectArgs a = args[f];
+  const int local = blockIdx.x - prefix[f];
+  const int bx = local % a.tp, r = local / a.tp;
+  kb_project_tile<TA, TI>(reinterpret_cast<const TA*>(a.A), a.X, a.P, a.ns, a.K, a.per, a.zpart,
+                          bx, r % a.ts, r / a.ts, red);
+}
+
 // Z[p][s] = sum_z part[z][p][s];  Y[p][s] = -2 E_p Z[p][s]
-__global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, int ns,
-                            const double* __restrict__ E, double* __restrict__ Z,
-                            double* __restrict__ Y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nl_reduce_elem(const double* __restrict__ part, int nsplit, int P,
+                                               int ns, const double* __restrict__ E,
+                                               double* __restrict__ Z, double* __restrict__ Y,
+                                               int i) {
   if (i >= P * ns) return;
   double s = 0.0;
   for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * P * ns + i];
@@ -165,20 +191,35 @@ __global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, 
   if (Y) Y[i] = -2.0 * E[p] * s;
 }
 
+__global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, int ns,
+                            const double* __restrict__ E, double* __restrict__ Z,
+                            double* __restrict__ Y) {
+  nl_reduce_elem(part, nsplit, P, ns, E, Z, Y, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_nl_reduce_batch(const KbReduceArgs* __restrict__ args,
+                                                         const int* __restrict__ prefix,
+                                                         int nfrag) {
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const KbReduceArgs a = args[f];
+  nl_reduce_elem(a.zpart, a.nsplit, a.P, a.ns, a.E, a.Z, a.Y,
+                 (blockIdx.x - prefix[f]) * 256 + threadIdx.x);
+}
+
 // ---- F[s][k] = sum_p Y[p][s] A[p][k] ---------------------------------------
 // grid (ceil(ns/32), ceil(K2/32)): one 32 (s) x 32 (k) tile per workgroup; its
 // 4 waves take interleaved 32-wide slices of the reduction index p and their
 // partial tiles are summed through LDS.  Loads are coalesced across the 16
 // lanes of a group (consecutive s for Y, consecutive k for A).
 template <typename TA, typename TI>
-__global__ __launch_bounds__(256) void k_kb_force(const double* __restrict__ Y,
-                                                  const TA* __restrict__ A, int P, int ns, int K2,
-                                                  double* __restrict__ F) {
+__device__ __forceinline__ void kb_force_tile(const double* __restrict__ Y,
+                                              const TA* __restrict__ A, int P, int ns, int K2,
+                                              double* __restrict__ F, int bx, int by,
+                                              double (*red)[32 * 32]) {
   typedef typename Mfma<TI>::acc_t acc_t;
-  __shared__ double red[4][32 * 32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int s0 = blockIdx.x * 32;
-  const int k0 = blockIdx.y * 32;
+  const int s0 = bx * 32;
+  const int k0 = by * 32;
   const int g = lane >> 4, r16 = lane & 15;
   acc_t acc[2][2];
 #pragma unroll
@@ -239,6 +280,26 @@ __global__ __launch_bounds__(256) void k_kb_force(const double* __restrict__ Y,
     const int s = s0 + (e >> 5), k = k0 + (e & 31);
     if (s < ns && k < K2) F[(size_t)s * K2 + k] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
   }
+}
+
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_force(const double* __restrict__ Y,
+                                                  const TA* __restrict__ A, int P, int ns, int K2,
+                                                  double* __restrict__ F) {
+  __shared__ double red[4][32 * 32];
+  kb_force_tile<TA, TI>(Y, A, P, ns, K2, F, blockIdx.x, blockIdx.y, red);
+}
+
+template <typename TA, typename TI>
+__global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __restrict__ args,
+                                                        const int* __restrict__ prefix,
+                                                        int nfrag) {
+  __shared__ double red[4][32 * 32];
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const KbForceArgs a = args[f];
+  const int local = blockIdx.x - prefix[f];
+  kb_force_tile<TA, TI>(a.Y, reinterpret_cast<const TA*>(a.A), a.P, a.ns, a.K2, a.F,
+                        local % a.ts, local / a.ts, red);
 }
 
 static int ensure(void** p, size_t* have, size_t need) {
@@ -379,6 +440,32 @@ int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
   else
     hipLaunchKernelGGL((k_kb_force<float, float>), grid, dim3(256), 0, c.stream, c.Y, c.A32, P, ns,
                        K2, fnl);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// kind 0: projection partials, 1: split reduction + Y, 2: force product
+int launch_kb_batch(int kind, int prec, const void* args, const int* prefix, int nfrag,
+                    int total, hipStream_t s) {
+  if (total <= 0) return 0;
+  if (kind == 0) {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_kb_project_batch<double, double>), dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbProjectArgs*>(args), prefix, nfrag);
+    else
+      hipLaunchKernelGGL((k_kb_project_batch<float, float>), dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbProjectArgs*>(args), prefix, nfrag);
+  } else if (kind == 1) {
+    hipLaunchKernelGGL(k_nl_reduce_batch, dim3(total), dim3(256), 0, s,
+                       reinterpret_cast<const KbReduceArgs*>(args), prefix, nfrag);
+  } else {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_kb_force_batch<double, double>), dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbForceArgs*>(args), prefix, nfrag);
+    else
+      hipLaunchKernelGGL((k_kb_force_batch<float, float>), dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbForceArgs*>(args), prefix, nfrag);
+  }
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -129,9 +129,99 @@ struct StepParams {
   int step;           // step index (moment partial row)
 };
 
+// ---- kernel argument blocks ----------------------------------------------
+// The FFT-sandwich kernels take one of these by value (single system) or an
+// array of them plus a workgroup prefix table (one entry per fragment of a
+// batch).  gx / gy are the per-system grid extents.
+struct ExpandArgs {
+  const double* X;
+  int ldx, ncp;
+  const int* chunk_col;
+  const int* ent_off;
+  const int* ent_j;
+  const int* ent_b;
+  const void* tw;
+  void* W;
+  int ncolp;
+  int gx;  // column chunks; gy = pairs
+};
+
+struct PlaneArgs {
+  void* W;
+  int ncolp;
+  const int* rowcol;
+  int M;
+  const void* v;
+  const void* tw;
+  int gx;  // = n (a-planes); gy = pairs
+};
+
+struct ContractArgs {
+  const void* W;
+  int ncolp;
+  const int* chunk_col;
+  const int* chunk_coef;
+  const int* coef_plus;
+  const int* coef_minus;
+  const double* ak2;
+  const void* tw;
+  const double* Xcur;
+  double* Xout;
+  double* acc;
+  const double* fnl;
+  const double* chi;
+  int ldx, ncp, ncoef;
+  double invN;
+  int mode;
+  StepParams sp;
+  double* mom_part;
+  // when non-null, c_k of the produced T_k is coeffs_dev[*step_dev] (the
+  // step counter of a replayed graph); otherwise sp.ck
+  const double* coeffs_dev;
+  const int* step_dev;
+  int gx, gy;  // column chunks, pairs
+};
+
 // kModeNoise writes the transformed noise to Xout and to acc (the caller
 // passes the block's chi buffer as acc).
 enum ContractMode { kModeForce = 0, kModeFilter = 1, kModeMoments = 2, kModeNoise = 3 };
+
+// KB GEMM arguments of one fragment of a batch (tile counts: tp over
+// projectors, ts over states, tk over the 2 ncp coefficient rows).
+struct KbProjectArgs {
+  const void* A;  // [P][K] (double or float)
+  const double* X;
+  int P, ns, K, per, nsplit, tp, ts;
+  double* zpart;
+};
+
+struct KbReduceArgs {
+  const double* zpart;
+  int nsplit, P, ns;
+  const double* E;
+  double* Z;
+  double* Y;
+};
+
+struct KbForceArgs {
+  const double* Y;
+  const void* A;
+  int P, ns, K2, ts, tk;
+  double* F;
+};
+
+// workgroup -> fragment of a batch launch: prefix[f] <= wg < prefix[f + 1]
+__device__ inline int find_frag(const int* __restrict__ prefix, int nfrag, int wg) {
+  int lo = 0, hi = nfrag;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= wg)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 // ---- launchers (sdft_kernels.hip) ----------------------------------------
 size_t w_bytes(const Ctx& c, int npairs, int prec);
@@ -147,14 +237,25 @@ int launch_rho_finish(Ctx& c, const double* rho_part, int ngroups, double scale,
                       int layout, int accumulate, double* rho);
 int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in, double* out,
                           const void* tw);
+ExpandArgs make_expand_args(const Ctx& c, int prec, const double* X, void* W);
+PlaneArgs make_plane_args(const Ctx& c, int prec, void* W);
+ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W, int npairs,
+                                const double* Xcur, double* Xout, double* acc, const double* fnl,
+                                const double* chi, const StepParams& sp, double* mom_part);
+size_t plane_lds_for(int n, int prec, int M);
+// one launch of kind 0 (expand), 1 (plane) or 2 (contract) over the fragments
+// whose argument blocks are args[0..nfrag) (device), all of grid size n
+int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
+                 int total, size_t lds, hipStream_t s);
 int chunk_cols_for(int n);
-int plane_rows_max_for(int n);
 int round_fft(int n_in);
 
 // ---- GEMM (sdft_gemm.hip) -----------------------------------------------
 int launch_nl_project(Ctx& c, int prec, const double* X, int nstate_pad);
 int launch_nl_force(Ctx& c, int prec, int nstate_pad, double* fnl);
 int kb_project_split(const Ctx& c, int ns, int* per);
+int launch_kb_batch(int kind, int prec, const void* args, const int* prefix, int nfrag,
+                    int total, hipStream_t s);
 int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gint, double* Ad,
                   double* Zd, double* part, double* S);
 
@@ -169,3 +270,11 @@ int energy_ke_device(Ctx& c, const double* X, int nstate, double* out);
 int energy_nl_device(Ctx& c, int nstate_pad, int nstate, double* out);
 
 }  // namespace sdft
+
+// The opaque handles of the C ABI.
+struct sdft_ctx {
+  sdft::Ctx c;
+};
+struct sdft_block {
+  sdft::Block b;
+};

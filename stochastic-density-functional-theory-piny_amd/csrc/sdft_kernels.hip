@@ -55,10 +55,6 @@ int chunk_cols_for(int n) {
   }
 }
 
-template <int N>
-constexpr int plane_rows_max() { return (3 * N) / 8 + 1; }
-
-int plane_rows_max_for(int n) { return (3 * n) / 8 + 1; }
 
 template <typename T, int N>
 size_t plane_lds_bytes(int M) {
@@ -97,16 +93,19 @@ __device__ inline void block_sum3(double* v, double* sh, int tid) {
 
 // ---- K1: pack + inverse FFT along a --------------------------------------
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_expand(
-    const double* __restrict__ X, int ldx, int ncp, const int* __restrict__ chunk_col,
-    const int* __restrict__ ent_off, const int* __restrict__ ent_j, const int* __restrict__ ent_b,
-    const Cx<T>* __restrict__ twg, Cx<T>* __restrict__ W, int ncolp) {
+__device__ __forceinline__ void expand_body(const ExpandArgs& a, int chunk, int q) {
   constexpr int CH = Cfg<N>::CH, LD = Cfg<N>::LD, ML = Cfg<N>::ML;
   __shared__ Cx<T> buf[CH * LD];
   __shared__ Cx<T> tw[N];
+  const double* __restrict__ X = a.X;
+  const int ldx = a.ldx, ncp = a.ncp, ncolp = a.ncolp;
+  const int* __restrict__ chunk_col = a.chunk_col;
+  const int* __restrict__ ent_off = a.ent_off;
+  const int* __restrict__ ent_j = a.ent_j;
+  const int* __restrict__ ent_b = a.ent_b;
+  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
+  Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
   const int tid = threadIdx.x;
-  const int chunk = blockIdx.x;
-  const int q = blockIdx.y;
   const int c0 = chunk_col[chunk];
   const int nc = chunk_col[chunk + 1] - c0;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
@@ -143,6 +142,21 @@ __global__ __launch_bounds__(kThreads) void k_expand(
   }
 }
 
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_expand(ExpandArgs a) {
+  expand_body<T, N>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_expand_batch(const ExpandArgs* __restrict__ args,
+                                                           const int* __restrict__ prefix,
+                                                           int nfrag) {
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const ExpandArgs a = args[f];
+  const int local = blockIdx.x - prefix[f];
+  expand_body<T, N>(a, local % a.gx, local / a.gx);
+}
+
 // ---- K2: one a-plane: c and b transforms around v.psi --------------------
 // rowcol[row * N + pos] is the sphere column whose (kb, kc) is (row - K, pos)
 // (pos = kc mod N), or -1: the c-lines' first pass reads W through it and
@@ -164,19 +178,19 @@ size_t plane2_lds_bytes(int M) {
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int ncolp,
-                                                    const int* __restrict__ rowcol, int M,
-                                                    const T* __restrict__ v,
-                                                    const Cx<T>* __restrict__ twg) {
+__device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
+  const int ncolp = a.ncolp, M = a.M;
+  const int* __restrict__ rowcol = a.rowcol;
+  const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
+  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
   Cx<T>* Pl = reinterpret_cast<Cx<T>*>(smem_raw);
   Cx<T>* Q = Pl + M * LD;
   Cx<T>* tw = Q + CB * LD;
   const int tid = threadIdx.x;
-  const int ra = blockIdx.x;
-  const int q = blockIdx.y;
   const int K = (M - 1) / 2;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
   __syncthreads();
@@ -229,6 +243,21 @@ __global__ __launch_bounds__(kThreads) void k_plane(Cx<T>* __restrict__ W, int n
     };
     fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lb, sw);
   }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_plane(PlaneArgs a) {
+  plane_body<T, N>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_plane_batch(const PlaneArgs* __restrict__ args,
+                                                          const int* __restrict__ prefix,
+                                                          int nfrag) {
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const PlaneArgs a = args[f];
+  const int local = blockIdx.x - prefix[f];
+  plane_body<T, N>(a, local % a.gx, local / a.gx);
 }
 
 // ---- noise: real-space Rademacher -> sphere columns ----------------------
@@ -311,21 +340,34 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
 
 // ---- K3: forward FFT along a + unpack + force / recurrence ---------------
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_contract(
-    const Cx<T>* __restrict__ W, int ncolp, const int* __restrict__ chunk_col,
-    const int* __restrict__ chunk_coef, const int* __restrict__ coef_plus,
-    const int* __restrict__ coef_minus, const double* __restrict__ ak2,
-    const Cx<T>* __restrict__ twg, const double* __restrict__ Xcur, double* __restrict__ Xout,
-    double* __restrict__ acc, const double* __restrict__ fnl, const double* __restrict__ chi,
-    int ldx, int ncp, int ncoef, double invN, int mode, StepParams sp,
-    double* __restrict__ mom_part) {
+__device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, int q) {
   constexpr int CH = Cfg<N>::CH, LD = Cfg<N>::LD, ML = Cfg<N>::ML;
   __shared__ Cx<T> buf[CH * LD];
   __shared__ Cx<T> tw[N];
   __shared__ double red[3 * (kThreads / 64)];
+  const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
+  const int ncolp = a.ncolp;
+  const int* __restrict__ chunk_col = a.chunk_col;
+  const int* __restrict__ chunk_coef = a.chunk_coef;
+  const int* __restrict__ coef_plus = a.coef_plus;
+  const int* __restrict__ coef_minus = a.coef_minus;
+  const double* __restrict__ ak2 = a.ak2;
+  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
+  const double* __restrict__ Xcur = a.Xcur;
+  double* __restrict__ Xout = a.Xout;
+  double* __restrict__ acc = a.acc;
+  const double* __restrict__ fnl = a.fnl;
+  const double* __restrict__ chi = a.chi;
+  const int ldx = a.ldx, ncp = a.ncp, ncoef = a.ncoef, mode = a.mode;
+  const double invN = a.invN;
+  double* __restrict__ mom_part = a.mom_part;
+  StepParams sp = a.sp;
+  if (a.coeffs_dev) {
+    const int st = *a.step_dev;
+    sp.ck = a.coeffs_dev[st];
+    sp.step = st;
+  }
   const int tid = threadIdx.x;
-  const int chunk = blockIdx.x;
-  const int q = blockIdx.y;
   const int c0 = chunk_col[chunk];
   const int nc = chunk_col[chunk + 1] - c0;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
@@ -431,12 +473,27 @@ __global__ __launch_bounds__(kThreads) void k_contract(
   if (mom_part) {
     block_sum3(dots, red, tid);
     if (tid == 0) {
-      const size_t row = (size_t)sp.step * gridDim.x * gridDim.y + (size_t)q * gridDim.x + chunk;
+      const size_t row = (size_t)sp.step * a.gx * a.gy + (size_t)q * a.gx + chunk;
       mom_part[row * 3 + 0] = dots[0];
       mom_part[row * 3 + 1] = dots[1];
       mom_part[row * 3 + 2] = dots[2];
     }
   }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_contract(ContractArgs a) {
+  contract_body<T, N>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_contract_batch(const ContractArgs* __restrict__ args,
+                                                             const int* __restrict__ prefix,
+                                                             int nfrag) {
+  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const ContractArgs a = args[f];
+  const int local = blockIdx.x - prefix[f];
+  contract_body<T, N>(a, local % a.gx, local / a.gx);
 }
 
 // ---- density: |psi1|^2 + |psi2|^2 summed over the pairs of a group -------
@@ -539,11 +596,6 @@ __global__ __launch_bounds__(kThreads) void k_fft_test(const double* __restrict_
 // launchers
 // ==========================================================================
 
-size_t w_bytes(const Ctx& c, int npairs, int prec) {
-  const size_t elem = prec == 0 ? sizeof(Cx<double>) : sizeof(Cx<float>);
-  return (size_t)npairs * c.g.n * c.g.ncolp * elem;
-}
-
 constexpr int SDFT_EINVAL_CODE = -1;
 
 #define SDFT_SIZE_SWITCH(n, CALL)          \
@@ -559,13 +611,126 @@ constexpr int SDFT_EINVAL_CODE = -1;
     default: set_error("unsupported grid size"); return SDFT_EINVAL_CODE; \
   }
 
+template <typename T>
+constexpr int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
+
+ExpandArgs make_expand_args(const Ctx& c, int prec, const double* X, void* W) {
+  ExpandArgs a;
+  a.X = X;
+  a.ldx = 2 * c.g.ncp;
+  a.ncp = c.g.ncp;
+  a.chunk_col = c.t.chunk_col;
+  a.ent_off = c.t.ent_off;
+  a.ent_j = c.t.ent_j;
+  a.ent_b = c.t.ent_b;
+  a.tw = prec == 0 ? c.t.tw64 : c.t.tw32;
+  a.W = W;
+  a.ncolp = c.g.ncolp;
+  a.gx = c.g.nchunk;
+  return a;
+}
+
+PlaneArgs make_plane_args(const Ctx& c, int prec, void* W) {
+  PlaneArgs a;
+  a.W = W;
+  a.ncolp = c.g.ncolp;
+  a.rowcol = c.t.rowcol;
+  a.M = c.g.M;
+  a.v = prec == 0 ? (const void*)c.vks64 : (const void*)c.vks32;
+  a.tw = prec == 0 ? c.t.tw64 : c.t.tw32;
+  a.gx = c.g.n;
+  return a;
+}
+
+ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W, int npairs,
+                                const double* Xcur, double* Xout, double* acc, const double* fnl,
+                                const double* chi, const StepParams& sp, double* mom_part) {
+  ContractArgs a;
+  a.W = W;
+  a.ncolp = c.g.ncolp;
+  a.chunk_col = c.t.chunk_col;
+  a.chunk_coef = c.t.chunk_coef;
+  a.coef_plus = c.t.coef_plus;
+  a.coef_minus = c.t.coef_minus;
+  a.ak2 = c.t.ak2;
+  a.tw = prec == 0 ? c.t.tw64 : c.t.tw32;
+  a.Xcur = Xcur;
+  a.Xout = Xout;
+  a.acc = acc;
+  a.fnl = fnl;
+  a.chi = chi;
+  a.ldx = 2 * c.g.ncp;
+  a.ncp = c.g.ncp;
+  a.ncoef = c.g.ncoef;
+  a.invN = 1.0 / (double)c.g.N;
+  a.mode = mode;
+  a.sp = sp;
+  a.mom_part = mom_part;
+  a.coeffs_dev = nullptr;
+  a.step_dev = nullptr;
+  a.gx = c.g.nchunk;
+  a.gy = npairs;
+  return a;
+}
+
+size_t plane_lds_for(int n, int prec, int M) {
+  switch (n) {
+#define SDFT_PL(NN)                                                                 \
+  case NN:                                                                          \
+    return prec == 0 ? plane2_lds_bytes<double, NN>(M) : plane2_lds_bytes<float, NN>(M);
+    SDFT_PL(24) SDFT_PL(32) SDFT_PL(36) SDFT_PL(48) SDFT_PL(64) SDFT_PL(96) SDFT_PL(128)
+    SDFT_PL(192)
+#undef SDFT_PL
+    default:
+      return 0;
+  }
+}
+
+// ---- batched launches: one launch per kernel over the fragments of one grid
+// size; `prefix` holds each fragment's first workgroup (size nfrag + 1).
+template <typename T, int N>
+static int batch_launch_impl(int kind, const void* args, const int* prefix, int nfrag, int total,
+                             size_t lds, hipStream_t s) {
+  if (kind == 0) {
+    hipLaunchKernelGGL((k_expand_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
+                       reinterpret_cast<const ExpandArgs*>(args), prefix, nfrag);
+  } else if (kind == 1) {
+    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_batch<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_plane_batch<T, N>), dim3(total), dim3(kThreads), lds, s,
+                       reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+  } else {
+    hipLaunchKernelGGL((k_contract_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
+                       reinterpret_cast<const ContractArgs*>(args), prefix, nfrag);
+  }
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
+                 int total, size_t lds, hipStream_t s) {
+  if (total <= 0) return 0;
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(n, return (batch_launch_impl<double, NN>(kind, args, prefix, nfrag, total,
+                                                               lds, s)));
+  } else {
+    SDFT_SIZE_SWITCH(n, return (batch_launch_impl<float, NN>(kind, args, prefix, nfrag, total, lds,
+                                                              s)));
+  }
+  return 0;
+}
+
+size_t w_bytes(const Ctx& c, int npairs, int prec) {
+  const size_t elem = prec == 0 ? sizeof(Cx<double>) : sizeof(Cx<float>);
+  return (size_t)npairs * c.g.n * c.g.ncolp * elem;
+}
+
 template <typename T, int N>
 static int expand_impl(Ctx& c, const double* X, int npairs, void* W, const void* tw) {
-  const Geometry& g = c.g;
-  dim3 grid(g.nchunk, npairs);
-  hipLaunchKernelGGL((k_expand<T, N>), grid, dim3(kThreads), 0, c.stream, X, 2 * g.ncp, g.ncp,
-                     c.t.chunk_col, c.t.ent_off, c.t.ent_j, c.t.ent_b,
-                     reinterpret_cast<const Cx<T>*>(tw), reinterpret_cast<Cx<T>*>(W), g.ncolp);
+  (void)tw;
+  const ExpandArgs a = make_expand_args(c, prec_of<T>(), X, W);
+  dim3 grid(a.gx, npairs);
+  hipLaunchKernelGGL((k_expand<T, N>), grid, dim3(kThreads), 0, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -589,10 +754,11 @@ static int plane_impl(Ctx& c, void* W, int npairs, const T* v, const void* tw) {
   }
   SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  (void)v;
+  (void)tw;
+  const PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
   dim3 grid(N, npairs);
-  hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream,
-                     reinterpret_cast<Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M, v,
-                     reinterpret_cast<const Cx<T>*>(tw));
+  hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -610,14 +776,11 @@ template <typename T, int N>
 static int contract_impl(Ctx& c, int mode, const void* W, int npairs, const double* Xcur,
                          double* Xout, double* acc, const double* fnl, const double* chi,
                          const StepParams& sp, double* mom_part, const void* tw) {
-  const Geometry& g = c.g;
-  dim3 grid(g.nchunk, npairs);
-  const double invN = 1.0 / (double)g.N;
-  hipLaunchKernelGGL((k_contract<T, N>), grid, dim3(kThreads), 0, c.stream,
-                     reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.chunk_col, c.t.chunk_coef,
-                     c.t.coef_plus, c.t.coef_minus, c.t.ak2, reinterpret_cast<const Cx<T>*>(tw),
-                     Xcur, Xout, acc, fnl, chi, 2 * g.ncp, g.ncp, g.ncoef, invN, mode, sp,
-                     mom_part);
+  (void)tw;
+  const ContractArgs a = make_contract_args(c, prec_of<T>(), mode, W, npairs, Xcur, Xout, acc,
+                                            fnl, chi, sp, mom_part);
+  dim3 grid(a.gx, npairs);
+  hipLaunchKernelGGL((k_contract<T, N>), grid, dim3(kThreads), 0, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

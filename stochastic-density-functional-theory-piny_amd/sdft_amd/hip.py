@@ -40,6 +40,8 @@ EXPORTS = [
     "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
     "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
+    "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
+    "sdft_hip_batch_autotune", "sdft_hip_batch_config",
 ]
 
 _LIB = None
@@ -86,6 +88,11 @@ def load(path: str | None = None):
                                 _dp], C.c_int),
         "sdft_hip_noise": ([vp, C.c_int, C.c_longlong, C.c_longlong, _dp, _dp], C.c_int),
         "sdft_hip_block_noise": ([vp, C.c_longlong, C.c_longlong], C.c_int),
+        "sdft_hip_batch_create": ([C.c_int, C.POINTER(vp), C.POINTER(vp)], C.c_int),
+        "sdft_hip_batch_destroy": ([vp], C.c_int),
+        "sdft_hip_batch_filter": ([vp, C.c_int, _dp, _dp, _dp, C.c_int], C.c_int),
+        "sdft_hip_batch_autotune": ([vp, C.c_int, _ip, _ip, C.c_int, _dp], C.c_int),
+        "sdft_hip_batch_config": ([vp, C.c_int, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -301,3 +308,52 @@ class Block:
         im = np.empty((self.nstate, ncoef))
         _check(self.ctx.lib.sdft_hip_block_get(self.h, which, _ptr(re), _ptr(im)))
         return re + 1j * im
+
+
+class Batch:
+    """A fragment batch (sdft_hip_batch_*): resident blocks of independent
+    systems filtered together."""
+
+    def __init__(self, blocks):
+        self.lib = load()
+        self.blocks = list(blocks)
+        arr = (C.c_void_p * len(self.blocks))(*[b.h.value for b in self.blocks])
+        h = C.c_void_p()
+        _check(self.lib.sdft_hip_batch_create(len(self.blocks), arr, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdft_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def filter(self, coeffs, scale, emean, use_graph=True):
+        """coeffs (nfrag, L); scale / emean (nfrag,).  Results in each block's
+        accumulator (Block.get(0))."""
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        nf, L = coeffs.shape
+        assert nf == len(self.blocks)
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        em = np.ascontiguousarray(emean, dtype=np.float64)
+        assert sc.shape == (nf,) and em.shape == (nf,)
+        _check(self.lib.sdft_hip_batch_filter(self.h, L, _ptr(coeffs), _ptr(sc), _ptr(em),
+                                              1 if use_graph else 0))
+
+    def autotune(self, steps=8):
+        split = C.c_int(0)
+        ov = C.c_int(0)
+        ms = np.zeros(16)
+        n = self.lib.sdft_hip_batch_autotune(self.h, steps, C.byref(split), C.byref(ov), 16,
+                                             _ptr(ms))
+        if n < 0:
+            _check(n)
+        return split.value, ov.value, ms[:n].copy()
+
+    def config(self, split, overlap):
+        _check(self.lib.sdft_hip_batch_config(self.h, split, overlap))

@@ -187,8 +187,17 @@ def _structure_factor(kfull: np.ndarray, L: float, pos: np.ndarray) -> np.ndarra
     return out
 
 
+def _structure_factor_grid(n: tuple, L: float, pos: np.ndarray) -> np.ndarray:
+    """S(G) on the full FFT grid (FFTW order) from per-axis phase tables:
+    exp(-i G.R) = e_a(ka) e_b(kb) e_c(kc); equal to _structure_factor up to
+    rounding, without one complex exponential per grid point and atom."""
+    frac = pos / L
+    e = [np.exp(-2j * math.pi * np.outer(_grid_freqs(m), frac[:, d])) for d, m in enumerate(n)]
+    return np.einsum("ia,ja,ka->ijk", e[0], e[1], e[2], optimize=True).ravel()
+
+
 def ks_potential(L: float, n: tuple, pos: np.ndarray, z_val: float, rc_loc: float,
-                 sigma_rho: float) -> np.ndarray:
+                 sigma_rho: float, separable: bool = False) -> np.ndarray:
     """v_KS(r) on the FFT grid, FFTW order, real.
 
     v_loc(G) = -4 pi Z/(V G^2) exp(-G^2 rc^2/4) S(G)      (erf-screened Coulomb)
@@ -202,7 +211,7 @@ def ks_potential(L: float, n: tuple, pos: np.ndarray, z_val: float, rc_loc: floa
     ka, kb, kc = np.meshgrid(f1, f2, f3, indexing="ij")
     kfull = np.stack([ka.ravel(), kb.ravel(), kc.ravel()], axis=1).astype(np.float64)
     g2 = (2.0 * math.pi / L) ** 2 * (kfull ** 2).sum(axis=1)
-    S = _structure_factor(kfull, L, pos)
+    S = _structure_factor_grid(n, L, pos) if separable else _structure_factor(kfull, L, pos)
     with np.errstate(divide="ignore", invalid="ignore"):
         vloc = -4.0 * math.pi * z_val / (V * g2) * np.exp(-g2 * rc_loc ** 2 / 4.0) * S
         rhog = (z_val / V) * np.exp(-g2 * sigma_rho ** 2 / 4.0) * S
@@ -265,8 +274,11 @@ def kb_table(L: float, pos: np.ndarray, k: np.ndarray, rc_nl: float,
 
 def build_system(name: str, a0: float, rep: int, n: int, g2max: float = 12.0,
                  pos: Optional[np.ndarray] = None, with_nl: bool = True,
-                 h_kb: tuple = (2.0, 1.0)) -> System:
+                 h_kb: tuple = (2.0, 1.0), cell: Optional[float] = None,
+                 separable_sf: bool = False) -> System:
     L, p = diamond_supercell(a0, rep)
+    if cell is not None:
+        L = float(cell)
     if pos is not None:
         p = np.asarray(pos, dtype=np.float64)
     k = half_sphere(L, g2max)
@@ -277,7 +289,7 @@ def build_system(name: str, a0: float, rep: int, n: int, g2max: float = 12.0,
     m, mc = fftw_maps(k, grid)
     gvec = k.astype(np.float64) * (2.0 * math.pi / L)
     ak2 = (gvec ** 2).sum(axis=1)
-    vks = ks_potential(L, grid, p, 4.0, 1.0, 1.2)
+    vks = ks_potential(L, grid, p, 4.0, 1.0, 1.2, separable=separable_sf)
     if with_nl:
         proj, pe, pa, pl, pm = kb_table(L, p, k, 0.8, h_kb)
     else:
@@ -331,3 +343,56 @@ WORKLOADS = {
     "si512": Workload("si512-128^3-B512x128-L2048", 10.26, 4, 128, 512, 128, 2048, 0.01, 2),
     "si1728": Workload("si1728-192^3-B1024x128-L3000", 10.26, 6, 192, 1024, 128, 3000, 0.005, 4),
 }
+
+
+# ---------------------------------------------------------------------------
+# Fragment batch: many small systems cut from the Si1728 cell
+# ---------------------------------------------------------------------------
+
+FRAGMENT_GRIDS = (24, 32, 36, 48)
+
+
+@dataclasses.dataclass
+class FragmentSpec:
+    index: int
+    natm: int
+    cell: float
+    n: int
+    nstate: int
+    pos: np.ndarray
+
+
+def fragment_specs(nfrag: int = 343, seed: int = 3, natm_range: tuple = (8, 64),
+                   g2max: float = 12.0, a0: float = 10.26, rep: int = 6,
+                   grids: tuple = FRAGMENT_GRIDS) -> list:
+    """Fragments of the Si1728 (6x6x6 diamond, a = 61.56 bohr) cell: fragment
+    i takes natm ~ U[8, 64] atoms (seeded) nearest, in the max-norm, to a
+    random centre -- a cube-shaped cluster -- in a cubic box of the bulk
+    density (side a0 (natm/8)^(1/3)), positions wrapped into the box.  Its
+    grid is the smallest of `grids` holding the density sphere (n >= 4K + 1,
+    K = max |k| of the wavefunction sphere) and its orbital count grows
+    linearly from 32 (8 atoms) to 128 (64 atoms), kept even."""
+    rng = np.random.default_rng(seed)
+    Lbig, pbig = diamond_supercell(a0, rep)
+    out = []
+    for i in range(nfrag):
+        natm = int(rng.integers(natm_range[0], natm_range[1] + 1))
+        centre = rng.random(3) * Lbig
+        d = pbig - centre
+        d -= Lbig * np.round(d / Lbig)
+        idx = np.argsort(np.abs(d).max(axis=1), kind="stable")[:natm]
+        side = a0 * (natm / 8.0) ** (1.0 / 3.0)
+        pos = np.mod(d[idx] + 0.5 * side, side)
+        K = int(math.floor(math.sqrt(g2max) * side / (2.0 * math.pi)))
+        fit = [g for g in grids if g >= 4 * K + 1]
+        n = min(fit) if fit else max(grids)
+        frac = (natm - natm_range[0]) / float(natm_range[1] - natm_range[0])
+        nstate = 32 + 2 * int(round(frac * 48))
+        out.append(FragmentSpec(i, natm, side, n, nstate, pos))
+    return out
+
+
+def build_fragment(spec: FragmentSpec, g2max: float = 12.0, with_nl: bool = True) -> System:
+    return build_system("frag%d-%da-%d^3" % (spec.index, spec.natm, spec.n), 10.26, 1, spec.n,
+                        g2max=g2max, pos=spec.pos, with_nl=with_nl, cell=spec.cell,
+                        separable_sf=True)

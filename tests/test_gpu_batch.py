@@ -1,0 +1,158 @@
+"""GPU parity of the fragment batch (sdft_hip_batch_*) against the CPU oracle.
+
+Fragments are cut from the Si1728 cell exactly as for the 343-fragment
+workload (seed 3) and filtered together; each fragment's accumulator must
+match the oracle's filterChebyPolyHerm on that fragment alone (FP64: 1e-10
+relative, FP32: 1e-4 relative, as for single systems), the graph replay must
+be bit-identical to direct launches, and the batch must agree with the
+single-system block filter.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from sdft_amd import hip as H
+    H.load()
+    return H
+
+
+@pytest.fixture(scope="module")
+def frags():
+    """Two fragments of each grid size (24, 32, 36, 48) with few orbitals
+    (odd counts included, to exercise the pair padding)."""
+    from sdft_amd import stodft, systems
+    specs = systems.fragment_specs()
+    picked = []
+    for n in systems.FRAGMENT_GRIDS:
+        picked += [s for s in specs if s.n == n][:2]
+    out = []
+    for i, s in enumerate(picked):
+        sysm = systems.build_fragment(s)
+        emin, emax = stodft.spectral_bounds(sysm)
+        nstate = 3 + (i % 4)  # 3..6
+        chi = systems.noise_orbitals(sysm, nstate, seed=11 + i)
+        out.append((sysm, chi, emin, emax))
+    return out
+
+
+def _coeffs(frags, L):
+    co = np.stack([O.cheby_coeffs(L, emin, emax, 0.1 + 0.02 * i, 100.0, 2)
+                   for i, (_, _, emin, emax) in enumerate(frags)])
+    scale = np.array([2.0 / (emax - emin) for (_, _, emin, emax) in frags])
+    emean = np.array([0.5 * (emin + emax) for (_, _, emin, emax) in frags])
+    return co, scale, emean
+
+
+def _make_batch(hip, frags, prec):
+    ctxs, blocks = [], []
+    for sysm, chi, _, _ in frags:
+        c = hip.Context(sysm)
+        b = hip.Block(c, chi.shape[0], prec)
+        b.set(chi)
+        ctxs.append(c)
+        blocks.append(b)
+    return ctxs, blocks, hip.Batch(blocks)
+
+
+@pytest.mark.parametrize("L", [2, 3, 17])
+def test_batch_fp64_matches_oracle(hip, frags, L):
+    say("case: batch fp64 8 fragments L=%d" % L)
+    co, scale, emean = _coeffs(frags, L)
+    ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP64)
+    bat.filter(co, scale, emean, use_graph=True)
+    for f, (sysm, chi, _, _) in enumerate(frags):
+        acc_r, tl_r = O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])
+        acc_g = blocks[f].get(0)
+        tl_g = blocks[f].get(1)
+        assert rel(acc_g, acc_r) < 1e-10, (f, sysm.name, rel(acc_g, acc_r))
+        assert rel(tl_g, tl_r) < 1e-10, (f, sysm.name, rel(tl_g, tl_r))
+    bat.close()
+
+
+def test_batch_graph_equals_direct_and_block(hip, frags):
+    say("case: batch fp64 graph vs direct vs block filter, L=24")
+    L = 24
+    co, scale, emean = _coeffs(frags, L)
+    ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP64)
+    bat.filter(co, scale, emean, use_graph=True)
+    g = [b.get(0) for b in blocks]
+    bat.filter(co, scale, emean, use_graph=False)
+    d = [b.get(0) for b in blocks]
+    # the same kernels in the same order: identical bits
+    for f in range(len(frags)):
+        assert np.array_equal(g[f], d[f]), f
+    # the single-system block filter (its KB split may differ: sums in
+    # another order)
+    for f, b in enumerate(blocks):
+        b.filter(co[f], scale[f], emean[f])
+        assert rel(b.get(0), g[f]) < 1e-12, f
+    # a second batch call re-filters chi and gives the same result again
+    bat.filter(co, scale, emean, use_graph=True)
+    for f in range(len(frags)):
+        assert np.array_equal(blocks[f].get(0), g[f]), f
+    bat.close()
+
+
+def test_batch_fp32_matches_oracle(hip, frags):
+    say("case: batch fp32 8 fragments L=40")
+    L = 40
+    co, scale, emean = _coeffs(frags, L)
+    ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP32)
+    bat.filter(co, scale, emean, use_graph=True)
+    for f, (sysm, chi, _, _) in enumerate(frags):
+        acc_r, _ = O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])
+        assert rel(blocks[f].get(0), acc_r) < 1e-4, (f, sysm.name)
+    bat.close()
+
+
+def test_batch_autotune_and_configs(hip, frags):
+    say("case: batch autotune + every KB split / overlap configuration, L=9")
+    L = 9
+    co, scale, emean = _coeffs(frags, L)
+    ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP64)
+    split, ov, ms = bat.autotune(steps=4)
+    assert split in (1, 2, 4, 8) and ov in (0, 1)
+    assert len(ms) == 8 and np.all(ms > 0)
+    refs = [O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])[0]
+            for f, (sysm, chi, _, _) in enumerate(frags)]
+    for sp in (1, 2, 4, 8):
+        for o in (0, 1):
+            bat.config(sp, o)
+            bat.filter(co, scale, emean, use_graph=True)
+            for f in range(len(frags)):
+                assert rel(blocks[f].get(0), refs[f]) < 1e-10, (sp, o, f)
+    bat.close()
+
+
+def test_batch_rejects_bad_input(hip, frags):
+    say("case: batch argument checks")
+    sysm, chi, _, _ = frags[0]
+    c = hip.Context(sysm)
+    b64 = hip.Block(c, 2, hip.PREC_FP64)
+    b32 = hip.Block(c, 2, hip.PREC_FP32)
+    with pytest.raises(hip.SdftError):
+        hip.Batch([b64, b32])  # mixed precision
+    with pytest.raises(hip.SdftError):
+        hip.Batch([b64, b64])  # same block twice
+    b64.set(chi[:2])
+    bat = hip.Batch([b64])
+    with pytest.raises(hip.SdftError):
+        bat.filter(np.ones((1, 1)), np.ones(1), np.zeros(1))  # L < 2
+    bat.close()
