@@ -85,6 +85,13 @@ def main():
     ap.add_argument("--workload", default="si64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 343-fragment batch measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 343-fragment batch measurement (JSON line of its own)")
+    ap.add_argument("--batch-config", default=None,
+                    help="fixed KB split,overlap for the batch (e.g. 2,0) instead of the "
+                         "autotuner")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware counters: an 8-term FP32 and FP64 filter of "
                          "the workload, then a 256 MiB plain copy (counter calibration)")
@@ -100,6 +107,9 @@ def main():
 
     from sdft_amd import hip, stodft, systems
 
+    if args.batch_only:
+        print(json.dumps(batch_bench(args, 0.2)), flush=True)
+        return
     wl = systems.WORKLOADS[args.workload]
     t0 = time.perf_counter()
     sysm = systems.build_system(wl.name, wl.a0, wl.rep, wl.n)
@@ -201,6 +211,8 @@ def main():
                          "scipy.fft oracle (%.1f s)" % (args.cpu_steps, nst, dt)}
         log("cpu baseline: %.1f H.psi/s on %d threads" % (rate, workers))
 
+    batch = None if args.no_batch else batch_bench(args, mu)
+
     r32, r64 = results["fp32"], results["fp64"]
     survey_frac32 = survey_bytes_per_hpsi(sysm, 8, 4) * r32["hpsi_per_s"] / (HBM_PEAK_GBS * 1e9)
     line = {
@@ -228,8 +240,120 @@ def main():
         "kernels_fp64_ms": {k: round(v[0] / v[1], 4) for k, v in r64["kernels"].items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fragment_batch": batch,
     }
     print(json.dumps(line), flush=True)
+
+
+def _build_fragment(spec):
+    from threadpoolctl import threadpool_limits
+
+    from sdft_amd import stodft, systems
+    with threadpool_limits(1):  # one BLAS thread per pool worker
+        s = systems.build_fragment(spec)
+        lo, hi = stodft.spectral_bounds(s)
+    return s, lo, hi
+
+
+def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
+    """The fragment batch: 343 fragments of the Si1728 cell (8-64 atoms, seed
+    3), grids 24^3-48^3, 32-128 orbitals each, Chebyshev degree 512, all
+    filtered together (sdft_hip_batch_filter: graph-replayed steps, launch
+    configuration from the autotuner).  FP32 form timed and checked against
+    the FP64 form on a sample of fragments; mu is shared by all fragments."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from sdft_amd import hip, systems
+    t0 = time.perf_counter()
+    specs = systems.fragment_specs(343, seed)
+    workers = min(16, os.cpu_count() or 1)
+    with ProcessPoolExecutor(workers) as ex:
+        built = list(ex.map(_build_fragment, specs, chunksize=4))
+    log("batch setup: %d fragments built (%.1fs)" % (len(built), time.perf_counter() - t0))
+    ctxs = [hip.Context(s) for s, _, _ in built]
+    L = degree
+    beta = 1.0 / kt
+    coeffs = np.stack([c.cheby_coeffs(L, lo, hi, mu, beta, 2)
+                       for c, (_, lo, hi) in zip(ctxs, built)])
+    scale = np.array([2.0 / (hi - lo) for _, lo, hi in built])
+    emean = np.array([0.5 * (hi + lo) for _, lo, hi in built])
+    nst = [sp.nstate for sp in specs]
+    first = np.concatenate([[0], np.cumsum(nst)[:-1]]).astype(int)
+    hpsi = float(sum(nst)) * (L - 1)
+    grids = {}
+    for sp in specs:
+        grids[sp.n] = grids.get(sp.n, 0) + 1
+    log("batch: %d orbitals, grids %s (%.1fs)" % (sum(nst), grids, time.perf_counter() - t0))
+    sample = list(range(0, len(specs), 43))
+    out, res = {}, {}
+    for prec, tag in ((hip.PREC_FP32, "fp32"), (hip.PREC_FP64, "fp64")):
+        blocks = [hip.Block(c, n, prec) for c, n in zip(ctxs, nst)]
+        for b, f0 in zip(blocks, first):
+            b.noise(seed, int(f0))
+        bat = hip.Batch(blocks)
+        if args.batch_config:
+            split, ov = (int(x) for x in args.batch_config.split(","))
+            bat.config(split, ov)
+            cand = np.zeros(0)
+        else:
+            split, ov, cand = bat.autotune(steps=6)
+        log("%s autotune: split=%d overlap=%d candidates(ms/step)=%s" % (
+            tag, split, ov, np.round(cand, 3).tolist()))
+        steps = args.steps if prec == hip.PREC_FP32 else 1
+        for _ in range(args.warmup):
+            bat.filter(coeffs, scale, emean)
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            bat.filter(coeffs, scale, emean)
+        dt = (time.perf_counter() - t1) / steps
+        t2 = time.perf_counter()
+        bat.filter(coeffs, scale, emean, use_graph=False)
+        dt_direct = time.perf_counter() - t2
+        res[tag] = dict(s_per_filter=dt, hpsi_per_s=hpsi / dt, direct_s=dt_direct,
+                        split=split, overlap=ov, cand=cand)
+        out[tag] = [blocks[i].get(0) for i in sample]
+        log("%s batch: %.3f s/filter (graph), %.3f s (direct), %.4g H.psi/s" % (
+            tag, dt, dt_direct, hpsi / dt))
+        bat.close()
+        for b in blocks:
+            b.close()
+    # FP32 vs FP64 on the sampled fragments
+    de, dl1 = [], []
+    for j, i in enumerate(sample):
+        sysm, c = built[i][0], ctxs[i]
+        N = sysm.ngrid
+        occ_over = 2.0 / nst[i]
+        e = []
+        rhos = []
+        for tag in ("fp64", "fp32"):
+            rho = c.density(out[tag][j], occ_over)
+            eke, enl = c.energies(out[tag][j], 2.0, nst[i])
+            e.append(eke + enl + float(rho @ sysm.vks) / N)
+            rhos.append(rho)
+        ne = rhos[0].sum() / N
+        dl1.append(float(np.abs(rhos[1] - rhos[0]).sum() / N / ne))
+        de.append(abs(e[1] - e[0]) / sysm.natm)
+    r32, r64 = res["fp32"], res["fp64"]
+    return {
+        "workload": "343 fragments of Si1728 (8-64 atoms, seed 3), grids 24-48^3, "
+                    "32-128 orbitals, L=%d" % L,
+        "grids": {str(k): v for k, v in sorted(grids.items())},
+        "orbitals": int(sum(nst)),
+        "hpsi_per_filter": hpsi,
+        "value_fp32": round(r32["hpsi_per_s"], 1),
+        "s_per_filter_fp32": round(r32["s_per_filter"], 4),
+        "s_per_filter_fp32_direct_launches": round(r32["direct_s"], 4),
+        "value_fp64": round(r64["hpsi_per_s"], 1),
+        "s_per_filter_fp64": round(r64["s_per_filter"], 4),
+        "fp32_over_fp64": round(r32["hpsi_per_s"] / r64["hpsi_per_s"], 3),
+        "autotune_fp32": {"kb_split": r32["split"], "kb_overlap": r32["overlap"],
+                          "candidates_ms_per_step": np.round(r32["cand"], 4).tolist()},
+        "autotune_fp64": {"kb_split": r64["split"], "kb_overlap": r64["overlap"]},
+        "fp32_error_sample": {"fragments": sample,
+                              "energy_per_atom_ha_max": float(max(de)),
+                              "density_l1_per_electron_max": float(max(dl1))},
+        "mu": mu,
+    }
 
 
 def measured_traffic(kernel, wl):

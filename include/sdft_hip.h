@@ -70,6 +70,25 @@ int sdft_hip_set_vks(sdft_ctx *ctx, const double *v_ks);
 int sdft_hip_set_projectors(sdft_ctx *ctx, int nproj, const double *proj_re,
                             const double *proj_im, const double *proj_energy);
 
+/* Real-space KB projectors (nlppKBRealEnergyFilter,
+ * energy/cp/cp_energy_eext_nonloc_real_filter.c:110-450, called from
+ * controlPotentialEnergyFilter :36-105 when nlpp_real_on).  Projector p takes
+ * the values values[offsets[p] .. offsets[p+1]) at the FFTW linear grid
+ * indices grid_index[...] (the atom's gridNlppMap points with one
+ * vnlPhiAtomGridRe/Im row per (l, radial, m) channel; indices distinct within
+ * a projector); energy[p] is its KB energy (vpsNormList, doubled for the m > 0
+ * Re/Im rows as at :318-404); vol_elem = V/N (volElem).  The reference adds
+ *   V psi(r) = sum_p energy_p beta_p(r) vol_elem sum_r' beta_p(r') psi(r')
+ * to v_KS psi on the grid before the forward FFT.  Restricted to the
+ * plane-wave set that operator is exactly a KB operator with table
+ * Q_p(G) = N^(-1/2) sum_r beta_p(r) e^{iG.r} and energies energy_p vol_elem:
+ * the library computes that table on the device (projectors scattered onto
+ * dense grids, its own forward transform) and applies it with the same MFMA
+ * products as sdft_hip_set_projectors, whose table it replaces. */
+int sdft_hip_set_projectors_real(sdft_ctx *ctx, int nproj, const long long *offsets,
+                                 const int *grid_index, const double *values,
+                                 const double *energy, double vol_elem);
+
 /* Raw coefficient force f = -dE/dc* of H applied to nstate orbitals
  * (-4 Hc for G != 0, -2 Hc at G = 0): calcCoefForceWrapSCF,
  * stodft/reduced-energy/energy-wrapper-scf.c:1066-1197 (zero f, G-space KB,

@@ -143,29 +143,53 @@ def nl_energy_per_state(C, proj, energy):
     return (Z * Z * energy[None, :]).sum(axis=1)
 
 
+def nl_apply_real(z, rs):
+    """nlppKBRealEnergyFilter (energy/cp/cp_energy_eext_nonloc_real_filter.c:
+    110-450) on one grid array z (the reference runs it on the real and the
+    imaginary part of the packed pair separately, :36-105; the operator is
+    real, so one pass over complex z is the same): per projector p the
+    neighbourhood values are gathered (wfNbhd, :229-241), dot = volElem *
+    sum beta_p psi * vpsNorm_p (:318-331), forceTemp += dot * beta_p (daxpy),
+    and the per-atom results are scattered onto the grid (:430-441)."""
+    out = np.zeros_like(z)
+    for p in range(len(rs.energy)):
+        sl = slice(rs.offsets[p], rs.offsets[p + 1])
+        ix = rs.index[sl]
+        b = rs.values[sl]
+        dot = rs.vol_elem * (b @ z[ix]) * rs.energy[p]
+        out[ix] += dot * b
+    return out
+
+
 # ---------------------------------------------------------------------------
 # H.psi (raw force)
 # ---------------------------------------------------------------------------
 
-def hpsi_raw(C, sysm, with_nl=True):
+def hpsi_raw(C, sysm, with_nl=True, nl_real=None):
     """calcCoefForceWrapSCF (stodft/reduced-energy/energy-wrapper-scf.c:1066-1197)
     -> calcNonLocalPseudoScf (G-space KB) + coefForceCalcHybridSCF
     (cp-energy-ee-rho-stodft.c:38-355): per orbital pair pack -> G->r FFT ->
     v.psi (cp_vpsi_threads, cp_energy_ee_rho.c:2817-2845) -> r->G FFT ->
     unpack-subtract; an odd last orbital is paired with zero (identical math
-    to the single pack); kinetic f -= 2|G|^2 c (G != 0) and f_im(0) = 0."""
+    to the single pack); kinetic f -= 2|G|^2 c (G != 0) and f_im(0) = 0.
+    With nl_real (systems.RealSpaceProjectors) the KB term is the real-space
+    one, added to v.psi on the grid (controlPotentialEnergyFilter,
+    cp_energy_eext_nonloc_real_filter.c:66-100), instead of the G-space one."""
     nstate, ncoef = C.shape
     shape = tuple(sysm.n)
     v = sysm.vks.reshape(shape)
     f = np.zeros_like(C)
-    if with_nl and sysm.nproj > 0:
+    if with_nl and nl_real is None and sysm.nproj > 0:
         f += nl_apply_gspace(C, sysm.proj, sysm.proj_energy)
     zero = np.zeros(ncoef, dtype=np.complex128)
     for s in range(0, nstate, 2):
         c1 = C[s]
         c2 = C[s + 1] if s + 1 < nstate else zero
         z = to_r(pack_pair(c1, c2, sysm.map, sysm.map_conj, shape))
-        z *= v
+        if with_nl and nl_real is not None:
+            z = z * v + nl_apply_real(z.ravel(), nl_real).reshape(shape)
+        else:
+            z *= v
         F = to_g(z)
         f1 = f[s]
         f2 = f[s + 1] if s + 1 < nstate else zero.copy()
@@ -214,7 +238,7 @@ def norm_h_cheby(c, f, t_prev, k, scale, emean):
     return f
 
 
-def filter_cheby(chi, coeffs, sysm, scale, emean, with_nl=True, moments=False):
+def filter_cheby(chi, coeffs, sysm, scale, emean, with_nl=True, moments=False, nl_real=None):
     """filterChebyPolyHerm, stodft/filter.c:318-706 (numChemPot = 1):
     acc = c_0 chi; for k = 1..L-1: T_k by normHCheby, acc += c_k T_k.
     Returns (acc, T_{L-1}) -- on return the reference leaves cre_up = T_{L-1}.
@@ -228,7 +252,7 @@ def filter_cheby(chi, coeffs, sysm, scale, emean, with_nl=True, moments=False):
     if moments:
         mom[0] = gamma_dot(chi, chi).sum()
     for k in range(1, L):
-        f = hpsi_raw(c, sysm, with_nl)
+        f = hpsi_raw(c, sysm, with_nl, nl_real)
         norm_h_cheby(c, f, t_prev, k, scale, emean)
         if k > 1:
             t_prev = c

@@ -263,6 +263,7 @@ static void free_ctx(Ctx& c) {
                   c.t.chunk_col,
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
+                  c.At64,       c.At32,
                   c.E,          c.W,            c.zpart,       c.Y,            c.Z,
                   c.red};
   for (void* p : ptrs)
@@ -579,6 +580,153 @@ static int block_filter_impl(Block& b, int L, const double* coeffs, double scale
   return 0;
 }
 
+// G-space KB table from Q_p(G) in the caller's order (see sdft_hip_set_projectors).
+static int set_projectors_impl(Ctx& c, int nproj, const double* pre, const double* pim,
+                               const double* pe) {
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream_nl));
+  void* old[] = {c.A64, c.A32, c.At64, c.At32, c.E};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  c.A64 = nullptr;
+  c.A32 = nullptr;
+  c.At64 = nullptr;
+  c.At32 = nullptr;
+  c.E = nullptr;
+  c.nproj = 0;
+  c.ppad = 0;
+  if (nproj == 0) return 0;
+  const int nc = c.g.ncoef, ncp = c.g.ncp;
+  const size_t K = 2 * (size_t)ncp;
+  std::vector<double> A((size_t)nproj * K, 0.0);
+  std::vector<float> A32((size_t)nproj * K, 0.f);
+  for (int p = 0; p < nproj; ++p) {
+    for (int j = 0; j < nc; ++j) {
+      const int i = c.g.perm[j];
+      const bool g0 = (j == nc - 1);
+      const double w = g0 ? 1.0 : 2.0;
+      const double qr = pre[(size_t)p * nc + i], qi = pim[(size_t)p * nc + i];
+      A[(size_t)p * K + j] = w * qr;
+      A[(size_t)p * K + ncp + j] = g0 ? 0.0 : -w * qi;
+    }
+  }
+  for (size_t i = 0; i < A.size(); ++i) A32[i] = (float)A[i];
+  const int ppad = (nproj + 31) / 32 * 32;
+  std::vector<double> At(K * ppad, 0.0);
+  std::vector<float> At32(K * ppad, 0.f);
+  for (int p = 0; p < nproj; ++p)
+    for (size_t k = 0; k < K; ++k) {
+      At[k * ppad + p] = A[(size_t)p * K + k];
+      At32[k * ppad + p] = A32[(size_t)p * K + k];
+    }
+  int rc;
+  if ((rc = upload(&c.A64, A))) return rc;
+  if ((rc = upload(&c.A32, A32))) return rc;
+  if ((rc = upload(&c.At64, At))) return rc;
+  if ((rc = upload(&c.At32, At32))) return rc;
+  c.ppad = ppad;
+  std::vector<double> E(pe, pe + nproj);
+  if ((rc = upload(&c.E, E))) return rc;
+  c.E_host = E;
+  c.nproj = nproj;
+  return 0;
+}
+
+
+// Real-space KB projectors -> the G-space table they restrict to on the
+// plane-wave set: Q_p(G) = N^(-1/2) sum_r beta_p(r) e^{iG.r}, energies
+// energy_p vol_elem.  Pairs of projectors are scattered onto dense complex
+// grids (re = p even, im = p odd) in batches, transformed with the noise
+// path's pruned forward transform (c(G) = (1/N) sum_r f(r) e^{-iG.r} for both
+// real fields) and conjugated: Q = sqrt(N) conj(c).
+static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, const int* idx,
+                                    const double* val, const double* energy, double vol_elem) {
+  if (nproj == 0) return set_projectors_impl(c, 0, nullptr, nullptr, nullptr);
+  const long long N = c.g.N;
+  if (off[0] != 0) {
+    set_error("real-space projector offsets must start at 0");
+    return SDFT_EINVAL;
+  }
+  for (int p = 0; p < nproj; ++p) {
+    if (off[p + 1] < off[p]) {
+      set_error("real-space projector offsets must be non-decreasing");
+      return SDFT_EINVAL;
+    }
+    std::vector<int> ix(idx + off[p], idx + off[p + 1]);
+    for (int v : ix)
+      if (v < 0 || v >= N) {
+        set_error("real-space projector grid index out of range");
+        return SDFT_EINVAL;
+      }
+    std::sort(ix.begin(), ix.end());
+    if (std::adjacent_find(ix.begin(), ix.end()) != ix.end()) {
+      set_error("a real-space projector lists a grid point twice");
+      return SDFT_EINVAL;
+    }
+  }
+  const long long nent = off[nproj];
+  int rc = 0;
+  long long* doff = nullptr;
+  int* didx = nullptr;
+  double* dval = nullptr;
+  double* field = nullptr;
+  const int pairs = (nproj + 1) / 2;
+  const size_t pair_bytes = sizeof(double) * 2 * (size_t)N;
+  int bq = (int)std::max<size_t>(1, std::min<size_t>((size_t)pairs, (1ull << 30) / pair_bytes));
+  std::vector<double> qre((size_t)nproj * c.g.ncoef), qim((size_t)nproj * c.g.ncoef);
+  std::vector<double> bre, bim;
+  Block tmp;
+  const double sq = std::sqrt((double)N);
+  do {
+    if ((rc = dalloc(&doff, sizeof(long long) * (nproj + 1)))) break;
+    if ((rc = dalloc(&didx, sizeof(int) * std::max<long long>(1, nent)))) break;
+    if ((rc = dalloc(&dval, sizeof(double) * std::max<long long>(1, nent)))) break;
+    if ((rc = dalloc(&field, pair_bytes * bq))) break;
+    SDFT_HIP_CHECK(hipMemcpyAsync(doff, off, sizeof(long long) * (nproj + 1),
+                                  hipMemcpyHostToDevice, c.stream));
+    if (nent > 0) {
+      SDFT_HIP_CHECK(hipMemcpyAsync(didx, idx, sizeof(int) * nent, hipMemcpyHostToDevice,
+                                    c.stream));
+      SDFT_HIP_CHECK(hipMemcpyAsync(dval, val, sizeof(double) * nent, hipMemcpyHostToDevice,
+                                    c.stream));
+    }
+    if ((rc = block_alloc(c, 2 * bq, 0, tmp))) break;
+    if ((rc = ensure_W(c, w_bytes(c, bq, 0)))) break;
+    bre.resize((size_t)2 * bq * c.g.ncoef);
+    bim.resize((size_t)2 * bq * c.g.ncoef);
+    for (int q0 = 0; q0 < pairs && !rc; q0 += bq) {
+      const int nq = std::min(bq, pairs - q0);
+      const int p0 = 2 * q0, p1 = std::min(nproj, 2 * (q0 + nq));
+      SDFT_HIP_CHECK(hipMemsetAsync(field, 0, pair_bytes * nq, c.stream));
+      if ((rc = launch_scatter_proj(c, doff, didx, dval, p0, p1, field))) break;
+      if ((rc = launch_grid_plane(c, c.W, nq, field))) break;
+      StepParams sp{};
+      if ((rc = launch_contract(c, 0, kModeNoise, c.W, nq, tmp.X0, tmp.X0, tmp.chi, nullptr,
+                                nullptr, sp, nullptr)))
+        break;
+      if ((rc = block_fetch(tmp, tmp.X0, bre.data(), bim.data()))) break;
+      for (int p = p0; p < p1; ++p) {
+        const double* r = bre.data() + (size_t)(p - p0) * c.g.ncoef;
+        const double* i = bim.data() + (size_t)(p - p0) * c.g.ncoef;
+        double* qr = qre.data() + (size_t)p * c.g.ncoef;
+        double* qi = qim.data() + (size_t)p * c.g.ncoef;
+        for (int j = 0; j < c.g.ncoef; ++j) {
+          qr[j] = sq * r[j];
+          qi[j] = -sq * i[j];
+        }
+      }
+    }
+  } while (0);
+  block_free(tmp);
+  void* ptrs[] = {doff, didx, dval, field};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (rc) return rc;
+  std::vector<double> e(nproj);
+  for (int p = 0; p < nproj; ++p) e[p] = energy[p] * vol_elem;
+  return set_projectors_impl(c, nproj, qre.data(), qim.data(), e.data());
+}
+
 }  // namespace sdft
 
 using namespace sdft;
@@ -656,40 +804,19 @@ int sdft_hip_set_projectors(sdft_ctx* h, int nproj, const double* pre, const dou
     set_error("bad projector arguments");
     return SDFT_EINVAL;
   }
-  Ctx& c = h->c;
-  if (c.A64) (void)hipFree(c.A64);
-  if (c.A32) (void)hipFree(c.A32);
-  if (c.E) (void)hipFree(c.E);
-  c.A64 = nullptr;
-  c.A32 = nullptr;
-  c.E = nullptr;
-  c.nproj = 0;
-  if (nproj == 0) return 0;
-  const int nc = c.g.ncoef, ncp = c.g.ncp;
-  const size_t K = 2 * (size_t)ncp;
-  std::vector<double> A((size_t)nproj * K, 0.0);
-  std::vector<float> A32((size_t)nproj * K, 0.f);
-  for (int p = 0; p < nproj; ++p) {
-    for (int j = 0; j < nc; ++j) {
-      const int i = c.g.perm[j];
-      const bool g0 = (j == nc - 1);
-      const double w = g0 ? 1.0 : 2.0;
-      const double qr = pre[(size_t)p * nc + i], qi = pim[(size_t)p * nc + i];
-      A[(size_t)p * K + j] = w * qr;
-      A[(size_t)p * K + ncp + j] = g0 ? 0.0 : -w * qi;
-    }
-  }
-  for (size_t i = 0; i < A.size(); ++i) A32[i] = (float)A[i];
-  int rc;
-  if ((rc = upload(&c.A64, A))) return rc;
-  if ((rc = upload(&c.A32, A32))) return rc;
-  std::vector<double> E(pe, pe + nproj);
-  if ((rc = upload(&c.E, E))) return rc;
-  c.E_host = E;
-  c.nproj = nproj;
-  return 0;
+  return set_projectors_impl(h->c, nproj, pre, pim, pe);
 }
 
+int sdft_hip_set_projectors_real(sdft_ctx* h, int nproj, const long long* offsets,
+                                 const int* grid_index, const double* values,
+                                 const double* energy, double vol_elem) {
+  if (!h || nproj < 0 || (nproj > 0 && (!offsets || !grid_index || !values || !energy)) ||
+      !(vol_elem > 0)) {
+    set_error("bad real-space projector arguments");
+    return SDFT_EINVAL;
+  }
+  return set_projectors_real_impl(h->c, nproj, offsets, grid_index, values, energy, vol_elem);
+}
 int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* im,
                        const double* gvec, const int* proj_atom, int natm, double occ,
                        double nsto, double* forces) {

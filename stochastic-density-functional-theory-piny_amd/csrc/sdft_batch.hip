@@ -283,16 +283,17 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
     r.nsplit = B.split_of[fi];
     r.P = P;
     r.ns = S;
+    r.ppad = c.ppad;
     r.E = c.E;
     r.Z = f.Z;
     r.Y = f.Y;
     rd.push_back(r);
     KbForceArgs q;
     q.Y = f.Y;
-    q.A = A;
-    q.P = P;
+    q.At = B.prec == 0 ? (const void*)c.At64 : (const void*)c.At32;
     q.ns = S;
     q.K2 = K2;
+    q.ppad = c.ppad;
     q.ts = (S + 31) / 32;
     q.tk = (K2 + 31) / 32;
     q.F = b.fnl;
@@ -480,7 +481,10 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     if (P > 0) {
       B.nl_frags.push_back(i);
       if ((rc = dmalloc(&f.zpart, sizeof(double) * B.max_zpart_slices * P * S))) return fail(rc);
-      if ((rc = dmalloc(&f.Y, sizeof(double) * (size_t)P * S))) return fail(rc);
+      const size_t yb = sizeof(double) * (size_t)f.c->ppad * S;
+      if ((rc = dmalloc(&f.Y, yb))) return fail(rc);
+      e = hipMemsetAsync(f.Y, 0, yb, B.stream);  // padding columns stay zero
+      if (e != hipSuccess) return fail(hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__));
       if ((rc = dmalloc(&f.Z, sizeof(double) * (size_t)P * S))) return fail(rc);
     }
     const long long nelem = (long long)S * 2 * f.c->g.ncp;

@@ -62,9 +62,9 @@ __device__ inline void load8(const TS* __restrict__ src, TI* dst) {
 // grid (ceil(P/32), ceil(ns/32), nsplit), 4 waves per workgroup share the
 // 32x32 output tile and interleave the split's K steps; the four partial
 // tiles are summed through LDS.
-template <typename TA, typename TI>
+template <typename TA, typename TX, typename TI>
 __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
-                                                const double* __restrict__ X, int P, int ns,
+                                                const TX* __restrict__ X, int P, int ns,
                                                 int K, int ksteps_per_split,
                                                 double* __restrict__ zpart, int bx, int by, int bz,
                                                 double (*red)[32 * 32]) {
@@ -83,7 +83,7 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
   const int nsteps_total = K / KS;
   const int step1 = min(nsteps_total, step0 + ksteps_per_split);
   const TA* arow[2];
-  const double* xrow[2];
+  const TX* xrow[2];
   bool aok[2], xok[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -152,14 +152,14 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
   }
 }
 
-template <typename TA, typename TI>
+template <typename TA, typename TX, typename TI>
 __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
-                                                    const double* __restrict__ X, int P, int ns,
+                                                    const TX* __restrict__ X, int P, int ns,
                                                     int K, int ksteps_per_split,
                                                     double* __restrict__ zpart) {
   __shared__ double red[4][32 * 32];
-  kb_project_tile<TA, TI>(A, X, P, ns, K, ksteps_per_split, zpart, blockIdx.x, blockIdx.y,
-                          blockIdx.z, red);
+  kb_project_tile<TA, TX, TI>(A, X, P, ns, K, ksteps_per_split, zpart, blockIdx.x, blockIdx.y,
+                              blockIdx.z, red);
 }
 
 // One launch over the fragments of a batch: workgroup w belongs to fragment
@@ -174,27 +174,28 @@ __global__ __launch_bounds__(256) void k_kb_project_batch(const KbProjectArgs* _
   const KbProjectArgs a = args[f];
   const int local = blockIdx.x - prefix[f];
   const int bx = local % a.tp, r = local / a.tp;
-  kb_project_tile<TA, TI>(reinterpret_cast<const TA*>(a.A), a.X, a.P, a.ns, a.K, a.per, a.zpart,
-                          bx, r % a.ts, r / a.ts, red);
+  kb_project_tile<TA, double, TI>(reinterpret_cast<const TA*>(a.A), a.X, a.P, a.ns, a.K, a.per,
+                                  a.zpart, bx, r % a.ts, r / a.ts, red);
 }
 
-// Z[p][s] = sum_z part[z][p][s];  Y[p][s] = -2 E_p Z[p][s]
+// Z[p][s] = sum_z part[z][p][s];  Y[s][p] = -2 E_p Z[p][s] (transposed, row
+// stride ppad; the padding columns p >= P are never written and stay zero)
 __device__ __forceinline__ void nl_reduce_elem(const double* __restrict__ part, int nsplit, int P,
-                                               int ns, const double* __restrict__ E,
+                                               int ns, int ppad, const double* __restrict__ E,
                                                double* __restrict__ Z, double* __restrict__ Y,
                                                int i) {
   if (i >= P * ns) return;
   double s = 0.0;
   for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * P * ns + i];
-  const int p = i / ns;
+  const int p = i / ns, st = i - p * ns;
   Z[i] = s;
-  if (Y) Y[i] = -2.0 * E[p] * s;
+  if (Y) Y[(size_t)st * ppad + p] = -2.0 * E[p] * s;
 }
 
-__global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, int ns,
+__global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, int ns, int ppad,
                             const double* __restrict__ E, double* __restrict__ Z,
                             double* __restrict__ Y) {
-  nl_reduce_elem(part, nsplit, P, ns, E, Z, Y, blockIdx.x * blockDim.x + threadIdx.x);
+  nl_reduce_elem(part, nsplit, P, ns, ppad, E, Z, Y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_nl_reduce_batch(const KbReduceArgs* __restrict__ args,
@@ -202,94 +203,15 @@ __global__ __launch_bounds__(256) void k_nl_reduce_batch(const KbReduceArgs* __r
                                                          int nfrag) {
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const KbReduceArgs a = args[f];
-  nl_reduce_elem(a.zpart, a.nsplit, a.P, a.ns, a.E, a.Z, a.Y,
+  nl_reduce_elem(a.zpart, a.nsplit, a.P, a.ns, a.ppad, a.E, a.Z, a.Y,
                  (blockIdx.x - prefix[f]) * 256 + threadIdx.x);
 }
 
-// ---- F[s][k] = sum_p Y[p][s] A[p][k] ---------------------------------------
-// grid (ceil(ns/32), ceil(K2/32)): one 32 (s) x 32 (k) tile per workgroup; its
-// 4 waves take interleaved 32-wide slices of the reduction index p and their
-// partial tiles are summed through LDS.  Loads are coalesced across the 16
-// lanes of a group (consecutive s for Y, consecutive k for A).
-template <typename TA, typename TI>
-__device__ __forceinline__ void kb_force_tile(const double* __restrict__ Y,
-                                              const TA* __restrict__ A, int P, int ns, int K2,
-                                              double* __restrict__ F, int bx, int by,
-                                              double (*red)[32 * 32]) {
-  typedef typename Mfma<TI>::acc_t acc_t;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int s0 = bx * 32;
-  const int k0 = by * 32;
-  const int g = lane >> 4, r16 = lane & 15;
-  acc_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
-  bool sok[2], kok[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    sok[i] = (s0 + 16 * i + r16) < ns;
-    kok[i] = (k0 + 16 * i + r16) < K2;
-  }
-  TI a0[2][KPL], b0[2][KPL], a1[2][KPL], b1[2][KPL];
-  auto load_step = [&](int pb, TI (&a)[2][KPL], TI (&b)[2][KPL]) {
-#pragma unroll
-    for (int t = 0; t < KPL; ++t) {
-      const int p = pb + KPL * g + t;
-      const bool pok = p < P;
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        a[i][t] = (pok && sok[i]) ? TI(Y[(size_t)p * ns + s0 + 16 * i + r16]) : TI(0);
-        b[i][t] = (pok && kok[i]) ? TI(A[(size_t)p * K2 + k0 + 16 * i + r16]) : TI(0);
-      }
-    }
-  };
-  auto mma_step = [&](TI (&a)[2][KPL], TI (&b)[2][KPL]) {
-#pragma unroll
-    for (int t = 0; t < KPL; ++t)
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<TI>::run(a[i][t], b[j][t], acc[i][j]);
-  };
-  int pb = wave * KS;
-  if (pb < P) load_step(pb, a0, b0);
-  while (pb < P) {
-    const int nx = pb + 4 * KS;
-    if (nx < P) load_step(nx, a1, b1);
-    mma_step(a0, b0);
-    pb = nx;
-    if (pb >= P) break;
-    const int nx2 = pb + 4 * KS;
-    if (nx2 < P) load_step(nx2, a0, b0);
-    mma_step(a1, b1);
-    pb = nx2;
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        red[wave][(16 * i + Mfma<TI>::row(lane, r)) * 32 + 16 * j + r16] = double(acc[i][j][r]);
-  __syncthreads();
-  for (int e = tid; e < 32 * 32; e += 256) {
-    const int s = s0 + (e >> 5), k = k0 + (e & 31);
-    if (s < ns && k < K2) F[(size_t)s * K2 + k] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-  }
-}
-
-template <typename TA, typename TI>
-__global__ __launch_bounds__(256) void k_kb_force(const double* __restrict__ Y,
-                                                  const TA* __restrict__ A, int P, int ns, int K2,
-                                                  double* __restrict__ F) {
-  __shared__ double red[4][32 * 32];
-  kb_force_tile<TA, TI>(Y, A, P, ns, K2, F, blockIdx.x, blockIdx.y, red);
-}
-
+// ---- F[s][k] = sum_p Y[s][p] At[k][p] --------------------------------------
+// The same tile as the projection with Y (states x ppad) as the row operand
+// and At (coefficient rows x ppad) as the column operand: both are read in
+// runs of 8 consecutive projectors per lane, and the K dimension (ppad, a
+// multiple of 32) is not split, so the tile writes F directly.
 template <typename TA, typename TI>
 __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __restrict__ args,
                                                         const int* __restrict__ prefix,
@@ -298,8 +220,8 @@ __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __res
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const KbForceArgs a = args[f];
   const int local = blockIdx.x - prefix[f];
-  kb_force_tile<TA, TI>(a.Y, reinterpret_cast<const TA*>(a.A), a.P, a.ns, a.K2, a.F,
-                        local % a.ts, local / a.ts, red);
+  kb_project_tile<double, TA, TI>(a.Y, reinterpret_cast<const TA*>(a.At), a.ns, a.K2, a.ppad,
+                                  a.ppad / KS, a.F, local % a.ts, local / a.ts, 0, red);
 }
 
 static int ensure(void** p, size_t* have, size_t need) {
@@ -338,29 +260,34 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
   int rc = ensure(reinterpret_cast<void**>(&c.zpart), &c.zpart_bytes,
                   sizeof(double) * (size_t)nsplit * P * ns);
   if (rc) return rc;
-  if (c.Y_bytes < sizeof(double) * (size_t)P * ns) {
+  // Y is [ns][ppad] with zero padding columns: (re)allocated and cleared
+  // whenever it grows or the padded projector count changes
+  const size_t ybytes = sizeof(double) * (size_t)c.ppad * ns;
+  if (c.Y_bytes < ybytes || c.Y_ppad != c.ppad) {
     if (c.Y) SDFT_HIP_CHECK(hipFree(c.Y));
     if (c.Z) SDFT_HIP_CHECK(hipFree(c.Z));
     c.Y = c.Z = nullptr;
     c.Y_bytes = 0;
-    if (hipMalloc(&c.Y, sizeof(double) * (size_t)P * ns) != hipSuccess ||
-        hipMalloc(&c.Z, sizeof(double) * (size_t)P * ns) != hipSuccess) {
+    if (hipMalloc(&c.Y, ybytes) != hipSuccess ||
+        hipMalloc(&c.Z, sizeof(double) * (size_t)c.ppad * ns) != hipSuccess) {
       set_error("device allocation failed (KB projections)");
       return -3;
     }
-    c.Y_bytes = sizeof(double) * (size_t)P * ns;
+    SDFT_HIP_CHECK(hipMemsetAsync(c.Y, 0, ybytes, c.stream));
+    c.Y_bytes = ybytes;
+    c.Y_ppad = c.ppad;
   }
   dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_project<double, double>), grid, dim3(256), 0, c.stream, c.A64, X, P,
-                       ns, K, per, c.zpart);
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, c.A64,
+                       X, P, ns, K, per, c.zpart);
   else
-    hipLaunchKernelGGL((k_kb_project<float, float>), grid, dim3(256), 0, c.stream, c.A32, X, P, ns,
-                       K, per, c.zpart);
+    hipLaunchKernelGGL((k_kb_project<float, double, float>), grid, dim3(256), 0, c.stream, c.A32,
+                       X, P, ns, K, per, c.zpart);
   SDFT_HIP_CHECK(hipGetLastError());
   const int tot = P * ns;
   hipLaunchKernelGGL(k_nl_reduce, dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
-                     P, ns, c.E, c.Z, c.Y);
+                     P, ns, c.ppad, c.E, c.Z, c.Y);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -417,12 +344,12 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
     hipLaunchKernelGGL(k_kb_deriv_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        c.stream, c.A64, gint + (size_t)d * c.g.ncp, P, c.g.ncp, Ad);
     dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
-    hipLaunchKernelGGL((k_kb_project<double, double>), grid, dim3(256), 0, c.stream, Ad, X, P, ns,
-                       K, per, part);
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, Ad,
+                       X, P, ns, K, per, part);
     // reduce the split partials into Zd[d] (the E factor is applied later)
     const int n2 = P * ns;
     hipLaunchKernelGGL(k_nl_reduce, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, part, nsplit,
-                       P, ns, c.E, Zd + (size_t)d * n2, (double*)nullptr);
+                       P, ns, c.ppad, c.E, Zd + (size_t)d * n2, (double*)nullptr);
     SDFT_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_force_sums, dim3(P), dim3(256), 0, c.stream, c.Z, Zd, P, ns, nstate, S);
@@ -430,16 +357,18 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
   return 0;
 }
 
+// F[s][k] = sum_p Y[s][p] At[k][p]: the projection tile with the reduction
+// over the padded projector index, unsplit (it writes F directly)
 int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
-  const int P = c.nproj;
   const int K2 = 2 * c.g.ncp;
-  dim3 grid((ns + 31) / 32, (K2 + 31) / 32);
+  dim3 grid((ns + 31) / 32, (K2 + 31) / 32, 1);
+  const int per = c.ppad / KS;
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_force<double, double>), grid, dim3(256), 0, c.stream, c.Y, c.A64, P,
-                       ns, K2, fnl);
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, c.Y,
+                       c.At64, ns, K2, c.ppad, per, fnl);
   else
-    hipLaunchKernelGGL((k_kb_force<float, float>), grid, dim3(256), 0, c.stream, c.Y, c.A32, P, ns,
-                       K2, fnl);
+    hipLaunchKernelGGL((k_kb_project<double, float, float>), grid, dim3(256), 0, c.stream, c.Y,
+                       c.At32, ns, K2, c.ppad, per, fnl);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

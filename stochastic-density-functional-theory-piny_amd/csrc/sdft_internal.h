@@ -86,6 +86,11 @@ struct Ctx {
   int nproj = 0;
   double* A64 = nullptr;     // [nproj][2 ncp]
   float* A32 = nullptr;      // [nproj][2 ncp]
+  // A transposed, projectors padded with zeros to ppad (multiple of 32):
+  // the force product's operand, read in runs of consecutive projectors
+  double* At64 = nullptr;    // [2 ncp][ppad]
+  float* At32 = nullptr;     // [2 ncp][ppad]
+  int ppad = 0;
   double* E = nullptr;       // [nproj]
   std::vector<double> E_host;
   // scratch (grown on demand)
@@ -93,9 +98,10 @@ struct Ctx {
   size_t W_bytes = 0;
   double* zpart = nullptr;   // split-K partials of Z
   size_t zpart_bytes = 0;
-  double* Y = nullptr;       // [nproj][nstate_pad] = -2 E Z
+  double* Y = nullptr;       // [nstate_pad][ppad] = -2 E Z, transposed, zero padded
   double* Z = nullptr;       // [nproj][nstate_pad]
   size_t Y_bytes = 0;
+  int Y_ppad = 0;
   double* red = nullptr;     // reduction scratch
   size_t red_bytes = 0;
   // profiling
@@ -197,16 +203,16 @@ struct KbProjectArgs {
 
 struct KbReduceArgs {
   const double* zpart;
-  int nsplit, P, ns;
+  int nsplit, P, ns, ppad;
   const double* E;
   double* Z;
-  double* Y;
+  double* Y;  // [ns][ppad]
 };
 
 struct KbForceArgs {
-  const double* Y;
-  const void* A;
-  int P, ns, K2, ts, tk;
+  const double* Y;  // [ns][ppad]
+  const void* At;   // [K2][ppad] (double or float)
+  int ns, K2, ppad, ts, tk;
   double* F;
 };
 
@@ -233,6 +239,9 @@ int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
 int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups);
 int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                        int nstate);
+int launch_grid_plane(Ctx& c, void* W, int npairs, const void* field);
+int launch_scatter_proj(Ctx& c, const long long* off, const int* idx, const double* val, int p0,
+                        int p1, double* field);
 int launch_rho_finish(Ctx& c, const double* rho_part, int ngroups, double scale,
                       int layout, int accumulate, double* rho);
 int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in, double* out,

@@ -286,12 +286,15 @@ size_t noise_lds_bytes(int M) {
 
 // per (pair, a-plane): forward FFT along c of the generated rows (keeping
 // |kc| <= K), then along b (keeping |kb| <= K), into W's sphere columns.
+// With `field` non-null the rows are read from a dense pair grid instead
+// (field[q][ra][rb][rc], FFTW order): the real-space projector path.
 template <int N>
 __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict__ W, int ncolp,
                                                           const int* __restrict__ rowcol, int M,
                                                           const Cx<double>* __restrict__ twg,
                                                           unsigned long long seed, int first,
-                                                          int nstate) {
+                                                          int nstate,
+                                                          const Cx<double>* __restrict__ field) {
   typedef double T;
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML;
   constexpr int P = n_passes(N);
@@ -312,6 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
     const int nl = N - rb0 < ML ? N - rb0 : ML;
     auto gen = [&](int l, int pos) {
       const unsigned long long r = ((unsigned long long)ra * N + rb0 + l) * N + pos;
+      if (field) return field[(size_t)q * N * N * N + r];
       const double x1 = amp * rademacher(seed, (unsigned long long)(first + o1), r);
       const double x2 = has2 ? amp * rademacher(seed, (unsigned long long)(first + o2), r) : 0.0;
       return cx(x1, x2);
@@ -825,7 +829,7 @@ int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngro
 
 template <int N>
 static int noise_impl(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
-                      int nstate) {
+                      int nstate, const void* field) {
   const Geometry& g = c.g;
   const size_t lds = noise_lds_bytes<N>(g.M);
   if (lds > 160 * 1024) {
@@ -837,14 +841,50 @@ static int noise_impl(Ctx& c, void* W, int npairs, unsigned long long seed, int 
   dim3 grid(N, npairs);
   hipLaunchKernelGGL((k_noise_plane<N>), grid, dim3(kThreads), lds, c.stream,
                      reinterpret_cast<Cx<double>*>(W), g.ncolp, c.t.rowcol, g.M,
-                     reinterpret_cast<const Cx<double>*>(c.t.tw64), seed, first, nstate);
+                     reinterpret_cast<const Cx<double>*>(c.t.tw64), seed, first, nstate,
+                     reinterpret_cast<const Cx<double>*>(field));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
 int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                        int nstate) {
-  SDFT_SIZE_SWITCH(c.g.n, return (noise_impl<NN>(c, W, npairs, seed, first, nstate)));
+  SDFT_SIZE_SWITCH(c.g.n, return (noise_impl<NN>(c, W, npairs, seed, first, nstate, nullptr)));
+  return 0;
+}
+
+int launch_grid_plane(Ctx& c, void* W, int npairs, const void* field) {
+  SDFT_SIZE_SWITCH(c.g.n, return (noise_impl<NN>(c, W, npairs, 0, 0, 2 * npairs, field)));
+  return 0;
+}
+
+// field[(p / 2) N + index] .x (p even) / .y (p odd) = value for the entries of
+// projectors p0 <= p < p1 (CSR offsets); the caller clears the field first
+__global__ void k_scatter_proj(const long long* __restrict__ off, const int* __restrict__ idx,
+                               const double* __restrict__ val, int p0, int p1, long long N,
+                               double* __restrict__ field) {
+  const long long e0 = off[p0], e1 = off[p1];
+  for (long long e = e0 + (long long)blockIdx.x * blockDim.x + threadIdx.x; e < e1;
+       e += (long long)gridDim.x * blockDim.x) {
+    // projector of entry e: binary search over off[p0..p1]
+    int lo = p0, hi = p1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (off[mid] <= e)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int q = (lo - p0) >> 1;
+    field[((size_t)q * N + idx[e]) * 2 + ((lo - p0) & 1)] = val[e];
+  }
+}
+
+int launch_scatter_proj(Ctx& c, const long long* off, const int* idx, const double* val, int p0,
+                        int p1, double* field) {
+  hipLaunchKernelGGL(k_scatter_proj, dim3(1024), dim3(256), 0, c.stream, off, idx, val, p0, p1,
+                     c.g.N, field);
+  SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
 

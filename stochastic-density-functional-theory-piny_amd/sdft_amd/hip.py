@@ -41,7 +41,7 @@ EXPORTS = [
     "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
-    "sdft_hip_batch_autotune", "sdft_hip_batch_config",
+    "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
 ]
 
 _LIB = None
@@ -93,6 +93,8 @@ def load(path: str | None = None):
         "sdft_hip_batch_filter": ([vp, C.c_int, _dp, _dp, _dp, C.c_int], C.c_int),
         "sdft_hip_batch_autotune": ([vp, C.c_int, _ip, _ip, C.c_int, _dp], C.c_int),
         "sdft_hip_batch_config": ([vp, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
+                                          C.c_double], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -174,6 +176,16 @@ class Context:
         pi = np.ascontiguousarray(proj.imag, dtype=np.float64)
         e = np.ascontiguousarray(energy, dtype=np.float64)
         _check(self.lib.sdft_hip_set_projectors(self.h, pr.shape[0], _ptr(pr), _ptr(pi), _ptr(e)))
+
+    def set_projectors_real(self, rs):
+        """Real-space KB projectors (systems.RealSpaceProjectors)."""
+        off = np.ascontiguousarray(rs.offsets, dtype=np.int64)
+        idx = np.ascontiguousarray(rs.index, dtype=np.int32)
+        val = np.ascontiguousarray(rs.values, dtype=np.float64)
+        e = np.ascontiguousarray(rs.energy, dtype=np.float64)
+        _check(self.lib.sdft_hip_set_projectors_real(
+            self.h, len(e), off.ctypes.data_as(C.POINTER(C.c_longlong)),
+            idx.ctypes.data_as(_ip), _ptr(val), _ptr(e), float(rs.vol_elem)))
 
     def clear_projectors(self):
         _check(self.lib.sdft_hip_set_projectors(self.h, 0, None, None, None))

@@ -272,6 +272,62 @@ def kb_table(L: float, pos: np.ndarray, k: np.ndarray, rc_nl: float,
             np.array(ls)[order], np.array(ms)[order])
 
 
+@dataclasses.dataclass
+class RealSpaceProjectors:
+    """The real-space KB tables of PSEUDO_REAL (typedefs_cp.h: numGridNlppMap,
+    gridNlppMap, vnlPhiAtomGridRe/Im, vpsNormList, volElem) as one CSR list
+    over projectors: projector p has values[offsets[p]:offsets[p+1]] at the
+    FFTW linear grid indices index[...] and KB energy energy[p]."""
+    offsets: np.ndarray    # (nproj + 1,) int64
+    index: np.ndarray      # (nent,) int32
+    values: np.ndarray     # (nent,) beta_p(r)
+    energy: np.ndarray     # (nproj,) h_p
+    vol_elem: float        # V / N
+    atom: np.ndarray       # (nproj,) atom of projector p
+
+
+def real_space_projectors(sysm, rcut: float = 4.0) -> RealSpaceProjectors:
+    """The system's KB projectors sampled on the FFT grid: for every atom the
+    grid points within rcut (minimum image, rcut < L/2) and, in the order of
+    kb_table (atom, l, m), beta(r) = N_l r^l e^{-r^2/(2 rc^2)} Y_lm(r^) with
+    real Y_lm (so the reference's m > 0 factor 2 is inside the harmonics) and
+    energy h_l."""
+    n = sysm.n[0]
+    L = sysm.L
+    h = L / n
+    if not (2.0 * rcut < L):
+        raise ValueError("rcut must be below half the cell")
+    rc = sysm.rc_nl
+    offs, idx, vals, en, atm = [0], [], [], [], []
+    for a, x in enumerate(sysm.pos):
+        lo = np.floor((x - rcut) / h).astype(int)
+        hi = np.ceil((x + rcut) / h).astype(int)
+        ii = [np.arange(lo[d], hi[d] + 1) for d in range(3)]
+        I, J, K = np.meshgrid(ii[0], ii[1], ii[2], indexing="ij")
+        d = np.stack([I.ravel() * h - x[0], J.ravel() * h - x[1], K.ravel() * h - x[2]], axis=1)
+        r2 = (d * d).sum(axis=1)
+        sel = r2 <= rcut * rcut
+        d, r2 = d[sel], r2[sel]
+        lin = ((np.mod(I.ravel()[sel], n) * n + np.mod(J.ravel()[sel], n)) * n
+               + np.mod(K.ravel()[sel], n)).astype(np.int32)
+        g = np.exp(-r2 / (2.0 * rc * rc))
+        for l, nm in ((0, 1), (1, 3)):
+            norm = 1.0 / math.sqrt(math.gamma(l + 1.5) * rc ** (2 * l + 3) / 2.0)
+            for m in range(nm):
+                if l == 0:
+                    v = norm * g / math.sqrt(4.0 * math.pi)
+                else:
+                    v = norm * math.sqrt(3.0 / (4.0 * math.pi)) * d[:, m] * g
+                idx.append(lin)
+                vals.append(v)
+                offs.append(offs[-1] + lin.size)
+                en.append(sysm.h_kb[l])
+                atm.append(a)
+    return RealSpaceProjectors(np.array(offs, dtype=np.int64), np.concatenate(idx),
+                               np.concatenate(vals), np.array(en), sysm.L ** 3 / float(n ** 3),
+                               np.array(atm))
+
+
 def build_system(name: str, a0: float, rep: int, n: int, g2max: float = 12.0,
                  pos: Optional[np.ndarray] = None, with_nl: bool = True,
                  h_kb: tuple = (2.0, 1.0), cell: Optional[float] = None,

@@ -213,3 +213,31 @@ def test_device_noise_matches_host_stream(hip, name, n, g2, nst, seed, first):
     assert rel(blk.get(1), ref) < 1e-12
     blk.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("nproj", [32, 31])
+def test_hpsi_real_space_kb(hip, si8_system, nproj):
+    """sdft_hip_set_projectors_real (device-built table from the grid-sampled
+    projectors) against the oracle's literal real-space KB
+    (nlppKBRealEnergyFilter restated): FP64 1e-12, FP32 2e-5 relative."""
+    import dataclasses
+
+    from sdft_amd import systems
+    say("case: hpsi real-space KB si8 nproj=%d" % nproj)
+    rs = systems.real_space_projectors(si8_system, 4.0)
+    rs = dataclasses.replace(rs, offsets=rs.offsets[:nproj + 1], energy=rs.energy[:nproj],
+                             atom=rs.atom[:nproj])
+    ctx = hip.Context(si8_system, with_nl=False)
+    ctx.set_projectors_real(rs)
+    C = systems.noise_orbitals(si8_system, 5, seed=21)
+    ref = O.hpsi_raw(C, si8_system, nl_real=rs)
+    assert rel(ctx.hpsi_raw(C), ref) < 1e-12
+    assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
+    # a short filter through the same table
+    emin, emax = bounds(si8_system)
+    L = 12
+    coeffs = O.cheby_coeffs(L, emin - 1.0, emax + 1.0, 0.2, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin + 2.0), 0.5 * (emin + emax)
+    acc_g, _ = ctx.cheby_filter(C, coeffs, scale, emean)
+    acc, _ = O.filter_cheby(C, coeffs, si8_system, scale, emean, nl_real=rs)
+    assert rel(acc_g, acc) < 1e-10
