@@ -264,8 +264,8 @@ static void free_ctx(Ctx& c) {
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
                   c.At64,       c.At32,
-                  c.E,          c.W,            c.zpart,       c.Y,            c.Z,
-                  c.red};
+                  c.E,          c.W,            c.W2,          c.zpart,        c.Y,
+                  c.Z,          c.red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c.ev_cur) (void)hipEventDestroy(c.ev_cur);
@@ -460,14 +460,15 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
     ProfScope ps(c, "expand_a");
     if ((rc = launch_expand(c, b.prec, cur, npairs, c.W))) return rc;
   }
+  void* Wres = c.W;
   {
     ProfScope ps(c, "plane_bc_vpsi");
-    if ((rc = launch_plane(c, b.prec, c.W, npairs))) return rc;
+    if ((rc = launch_plane(c, b.prec, c.W, npairs, &Wres))) return rc;
   }
   if (overlap) SDFT_HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_nl, 0));
   {
     ProfScope ps(c, "contract_a");
-    if ((rc = launch_contract(c, b.prec, mode, c.W, npairs, cur, out, b.acc, fnl, b.chi, sp,
+    if ((rc = launch_contract(c, b.prec, mode, Wres, npairs, cur, out, b.acc, fnl, b.chi, sp,
                               mom_part)))
       return rc;
   }

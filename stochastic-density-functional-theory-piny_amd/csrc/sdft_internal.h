@@ -96,6 +96,8 @@ struct Ctx {
   // scratch (grown on demand)
   void* W = nullptr;
   size_t W_bytes = 0;
+  void* W2 = nullptr;        // output of the split-plane form
+  size_t W2_bytes = 0;
   double* zpart = nullptr;   // split-K partials of Z
   size_t zpart_bytes = 0;
   double* Y = nullptr;       // [nstate_pad][ppad] = -2 E Z, transposed, zero padded
@@ -154,6 +156,7 @@ struct ExpandArgs {
 
 struct PlaneArgs {
   void* W;
+  void* Wout;  // split-plane form only: the output buffer (W stays the input)
   int ncolp;
   const int* rowcol;
   int M;
@@ -232,7 +235,10 @@ __device__ inline int find_frag(const int* __restrict__ prefix, int nfrag, int w
 // ---- launchers (sdft_kernels.hip) ----------------------------------------
 size_t w_bytes(const Ctx& c, int npairs, int prec);
 int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W);
-int launch_plane(Ctx& c, int prec, void* W, int npairs);
+// *Wout receives the buffer holding the result: W itself, or the context's
+// second buffer W2 when the plane runs in the split form (M x (n+1) plane
+// buffer above the LDS size)
+int launch_plane(Ctx& c, int prec, void* W, int npairs, void** Wout);
 int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
                     const double* Xcur, double* Xout, double* acc, const double* fnl,
                     const double* chi, const StepParams& sp, double* mom_part);

@@ -260,6 +260,111 @@ __global__ __launch_bounds__(kThreads) void k_plane_batch(const PlaneArgs* __res
   plane_body<T, N>(a, local % a.gx, local / a.gx);
 }
 
+// ---- K2, split form: for planes whose M x (n+1) buffer exceeds LDS --------
+// The plane is processed in two column halves (rc in [h n/2, (h+1) n/2)).
+// Per half the inverse c-transform of the M rows is recomputed from W (ML
+// rows at a time through Q), only the half's columns are kept in LDS
+// (M x (n/2+1)), their b-lines run as in plane_body, and the forward
+// c-transform of the rows with the other half's entries zero is stored
+// (h = 0) or added (h = 1) to Wout: the forward transform is linear in its
+// input, and W (the input of both halves) is left intact.
+template <typename T, int N>
+struct SplitCfg {
+  static constexpr int H = N / 2, LDH = H + 1;
+  static constexpr int QL = Cfg<N>::ML > PlaneCfg<T, N>::CB ? Cfg<N>::ML : PlaneCfg<T, N>::CB;
+};
+
+template <typename T, int N>
+size_t plane_split_lds_bytes(int M) {
+  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N>::LDH + (size_t)SplitCfg<T, N>::QL *
+                          Cfg<N>::LD + N);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int q) {
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
+  constexpr int H = SplitCfg<T, N>::H, LDH = SplitCfg<T, N>::LDH, QL = SplitCfg<T, N>::QL;
+  constexpr int P = n_passes(N);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
+  Cx<T>* __restrict__ Wo = reinterpret_cast<Cx<T>*>(a.Wout);
+  const int ncolp = a.ncolp, M = a.M;
+  const int* __restrict__ rowcol = a.rowcol;
+  const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
+  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
+  Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows x the half's columns
+  Cx<T>* Q = Ph + M * LDH;                         // QL lines of length N
+  Cx<T>* tw = Q + QL * LD;
+  const int tid = threadIdx.x;
+  const int K = (M - 1) / 2;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  __syncthreads();
+  const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+  Cx<T>* Wq = Wo + ((size_t)q * N + ra) * ncolp;
+  const T* vplane = v + (size_t)ra * N * N;
+  const Cx<T> zero = cx(T(0), T(0));
+  for (int h = 0; h < 2; ++h) {
+    const int c_lo = h * H;
+    // inverse along c, keeping this half's columns
+    for (int r0 = 0; r0 < M; r0 += ML) {
+      const int nl = M - r0 < ML ? M - r0 : ML;
+      const int* rc = rowcol + (size_t)r0 * N;
+      auto lw = [&](int l, int pos) {
+        const int d = rc[l * N + pos];
+        return d >= 0 ? Wp[d] : zero;
+      };
+      auto sh = [&](int l, int pos, Cx<T> x) {
+        const int c = pos - c_lo;
+        if (c >= 0 && c < H) Ph[(r0 + l) * LDH + c] = x;
+      };
+      fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lw, sh);
+    }
+    // along b, CB columns at a time
+    for (int rc0 = 0; rc0 < H; rc0 += CB) {
+      const int nb = H - rc0 < CB ? H - rc0 : CB;
+      auto lp = [&](int l, int pos) {
+        const int kb = pos <= N / 2 ? pos : pos - N;
+        return (kb >= -K && kb <= K) ? Ph[(kb + K) * LDH + rc0 + l] : zero;
+      };
+      auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
+      auto sq = [&](int l, int pos, Cx<T> x) { Q[l * LD + pos] = x; };
+      auto vf = [&](int l, int pos) { return vplane[pos * N + c_lo + rc0 + l]; };
+      auto sp = [&](int l, int pos, Cx<T> x) {
+        const int kb = pos <= N / 2 ? pos : pos - N;
+        if (kb >= -K && kb <= K) Ph[(kb + K) * LDH + rc0 + l] = x;
+      };
+      if constexpr (P >= 2)
+        fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true>(Q, LD, nb, tw, tid, lp, sq);
+      fft_fused_vmul<T, N, CB, kThreads, true>(nb, tw, tid, lq, vf, sq);
+      if constexpr (P >= 2)
+        fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lq, sp);
+    }
+    // forward along c of the rows with the other half zero: store / add
+    for (int r0 = 0; r0 < M; r0 += ML) {
+      const int nl = M - r0 < ML ? M - r0 : ML;
+      const int* rc = rowcol + (size_t)r0 * N;
+      auto lb = [&](int l, int pos) {
+        const int c = pos - c_lo;
+        return (c >= 0 && c < H) ? Ph[(r0 + l) * LDH + c] : zero;
+      };
+      auto sw = [&](int l, int pos, Cx<T> x) {
+        const int d = rc[l * N + pos];
+        if (d < 0) return;
+        if (h == 0)
+          Wq[d] = x;
+        else
+          Wq[d] = Wq[d] + x;
+      };
+      fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lb, sw);
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_plane_split(PlaneArgs a) {
+  plane_split_body<T, N>(a, blockIdx.x, blockIdx.y);
+}
+
 // ---- noise: real-space Rademacher -> sphere columns ----------------------
 // genNoiseOrbitalReal (stodft/gen-noise.c:162-374): real-space values
 // +-sqrt(N) (ranValue = sqrt(nfft2), :186-195), transformed to G space.  The
@@ -280,28 +385,32 @@ __device__ inline double rademacher(unsigned long long seed, unsigned long long 
 }
 
 template <int N>
-size_t noise_lds_bytes(int M) {
-  return sizeof(Cx<double>) * ((size_t)Cfg<N>::ML * Cfg<N>::LD + (size_t)M * Cfg<N>::LD + N);
+size_t noise_lds_bytes(int mc) {
+  return sizeof(Cx<double>) * ((size_t)Cfg<N>::ML * Cfg<N>::LD + (size_t)mc * Cfg<N>::LD + N);
 }
 
 // per (pair, a-plane): forward FFT along c of the generated rows (keeping
 // |kc| <= K), then along b (keeping |kb| <= K), into W's sphere columns.
 // With `field` non-null the rows are read from a dense pair grid instead
-// (field[q][ra][rb][rc], FFTW order): the real-space projector path.
+// (field[q][ra][rb][rc], FFTW order): the real-space projector path.  The
+// kc lines are produced mc at a time (mc = M, or fewer when M lines of the
+// plane do not fit LDS: the rows are then generated and c-transformed once
+// per chunk of kc).
 template <int N>
 __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict__ W, int ncolp,
                                                           const int* __restrict__ rowcol, int M,
                                                           const Cx<double>* __restrict__ twg,
                                                           unsigned long long seed, int first,
                                                           int nstate,
-                                                          const Cx<double>* __restrict__ field) {
+                                                          const Cx<double>* __restrict__ field,
+                                                          int mc) {
   typedef double T;
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* Rb = reinterpret_cast<Cx<T>*>(smem_raw);  // ML rows (rb) of the plane
-  Cx<T>* Bk = Rb + ML * LD;                        // M lines (kc) along b
-  Cx<T>* tw = Bk + M * LD;
+  Cx<T>* Bk = Rb + ML * LD;                        // mc lines (kc) along b
+  Cx<T>* tw = Bk + mc * LD;
   const int tid = threadIdx.x;
   const int ra = blockIdx.x;
   const int q = blockIdx.y;
@@ -311,6 +420,9 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
   const int o1 = 2 * q, o2 = 2 * q + 1;
   const double amp = sqrt((double)N * N * N);
   const bool has2 = o2 < nstate;
+  Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+  for (int k0 = 0; k0 < M; k0 += mc) {  // kc lines k0 - K .. k0 + mc - 1 - K
+  const int mk = M - k0 < mc ? M - k0 : mc;
   for (int rb0 = 0; rb0 < N; rb0 += ML) {
     const int nl = N - rb0 < ML ? N - rb0 : ML;
     auto gen = [&](int l, int pos) {
@@ -322,14 +434,14 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
     };
     auto sk = [&](int l, int pos, Cx<T> x) {
       const int kc = pos <= N / 2 ? pos : pos - N;
-      if (kc >= -K && kc <= K) Bk[(kc + K) * LD + rb0 + l] = x;
+      const int i = kc + K - k0;
+      if (i >= 0 && i < mk) Bk[i * LD + rb0 + l] = x;
     };
     fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(Rb, LD, nl, tw, tid, gen, sk);
   }
-  Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
-  for (int c0 = 0; c0 < M; c0 += ML) {
-    const int nl = M - c0 < ML ? M - c0 : ML;
-    Cx<T>* buf = Bk + c0 * LD;
+  for (int c0 = k0; c0 < k0 + mk; c0 += ML) {
+    const int nl = k0 + mk - c0 < ML ? k0 + mk - c0 : ML;
+    Cx<T>* buf = Bk + (c0 - k0) * LD;
     auto lb = [&](int l, int pos) { return buf[l * LD + pos]; };
     auto sw = [&](int l, int pos, Cx<T> x) {
       const int kb = pos <= N / 2 ? pos : pos - N;
@@ -339,6 +451,7 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
       if (d >= 0) Wp[d] = x;
     };
     fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lb, sw);
+  }
   }
 }
 
@@ -551,6 +664,67 @@ __global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ 
   }
 }
 
+// Split form of k_density for planes whose M x (n+1) buffer exceeds LDS: per
+// pair and column half, the inverse c-transform of the M rows (through Q,
+// recomputed per half) keeps the half's columns, and its b-lines accumulate
+// |psi1|^2 + |psi2|^2 straight from the last pass's outputs.
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_density_split(
+    const Cx<T>* __restrict__ W, int ncolp, const int* __restrict__ rowcol, int M,
+    const Cx<T>* __restrict__ twg, double* __restrict__ rho_part, int npairs, int ngroups) {
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
+  constexpr int H = SplitCfg<T, N>::H, LDH = SplitCfg<T, N>::LDH, QL = SplitCfg<T, N>::QL;
+  constexpr int P = n_passes(N);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);
+  Cx<T>* Q = Ph + M * LDH;
+  Cx<T>* tw = Q + QL * LD;
+  const int tid = threadIdx.x;
+  const int ra = blockIdx.x;
+  const int g = blockIdx.y;
+  const int K = (M - 1) / 2;
+  double* rp = rho_part + ((size_t)g * N + ra) * N * N;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  __syncthreads();
+  const Cx<T> zero = cx(T(0), T(0));
+  for (int q = g; q < npairs; q += ngroups) {
+    const bool first = (q == g);
+    const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
+    for (int h = 0; h < 2; ++h) {
+      const int c_lo = h * H;
+      for (int r0 = 0; r0 < M; r0 += ML) {
+        const int nl = M - r0 < ML ? M - r0 : ML;
+        const int* rc = rowcol + (size_t)r0 * N;
+        auto lw = [&](int l, int pos) {
+          const int d = rc[l * N + pos];
+          return d >= 0 ? Wp[d] : zero;
+        };
+        auto sh = [&](int l, int pos, Cx<T> x) {
+          const int c = pos - c_lo;
+          if (c >= 0 && c < H) Ph[(r0 + l) * LDH + c] = x;
+        };
+        fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lw, sh);
+      }
+      for (int rc0 = 0; rc0 < H; rc0 += CB) {
+        const int nb = H - rc0 < CB ? H - rc0 : CB;
+        auto lp = [&](int l, int pos) {
+          const int kb = pos <= N / 2 ? pos : pos - N;
+          return (kb >= -K && kb <= K) ? Ph[(kb + K) * LDH + rc0 + l] : zero;
+        };
+        auto sr = [&](int l, int pos, Cx<T> z) {
+          const double s2 = double(z.x) * double(z.x) + double(z.y) * double(z.y);
+          const size_t idx = (size_t)pos * N + c_lo + rc0 + l;
+          rp[idx] = first ? s2 : rp[idx] + s2;
+        };
+        fft_passes_io<T, N, +1, false, 0, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lp, sr);
+      }
+    }
+  }
+  if (g >= npairs) {
+    for (int e = tid; e < N * N; e += kThreads) rp[e] = 0.0;
+  }
+}
+
 // rho = scale * sum_g rho_part[g] (+ rho), FFTW or generic (a fastest) order
 __global__ void k_rho_finish(const double* __restrict__ part, int ngroups, long long Ntot, int n,
                              double scale, int layout, int accumulate, double* __restrict__ rho) {
@@ -637,6 +811,7 @@ ExpandArgs make_expand_args(const Ctx& c, int prec, const double* X, void* W) {
 PlaneArgs make_plane_args(const Ctx& c, int prec, void* W) {
   PlaneArgs a;
   a.W = W;
+  a.Wout = nullptr;
   a.ncolp = c.g.ncolp;
   a.rowcol = c.t.rowcol;
   a.M = c.g.M;
@@ -749,29 +924,49 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
 }
 
 template <typename T, int N>
-static int plane_impl(Ctx& c, void* W, int npairs, const T* v, const void* tw) {
+static int plane_impl(Ctx& c, void* W, int npairs, void** Wout) {
   const Geometry& g = c.g;
   const size_t lds = plane2_lds_bytes<T, N>(g.M);
-  if (lds > 160 * 1024) {
+  PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
+  dim3 grid(N, npairs);
+  if (lds <= 160 * 1024) {
+    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream, a);
+    SDFT_HIP_CHECK(hipGetLastError());
+    *Wout = W;
+    return 0;
+  }
+  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
+  if (lds2 > 160 * 1024) {
     set_error("plane buffer exceeds LDS for this grid in this precision");
     return SDFT_EINVAL_CODE;
   }
-  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  (void)v;
-  (void)tw;
-  const PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
-  dim3 grid(N, npairs);
-  hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream, a);
+  const size_t need = w_bytes(c, npairs, prec_of<T>());
+  if (c.W2_bytes < need) {
+    if (c.W2) SDFT_HIP_CHECK(hipFree(c.W2));
+    c.W2 = nullptr;
+    c.W2_bytes = 0;
+    if (hipMalloc(&c.W2, need) != hipSuccess) {
+      set_error("device allocation failed (split-plane output)");
+      return -3;
+    }
+    c.W2_bytes = need;
+  }
+  a.Wout = c.W2;
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_split<T, N>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+  hipLaunchKernelGGL((k_plane_split<T, N>), grid, dim3(kThreads), lds2, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
+  *Wout = c.W2;
   return 0;
 }
 
-int launch_plane(Ctx& c, int prec, void* W, int npairs) {
+int launch_plane(Ctx& c, int prec, void* W, int npairs, void** Wout) {
   if (prec == 0) {
-    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<double, NN>(c, W, npairs, c.vks64, c.t.tw64)));
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<double, NN>(c, W, npairs, Wout)));
   } else {
-    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<float, NN>(c, W, npairs, c.vks32, c.t.tw32)));
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_impl<float, NN>(c, W, npairs, Wout)));
   }
   return 0;
 }
@@ -808,8 +1003,18 @@ static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int
   const Geometry& g = c.g;
   const size_t lds = plane_lds_bytes<T, N>(g.M);
   if (lds > 160 * 1024) {
-    set_error("plane buffer exceeds LDS for this grid in this precision");
-    return SDFT_EINVAL_CODE;
+    const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
+    if (lds2 > 160 * 1024) {
+      set_error("plane buffer exceeds LDS for this grid in this precision");
+      return SDFT_EINVAL_CODE;
+    }
+    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_split<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL((k_density_split<T, N>), dim3(N, ngroups), dim3(kThreads), lds2, c.stream,
+                       reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M,
+                       reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
+    SDFT_HIP_CHECK(hipGetLastError());
+    return 0;
   }
   SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density<T, N>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -831,18 +1036,23 @@ template <int N>
 static int noise_impl(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                       int nstate, const void* field) {
   const Geometry& g = c.g;
-  const size_t lds = noise_lds_bytes<N>(g.M);
-  if (lds > 160 * 1024) {
-    set_error("noise plane buffer exceeds LDS for this grid");
-    return SDFT_EINVAL_CODE;
+  // kc lines per chunk: all M when they fit, else the fewest chunks that do
+  int mc = g.M;
+  for (int parts = 1; noise_lds_bytes<N>(mc) > 160 * 1024; ++parts) {
+    mc = (g.M + parts) / (parts + 1);
+    if (mc < 1) {
+      set_error("noise plane buffer exceeds LDS for this grid");
+      return SDFT_EINVAL_CODE;
+    }
   }
+  const size_t lds = noise_lds_bytes<N>(mc);
   SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_noise_plane<N>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid(N, npairs);
   hipLaunchKernelGGL((k_noise_plane<N>), grid, dim3(kThreads), lds, c.stream,
                      reinterpret_cast<Cx<double>*>(W), g.ncolp, c.t.rowcol, g.M,
                      reinterpret_cast<const Cx<double>*>(c.t.tw64), seed, first, nstate,
-                     reinterpret_cast<const Cx<double>*>(field));
+                     reinterpret_cast<const Cx<double>*>(field), mc);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -232,11 +232,17 @@ class Context:
                                           mu_init, gap, occ, tol, C.byref(mu), C.byref(ne)))
         return mu.value, ne.value
 
-    def density(self, Cz, occ_over_nsto, layout=RHO_FFTW):
+    def density(self, Cz, occ_over_nsto, layout=RHO_FFTW, accumulate_into=None):
+        """rho in FFTW or generic order; with accumulate_into (an FP64 array of
+        ngrid) the density is added to it in place and it is returned."""
         re, im = _split(Cz)
-        rho = np.zeros(self.sys.ngrid)
+        if accumulate_into is None:
+            rho = np.zeros(self.sys.ngrid)
+        else:
+            rho = accumulate_into
+            assert rho.dtype == np.float64 and rho.flags.c_contiguous and rho.size == self.sys.ngrid
         _check(self.lib.sdft_hip_density(self.h, re.shape[0], _ptr(re), _ptr(im), occ_over_nsto,
-                                         _ptr(rho), layout, 0))
+                                         _ptr(rho), layout, 0 if accumulate_into is None else 1))
         return rho
 
     def energies(self, Cz, occ, nsto):

@@ -241,3 +241,23 @@ def test_hpsi_real_space_kb(hip, si8_system, nproj):
     acc_g, _ = ctx.cheby_filter(C, coeffs, scale, emean)
     acc, _ = O.filter_cheby(C, coeffs, si8_system, scale, emean, nl_real=rs)
     assert rel(acc_g, acc) < 1e-10
+
+
+def test_density_generic_layout_and_accumulate(hip, si8_ctx, si8_system):
+    """SDFT_RHO_GENERIC: the density in PINY's generic-FFT order (a fastest),
+    which the reference converts to and from the FFTW order by a pure
+    transpose (energy/cp/cp_energy_ee_rho.c:1944-1956: generic index
+    (c n2 + b) n1 + a <-> FFTW index (a n2 + b) n3 + c); accumulate = 1 adds
+    to the caller's array (the reference sums rho over state groups)."""
+    from sdft_amd import systems
+    say("case: density generic layout + accumulate")
+    C = systems.noise_orbitals(si8_system, 5, seed=61)
+    n = si8_system.n[0]
+    rho_f = si8_ctx.density(C, 0.4, layout=hip.RHO_FFTW)
+    rho_g = si8_ctx.density(C, 0.4, layout=hip.RHO_GENERIC)
+    assert np.array_equal(rho_g, rho_f.reshape(n, n, n).transpose(2, 1, 0).ravel())
+    rho_ref = O.density_real(C, si8_system, 0.4)
+    assert rel(rho_f, rho_ref) < 1e-12
+    acc = rho_f.copy()
+    si8_ctx.density(C, 0.4, layout=hip.RHO_FFTW, accumulate_into=acc)
+    assert rel(acc, 2.0 * rho_f) < 1e-15
