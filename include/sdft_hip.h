@@ -97,6 +97,32 @@ int sdft_hip_set_projectors_real(sdft_ctx *ctx, int nproj, const long long *offs
 int sdft_hip_hpsi(sdft_ctx *ctx, int nstate, const double *cre, const double *cim,
                   double *fre, double *fim, int prec);
 
+/* Newton-Hermite filter: filterNewtonPolyHerm (stodft/filter.c:33-312) with
+ * normHNewtonHerm (stodft/normh.c:35-176), PINY's default poly_type.  With
+ * H^ = scale (H - emean) (scale = the Newton map's newtonInfo->scale):
+ * P_0 = chi, P_n = (H^ - z_{n-1}) P_{n-1} for n < L (z = samp_point, L - 1
+ * values), and for each of the nmu chemical potentials
+ *   out[mu] = sum_{n<L} coeffs[n * nmu + mu] P_n
+ * (expanCoeff[iPoly * numChemPot + imu]).  cre/cim hold chi on entry and
+ * P_{L-1} on return; out_re/out_im are nmu consecutive blocks of
+ * nstate * ncoef values (stoWfUpRe/Im[imu]). */
+int sdft_hip_newton_filter(sdft_ctx *ctx, int nstate, int L, int nmu, const double *coeffs,
+                           const double *samp_point, double scale, double emean, double *cre,
+                           double *cim, double *out_re, double *out_im, int prec);
+
+/* Spectral range: genEnergyMax (which = 0) / genEnergyMin (which = 1),
+ * stodft/calc-spectral-range.c:33-333 / :339-566, on the device.  Power
+ * iteration of H (which 0) or of (emax - H) (which 1) on one orbital that
+ * starts from vre/vim (reference order, normalised first): per iteration
+ * E = <phi|H|phi> (Gamma dot), then phi <- H phi (or emax phi - H phi),
+ * normalised; it stops when |E_old - E| <= conv (the reference's energyConv
+ * 1e-5, starting from E_old = 1e7, E = 1e5) or after max_iter iterations.
+ * *energy = the last E (the reference sets energyMax = E + 0.1, energyMin =
+ * E - 0.1); vre/vim receive the final vector (testWfMax / testWfMin);
+ * *iters (nullable) the iteration count. */
+int sdft_hip_spectral_bound(sdft_ctx *ctx, int which, double emax, double conv, int max_iter,
+                            double *vre, double *vim, double *energy, int *iters, int prec);
+
 /* Chebyshev expansion coefficients c_0..c_{L-1} of sqrt(f_FD) (fun_flag 2) or
  * f_FD (fun_flag 1) on [emin, emax]: calcChebyCoeff, stodft/coeff.c:1093-1223
  * (M = roundFFT(32 L) cosine grid, coeff.c:1556-1600). FP64 on the device. */

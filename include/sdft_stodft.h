@@ -41,6 +41,15 @@ typedef struct SDFT_STODFT {
   double *chebyMomentsUp;   /* [polynormLength + 1], 0-based */
   double *chemPot;          /* [1] */
   double *rhoUp;            /* [n1*n2*n3] density output, 0-based, FFTW order */
+  /* Newton-Hermite filter (NEWTONINFO): numChemPot columns of expanCoeff
+   * ([polynormLength * numChemPot], iPoly-major), sample points
+   * [polynormLength - 1], the Newton map's scale; stoWfUpRe/Im[imu] */
+  int numChemPot;
+  double newtonScale;
+  double *sampPoint;
+  /* spectral range start / result vectors (STODFTCOEFPOS testWfMax/Min,
+   * 0-based [numCoeff]) */
+  double *testWfMaxRe, *testWfMaxIm, *testWfMinRe, *testWfMinIm;
 } SDFT_STODFT;
 
 /* stodft/filter.c:318-706: acc = sum_k c_k T_k chi; cre_up left = T_{L-1} chi. */
@@ -60,6 +69,15 @@ double calcNumElecCheby(SDFT_STODFT *st, double chemPot, double *chebyCoeffs);
 void calcChemPotCheby(SDFT_STODFT *st);
 /* stodft/density.c:1276-1520 (real-space part): rhoUp = occ/N_sto sum psi^2. */
 void calcRhoStoHybridCheby(SDFT_STODFT *st);
+/* stodft/filter.c:33-312 with normHNewtonHerm (normh.c:35-176):
+ * stoWfUpRe/Im[imu] = sum_n expanCoeff[n*numChemPot+imu] P_n chi;
+ * cre_up left = P_{L-1} chi (fcre_up likewise, as normHNewtonHerm leaves it). */
+void filterNewtonPolyHerm(SDFT_STODFT *st);
+/* stodft/calc-spectral-range.c:33-333 / :339-566: power iteration from
+ * testWfMax / testWfMin (written back), energyMax = E + 0.1 /
+ * energyMin = E - 0.1 (eigValMin's E), energyConv 1e-5. */
+void genEnergyMax(SDFT_STODFT *st);
+void genEnergyMin(SDFT_STODFT *st);
 
 #ifdef __cplusplus
 }

@@ -216,6 +216,33 @@ def gamma_dot(a, b):
 
 
 # ---------------------------------------------------------------------------
+# Spectral range
+# ---------------------------------------------------------------------------
+
+def gen_energy_bound(v0, sysm, which, emax=0.0, conv=1e-5, max_iter=100000, with_nl=True):
+    """genEnergyMax (which 0) / genEnergyMin (which 1), stodft/calc-spectral-
+    range.c:33-333 / :339-566: normalise phi; energyOld = 1e7, energy = 1e5;
+    while |energyOld - energy| > conv: H phi from the raw force (x -1/4,
+    G = 0 x -1/2), energy = <phi|H phi> (Gamma dot), phi = H phi (or
+    emax phi - H phi), normalised.  Returns (energy, phi, iterations); the
+    reference's bounds are energy + 0.1 / energy - 0.1."""
+    phi = np.array(v0, dtype=np.complex128).reshape(1, -1)
+    phi /= math.sqrt(gamma_dot(phi, phi).sum())
+    e_old, e = 1.0e7, 1.0e5
+    it = 0
+    while abs(e_old - e) > conv and it < max_iter:
+        f = hpsi_raw(phi, sysm, with_nl)
+        h = -0.25 * f
+        h[:, -1] = -0.5 * f[:, -1]
+        e_old = e
+        e = float(gamma_dot(h, phi).sum())
+        w = h if which == 0 else emax * phi - h
+        phi = w / math.sqrt(gamma_dot(w, w).sum())
+        it += 1
+    return e, phi[0], it
+
+
+# ---------------------------------------------------------------------------
 # Chebyshev recurrence and filter
 # ---------------------------------------------------------------------------
 
@@ -268,6 +295,53 @@ def filter_cheby(chi, coeffs, sysm, scale, emean, with_nl=True, moments=False, n
 # ---------------------------------------------------------------------------
 # Chebyshev coefficients (stodft/coeff.c)
 # ---------------------------------------------------------------------------
+
+def newton_filter(chi, coeffs, samp_point, sysm, scale, emean, with_nl=True, nl_real=None):
+    """filterNewtonPolyHerm (stodft/filter.c:33-312) with normHNewtonHerm
+    (stodft/normh.c:35-176): P_0 = chi; P_n = (scale (H - emean) - z_{n-1})
+    P_{n-1}, computed from the raw force as f * (-scale/4) (G != 0, real part
+    at G = 0 * -scale/2) + (-scale emean - z) P (normh.c:101-136); out[mu] =
+    sum_n coeffs[n, mu] P_n.  Returns (out (nmu, nstate, ncoef), P_{L-1})."""
+    co = np.asarray(coeffs, dtype=np.float64)
+    if co.ndim == 1:
+        co = co[:, None]
+    L, nmu = co.shape
+    P = chi.copy()
+    out = np.stack([co[0, m] * chi for m in range(nmu)])
+    for n in range(1, L):
+        f = hpsi_raw(P, sysm, with_nl, nl_real)
+        pref = -scale * emean - samp_point[n - 1]
+        nxt = -0.25 * scale * f + pref * P
+        nxt[:, -1] = (-0.5 * scale * f[:, -1].real + pref * P[:, -1].real) \
+            + 1j * (f[:, -1].imag + pref * P[:, -1].imag)
+        P = nxt
+        for m in range(nmu):
+            out[m] += co[n, m] * P
+    return out, P
+
+
+def newton_coeffs(fun, nodes):
+    """Newton divided differences of fun at nodes (the interpolation form
+    the Newton filter evaluates: p(x) = sum_n a_n prod_{j<n} (x - z_j))."""
+    z = np.asarray(nodes, dtype=np.float64)
+    a = np.array([fun(x) for x in z], dtype=np.float64)
+    for k in range(1, len(z)):
+        a[k:] = (a[k:] - a[k - 1:-1]) / (z[k:] - z[:-k])
+    return a
+
+
+def leja_order(nodes):
+    """Leja ordering (start at the largest |z|, then maximise the product of
+    distances), the stable order for Newton interpolation on an interval."""
+    z = list(np.asarray(nodes, dtype=np.float64))
+    out = [max(z, key=abs)]
+    z.remove(out[0])
+    while z:
+        nxt = max(z, key=lambda x: np.prod([abs(x - y) for y in out]))
+        out.append(nxt)
+        z.remove(nxt)
+    return np.array(out)
+
 
 def round_fft(n_in: int) -> int:
     """roundFFT, coeff.c:1556-1600: nearest 2^n 3^m to n_in by the

@@ -361,14 +361,17 @@ static int block_alloc(Ctx& c, int nstate, int prec, Block& b) {
   b.nstate_pad = nstate + (nstate & 1);
   b.prec = prec;
   const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
+  // the T buffers are the KB projection's column operand: rows padded to a
+  // multiple of 32 (zeros, never written) so that its K loop needs no bounds
+  const size_t xbytes = sizeof(double) * (size_t)((b.nstate_pad + 31) / 32 * 32) * 2 * c.g.ncp;
   int rc;
-  if ((rc = dalloc(&b.X0, bytes))) return rc;
-  if ((rc = dalloc(&b.X1, bytes))) return rc;
+  if ((rc = dalloc(&b.X0, xbytes))) return rc;
+  if ((rc = dalloc(&b.X1, xbytes))) return rc;
   if ((rc = dalloc(&b.acc, bytes))) return rc;
   if ((rc = dalloc(&b.chi, bytes))) return rc;
   if ((rc = dalloc(&b.fnl, bytes))) return rc;
-  SDFT_HIP_CHECK(hipMemsetAsync(b.X0, 0, bytes, c.stream));
-  SDFT_HIP_CHECK(hipMemsetAsync(b.X1, 0, bytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.X0, 0, xbytes, c.stream));
+  SDFT_HIP_CHECK(hipMemsetAsync(b.X1, 0, xbytes, c.stream));
   SDFT_HIP_CHECK(hipMemsetAsync(b.acc, 0, bytes, c.stream));
   SDFT_HIP_CHECK(hipMemsetAsync(b.chi, 0, bytes, c.stream));
   SDFT_HIP_CHECK(hipMemsetAsync(b.fnl, 0, bytes, c.stream));
@@ -430,10 +433,9 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
   int rc;
   if ((rc = ensure_W(c, w_bytes(c, npairs, b.prec)))) return rc;
   const double* fnl = nullptr;
-  // Without profiling the KB chain runs on its own stream, overlapping the
-  // FFT kernels; with profiling everything stays on one stream so that the
-  // per-kernel event times are not inflated by contention.
-  const bool overlap = c.nproj > 0 && !c.profiling;
+  // The KB chain runs on its own stream, overlapping the FFT kernels; with
+  // profiling on, each kernel's events are recorded on the stream it runs on.
+  const bool overlap = c.nproj > 0;
   if (c.nproj > 0) {
     hipStream_t main = c.stream;
     if (overlap) {
@@ -599,8 +601,9 @@ static int set_projectors_impl(Ctx& c, int nproj, const double* pre, const doubl
   if (nproj == 0) return 0;
   const int nc = c.g.ncoef, ncp = c.g.ncp;
   const size_t K = 2 * (size_t)ncp;
-  std::vector<double> A((size_t)nproj * K, 0.0);
-  std::vector<float> A32((size_t)nproj * K, 0.f);
+  const int ppad = (nproj + 31) / 32 * 32;  // zero rows up to ppad (GEMM row padding)
+  std::vector<double> A((size_t)ppad * K, 0.0);
+  std::vector<float> A32((size_t)ppad * K, 0.f);
   for (int p = 0; p < nproj; ++p) {
     for (int j = 0; j < nc; ++j) {
       const int i = c.g.perm[j];
@@ -612,7 +615,6 @@ static int set_projectors_impl(Ctx& c, int nproj, const double* pre, const doubl
     }
   }
   for (size_t i = 0; i < A.size(); ++i) A32[i] = (float)A[i];
-  const int ppad = (nproj + 31) / 32 * 32;
   std::vector<double> At(K * ppad, 0.0);
   std::vector<float> At32(K * ppad, 0.f);
   for (int p = 0; p < nproj; ++p)
@@ -846,7 +848,12 @@ int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* 
   int per = 0;
   const int nsplit = kb_project_split(c, ns, &per);
   if (!rc) rc = upload(&dg, gint);
-  if (!rc) rc = dalloc(&Ad, sizeof(double) * (size_t)P * 2 * ncp);
+  const size_t adbytes = sizeof(double) * (size_t)c.ppad * 2 * ncp;  // zero padding rows
+  if (!rc) rc = dalloc(&Ad, adbytes);
+  if (!rc && hipMemsetAsync(Ad, 0, adbytes, c.stream) != hipSuccess) {
+    set_error("hipMemsetAsync failed");
+    rc = SDFT_EHIP;
+  }
   if (!rc) rc = dalloc(&Zd, sizeof(double) * (size_t)3 * P * ns);
   if (!rc) rc = dalloc(&part, sizeof(double) * (size_t)nsplit * P * ns);
   if (!rc) rc = dalloc(&S, sizeof(double) * (size_t)3 * P);
@@ -892,6 +899,117 @@ int sdft_hip_hpsi(sdft_ctx* h, int nstate, const double* cre, const double* cim,
   if (!rc) rc = prof_collect(c);
   block_free(b);
   return rc;
+}
+
+int sdft_hip_newton_filter(sdft_ctx* h, int nstate, int L, int nmu, const double* coeffs,
+                           const double* samp_point, double scale, double emean, double* cre,
+                           double* cim, double* out_re, double* out_im, int prec) {
+  if (!h || nstate < 1 || L < 1 || nmu < 1 || !coeffs || (L > 1 && !samp_point) || !cre ||
+      !cim || !out_re || !out_im || (prec != 0 && prec != 1)) {
+    set_error("bad Newton filter arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  Block b;
+  if ((rc = block_alloc(c, nstate, prec, b))) {
+    block_free(b);
+    return rc;
+  }
+  const long long stride = (long long)b.nstate_pad * 2 * c.g.ncp;
+  double* accs = nullptr;
+  double* cdev = nullptr;
+  do {
+    if ((rc = dalloc(&accs, sizeof(double) * (size_t)stride * nmu))) break;
+    if ((rc = dalloc(&cdev, sizeof(double) * (size_t)L * nmu))) break;
+    SDFT_HIP_CHECK(hipMemcpyAsync(cdev, coeffs, sizeof(double) * (size_t)L * nmu,
+                                  hipMemcpyHostToDevice, c.stream));
+    if ((rc = block_put(b, b.X0, cre, cim))) break;
+    if (L == 1) {
+      for (int mu = 0; mu < nmu; ++mu) {
+        hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0,
+                           c.stream, b.X0, coeffs[mu], accs + (size_t)mu * stride,
+                           (size_t)stride);
+        SDFT_HIP_CHECK(hipGetLastError());
+      }
+    }
+    // the contract kernel writes accumulator mu at acc + mu * stride: point
+    // the block's accumulator at the nmu-accumulator buffer for the loop
+    c.newton_nmu = nmu;
+    c.newton_stride = stride;
+    c.newton_c = cdev;
+    double* own_acc = b.acc;
+    b.acc = accs;
+    prof_reset(c);
+    double* X[2] = {b.X0, b.X1};
+    // filter.c:151-208: P_n = (H^ - z_{n-1}) P_{n-1}, acc_mu += a[n][mu] P_n
+    for (int n = 1; n < L && !rc; ++n) {
+      StepParams sp;
+      sp.s1 = -scale * 0.25;
+      sp.s2 = -scale * 0.5;
+      sp.p = -scale * emean - samp_point[n - 1];
+      sp.ck = 0.0;
+      sp.c0 = 0.0;
+      sp.first = (n == 1);
+      sp.step = n;
+      rc = apply_step(b, kModeNewton, X[(n - 1) & 1], X[n & 1], sp, nullptr);
+    }
+    b.acc = own_acc;
+    c.newton_nmu = 0;
+    c.newton_stride = 0;
+    c.newton_c = nullptr;
+    if (rc) break;
+    if ((rc = prof_collect(c))) break;
+    const size_t per = (size_t)nstate * c.g.ncoef;
+    for (int mu = 0; mu < nmu && !rc; ++mu)
+      rc = block_fetch(b, accs + (size_t)mu * stride, out_re + mu * per, out_im + mu * per);
+    if (!rc) rc = block_fetch(b, X[(L - 1) & 1], cre, cim);
+  } while (0);
+  c.newton_nmu = 0;
+  c.newton_stride = 0;
+  c.newton_c = nullptr;
+  void* ptrs[] = {accs, cdev};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  block_free(b);
+  return rc;
+}
+
+int sdft_hip_spectral_bound(sdft_ctx* h, int which, double emax, double conv, int max_iter,
+                            double* vre, double* vim, double* energy, int* iters, int prec) {
+  if (!h || (which != 0 && which != 1) || max_iter < 1 || !vre || !vim || !energy ||
+      (prec != 0 && prec != 1) || !(conv >= 0.0)) {
+    set_error("bad spectral-bound arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  Block b;
+  if ((rc = block_alloc(c, 1, prec, b))) {
+    block_free(b);
+    return rc;
+  }
+  // the reference starts from energyOld = 1e7, energy = 1e5 (:168-170) and
+  // iterates while |energyOld - energy| > energyConv
+  double e_old = 1.0e7, e = 1.0e5;
+  int it = 0;
+  StepParams sp{};
+  if (!(rc = block_put(b, b.X0, vre, vim))) rc = normalize0_device(c, b.X0);
+  while (!rc && std::fabs(e_old - e) > conv && it < max_iter) {
+    if ((rc = apply_step(b, kModeForce, b.X0, b.X1, sp, nullptr))) break;
+    e_old = e;
+    if ((rc = power_update_device(c, b.X0, b.X1, which, emax, &e))) break;
+    ++it;
+  }
+  if (!rc) rc = block_fetch(b, b.X0, vre, vim);
+  if (!rc) rc = prof_collect(c);
+  block_free(b);
+  if (rc) return rc;
+  *energy = e;
+  if (iters) *iters = it;
+  return 0;
 }
 
 int sdft_hip_cheby_coeffs(sdft_ctx* h, int L, double emin, double emax, double mu, double beta,

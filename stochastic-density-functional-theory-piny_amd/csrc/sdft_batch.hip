@@ -481,7 +481,7 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     if (P > 0) {
       B.nl_frags.push_back(i);
       if ((rc = dmalloc(&f.zpart, sizeof(double) * B.max_zpart_slices * P * S))) return fail(rc);
-      const size_t yb = sizeof(double) * (size_t)f.c->ppad * S;
+      const size_t yb = sizeof(double) * (size_t)f.c->ppad * ((S + 31) / 32 * 32);
       if ((rc = dmalloc(&f.Y, yb))) return fail(rc);
       e = hipMemsetAsync(f.Y, 0, yb, B.stream);  // padding columns stay zero
       if (e != hipSuccess) return fail(hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__));

@@ -179,6 +179,64 @@ __global__ void k_enl(const double* __restrict__ Z, const double* __restrict__ E
   if (threadIdx.x == 0) out[0] = acc;
 }
 
+// One power-iteration update of genEnergyMax / genEnergyMin
+// (stodft/calc-spectral-range.c:180-221 / :380-420) on orbital 0 of X with
+// raw force F: h = -f/4 (G != 0), -f/2 (G = 0); E = 2 sum_{G!=0} (h.re v.re +
+// h.im v.im) + h.re(0) v.re(0); w = h (mode 0) or emax v - h (mode 1);
+// v = w / sqrt(2 sum_{G!=0} |w|^2 + w.re(0)^2).  out[0] = E.  One workgroup.
+__global__ void k_power_update(double* __restrict__ X, const double* __restrict__ F, int ncp,
+                               int ncoef, int mode, double emax, double* __restrict__ out) {
+  __shared__ double sh[16];
+  __shared__ double len_sh;
+  double e = 0.0, l2 = 0.0;
+  const int g0 = ncoef - 1;
+  for (int j = threadIdx.x; j < ncoef; j += blockDim.x) {
+    const double s = (j == g0) ? -0.5 : -0.25;
+    const double hr = s * F[j], hi = s * F[ncp + j];
+    const double vr = X[j], vi = X[ncp + j];
+    const double wr = mode ? emax * vr - hr : hr;
+    const double wi = mode ? emax * vi - hi : hi;
+    if (j == g0) {
+      e += hr * vr;
+      l2 += wr * wr;
+    } else {
+      e += 2.0 * (hr * vr + hi * vi);
+      l2 += 2.0 * (wr * wr + wi * wi);
+    }
+    X[j] = wr;
+    X[ncp + j] = wi;
+  }
+  e = block_sum(e, sh);
+  l2 = block_sum(l2, sh);
+  if (threadIdx.x == 0) {
+    out[0] = e;
+    len_sh = sqrt(l2);
+  }
+  __syncthreads();
+  const double inv = 1.0 / len_sh;
+  for (int j = threadIdx.x; j < ncoef; j += blockDim.x) {
+    X[j] *= inv;
+    X[ncp + j] *= inv;
+  }
+}
+
+// v /= sqrt(<v|v>) (Gamma norm) for orbital 0 of X
+__global__ void k_normalize0(double* __restrict__ X, int ncp, int ncoef) {
+  __shared__ double sh[16];
+  __shared__ double len_sh;
+  double l2 = 0.0;
+  for (int j = threadIdx.x; j < ncoef; j += blockDim.x)
+    l2 += (j == ncoef - 1) ? X[j] * X[j] : 2.0 * (X[j] * X[j] + X[ncp + j] * X[ncp + j]);
+  l2 = block_sum(l2, sh);
+  if (threadIdx.x == 0) len_sh = sqrt(l2);
+  __syncthreads();
+  const double inv = 1.0 / len_sh;
+  for (int j = threadIdx.x; j < ncoef; j += blockDim.x) {
+    X[j] *= inv;
+    X[ncp + j] *= inv;
+  }
+}
+
 // ==========================================================================
 
 static int scratch(Ctx& c, size_t need) {
@@ -267,6 +325,25 @@ int energy_ke_device(Ctx& c, const double* X, int nstate, double* out) {
   double s = 0.0;
   for (int i = 0; i < nstate; ++i) s += h[i];
   *out = s;
+  return 0;
+}
+
+int normalize0_device(Ctx& c, double* X) {
+  hipLaunchKernelGGL(k_normalize0, dim3(1), dim3(1024), 0, c.stream, X, c.g.ncp, c.g.ncoef);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int power_update_device(Ctx& c, double* X, const double* F, int mode, double emax,
+                        double* energy_host) {
+  int rc = scratch(c, sizeof(double));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_power_update, dim3(1), dim3(1024), 0, c.stream, X, F, c.g.ncp, c.g.ncoef,
+                     mode, emax, c.red);
+  SDFT_HIP_CHECK(hipGetLastError());
+  SDFT_HIP_CHECK(hipMemcpyAsync(energy_host, c.red, sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
   return 0;
 }
 

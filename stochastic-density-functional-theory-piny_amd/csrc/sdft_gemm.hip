@@ -51,17 +51,42 @@ struct Mfma<float> {
 constexpr int KS = 32;   // K per step: 4 lane groups x 8 consecutive k
 constexpr int KPL = 8;   // k per lane per step
 
-// 8 consecutive values src[0..7] converted to TI (vector loads)
-template <typename TI, typename TS>
-__device__ inline void load8(const TS* __restrict__ src, TI* dst) {
+// 8 consecutive values src[0..7] in 16-byte vector loads (src is 32-byte
+// aligned for float, 64-byte for double: row strides and run starts are
+// multiples of 8 elements of 256-byte aligned buffers)
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ inline void load_run(const T* __restrict__ src, T (&dst)[KPL]) {
+  typedef typename Vec16<T>::type V;
+  constexpr int NV = Vec16<T>::n;
+  const V* s = reinterpret_cast<const V*>(src);
 #pragma unroll
-  for (int t = 0; t < KPL; ++t) dst[t] = TI(src[t]);
+  for (int v = 0; v < KPL / NV; ++v) {
+    const V x = s[v];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) dst[v * NV + e] = x[e];
+  }
 }
 
 // ---- Z partials: zpart[z][p][s] = sum_{k in split z} A[p][k] X[s][k] -------
-// grid (ceil(P/32), ceil(ns/32), nsplit), 4 waves per workgroup share the
+// Tiles (ceil(P/32), ceil(ns/32), nsplit); 4 waves per workgroup share the
 // 32x32 output tile and interleave the split's K steps; the four partial
-// tiles are summed through LDS.
+// tiles are summed through LDS.  Both operands must have their row count
+// padded to a multiple of 32 with readable rows (zeros, or values whose
+// products land only in the rows / columns the epilogue does not store):
+// the K loop loads without bounds tests.
 template <typename TA, typename TX, typename TI>
 __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
                                                 const TX* __restrict__ X, int P, int ns,
@@ -69,7 +94,8 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
                                                 double* __restrict__ zpart, int bx, int by, int bz,
                                                 double (*red)[32 * 32]) {
   typedef typename Mfma<TI>::acc_t acc_t;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar K loop
   const int p0 = bx * 32, s0 = by * 32;
   const int g = lane >> 4, r16 = lane & 15;
   acc_t acc[2][2];
@@ -84,53 +110,58 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
   const int step1 = min(nsteps_total, step0 + ksteps_per_split);
   const TA* arow[2];
   const TX* xrow[2];
-  bool aok[2], xok[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const int p = p0 + 16 * i + r16;
-    aok[i] = p < P;
-    arow[i] = A + (size_t)(aok[i] ? p : 0) * K + KPL * g;
-    const int s = s0 + 16 * i + r16;
-    xok[i] = s < ns;
-    xrow[i] = X + (size_t)(xok[i] ? s : 0) * K + KPL * g;
+    arow[i] = A + (size_t)(p0 + 16 * i + r16) * K + KPL * g;
+    xrow[i] = X + (size_t)(s0 + 16 * i + r16) * K + KPL * g;
   }
-  // software pipeline: the operands of the next K step are loaded while the
-  // MFMAs of the current one run
-  TI a0[2][KPL], b0[2][KPL], a1[2][KPL], b1[2][KPL];
-  auto load_step = [&](int st, TI (&a)[2][KPL], TI (&b)[2][KPL]) {
+  // Software pipeline with D steps in flight: each step's operands are
+  // loaded in their storage types D steps ahead of the MFMAs that consume
+  // them and converted to TI only there, so a wave waits for a load only
+  // when its step comes up.  This wave's steps are step0 + wave + 4n.
+  constexpr int D = sizeof(TI) == 4 ? 4 : 3;
+  TA ra[D][2][KPL];
+  TX rb[D][2][KPL];
+  auto load_raw = [&](int st, TA (&a)[2][KPL], TX (&b)[2][KPL]) {
     const int k0 = st * KS;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      if (aok[i]) load8<TI>(arow[i] + k0, a[i]);
-      else
-#pragma unroll
-        for (int t = 0; t < KPL; ++t) a[i][t] = TI(0);
-      if (xok[i]) load8<TI>(xrow[i] + k0, b[i]);
-      else
-#pragma unroll
-        for (int t = 0; t < KPL; ++t) b[i][t] = TI(0);
+      load_run<TA>(arow[i] + k0, a[i]);
+      load_run<TX>(xrow[i] + k0, b[i]);
     }
   };
-  auto mma_step = [&](TI (&a)[2][KPL], TI (&b)[2][KPL]) {
+  auto mma_raw = [&](TA (&a)[2][KPL], TX (&b)[2][KPL]) {
 #pragma unroll
     for (int t = 0; t < KPL; ++t)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<TI>::run(a[i][t], b[j][t], acc[i][j]);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = Mfma<TI>::run(TI(a[i][t]), TI(b[j][t]), acc[i][j]);
   };
-  int st = step0 + wave;
-  if (st < step1) load_step(st, a0, b0);
-  while (st < step1) {
-    const int nx = st + 4;
-    if (nx < step1) load_step(nx, a1, b1);
-    mma_step(a0, b0);
-    st = nx;
-    if (st >= step1) break;
-    const int nx2 = st + 4;
-    if (nx2 < step1) load_step(nx2, a0, b0);
-    mma_step(a1, b1);
-    st = nx2;
+  // Every path issues the same loads (a step past this wave's range reloads
+  // the split's last step, unused), so the compiler's wait counters can let
+  // the D - 1 younger steps stay in flight while a step's MFMAs run.
+  const int first = step0 + wave;
+  const int nst = first < step1 ? (step1 - first + 3) / 4 : 0;
+  auto stidx = [&](int m) {
+    const int st = first + 4 * m;
+    return st < step1 ? st : step1 - 1;
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    load_raw(stidx(u), ra[u], rb[u]);
+    __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
+  }
+  for (int n0 = 0; n0 < nst; n0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      if (n0 + u < nst) mma_raw(ra[u], rb[u]);
+      load_raw(stidx(n0 + u + D), ra[u], rb[u]);
+      // keep each step's loads in its slot: moved later, they would become
+      // the youngest loads at the next step and force a full wait there
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
   // sum the four waves' tiles
 #pragma unroll
@@ -152,14 +183,19 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
   }
 }
 
+// 1-D launch over tp * ts * nsplit tiles.  Logical tile L (XCD-swizzled) is
+// split-major, then column tile, then row tile: the tiles of one K slice sit
+// on one XCD, whose L2 then holds that slice of both operands.
 template <typename TA, typename TX, typename TI>
 __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
                                                     const TX* __restrict__ X, int P, int ns,
                                                     int K, int ksteps_per_split,
-                                                    double* __restrict__ zpart) {
+                                                    double* __restrict__ zpart, int tp, int ts,
+                                                    int total) {
   __shared__ double red[4][32 * 32];
-  kb_project_tile<TA, TX, TI>(A, X, P, ns, K, ksteps_per_split, zpart, blockIdx.x, blockIdx.y,
-                              blockIdx.z, red);
+  const int L = xcd_swizzle(blockIdx.x, total);
+  const int bx = L % tp, r = L / tp;
+  kb_project_tile<TA, TX, TI>(A, X, P, ns, K, ksteps_per_split, zpart, bx, r % ts, r / ts, red);
 }
 
 // One launch over the fragments of a batch: workgroup w belongs to fragment
@@ -170,9 +206,10 @@ __global__ __launch_bounds__(256) void k_kb_project_batch(const KbProjectArgs* _
                                                           const int* __restrict__ prefix,
                                                           int nfrag) {
   __shared__ double red[4][32 * 32];
-  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const int L = xcd_swizzle(blockIdx.x, prefix[nfrag]);
+  const int f = find_frag(prefix, nfrag, L);
   const KbProjectArgs a = args[f];
-  const int local = blockIdx.x - prefix[f];
+  const int local = L - prefix[f];
   const int bx = local % a.tp, r = local / a.tp;
   kb_project_tile<TA, double, TI>(reinterpret_cast<const TA*>(a.A), a.X, a.P, a.ns, a.K, a.per,
                                   a.zpart, bx, r % a.ts, r / a.ts, red);
@@ -217,9 +254,10 @@ __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __res
                                                         const int* __restrict__ prefix,
                                                         int nfrag) {
   __shared__ double red[4][32 * 32];
-  const int f = find_frag(prefix, nfrag, blockIdx.x);
+  const int L = xcd_swizzle(blockIdx.x, prefix[nfrag]);
+  const int f = find_frag(prefix, nfrag, L);
   const KbForceArgs a = args[f];
-  const int local = blockIdx.x - prefix[f];
+  const int local = L - prefix[f];
   kb_project_tile<double, TA, TI>(a.Y, reinterpret_cast<const TA*>(a.At), a.ns, a.K2, a.ppad,
                                   a.ppad / KS, a.F, local % a.ts, local / a.ts, 0, red);
 }
@@ -262,7 +300,7 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
   if (rc) return rc;
   // Y is [ns][ppad] with zero padding columns: (re)allocated and cleared
   // whenever it grows or the padded projector count changes
-  const size_t ybytes = sizeof(double) * (size_t)c.ppad * ns;
+  const size_t ybytes = sizeof(double) * (size_t)c.ppad * ((ns + 31) / 32 * 32);
   if (c.Y_bytes < ybytes || c.Y_ppad != c.ppad) {
     if (c.Y) SDFT_HIP_CHECK(hipFree(c.Y));
     if (c.Z) SDFT_HIP_CHECK(hipFree(c.Z));
@@ -277,16 +315,16 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
     c.Y_bytes = ybytes;
     c.Y_ppad = c.ppad;
   }
-  dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
+  const int tp = (P + 31) / 32, ts = (ns + 31) / 32, tot = tp * ts * nsplit;
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, c.A64,
-                       X, P, ns, K, per, c.zpart);
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
+                       c.A64, X, P, ns, K, per, c.zpart, tp, ts, tot);
   else
-    hipLaunchKernelGGL((k_kb_project<float, double, float>), grid, dim3(256), 0, c.stream, c.A32,
-                       X, P, ns, K, per, c.zpart);
+    hipLaunchKernelGGL((k_kb_project<float, double, float>), dim3(tot), dim3(256), 0, c.stream,
+                       c.A32, X, P, ns, K, per, c.zpart, tp, ts, tot);
   SDFT_HIP_CHECK(hipGetLastError());
-  const int tot = P * ns;
-  hipLaunchKernelGGL(k_nl_reduce, dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
+  const int nel = P * ns;
+  hipLaunchKernelGGL(k_nl_reduce, dim3((nel + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
                      P, ns, c.ppad, c.E, c.Z, c.Y);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
@@ -343,9 +381,9 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
     const size_t tot = (size_t)P * c.g.ncp;
     hipLaunchKernelGGL(k_kb_deriv_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        c.stream, c.A64, gint + (size_t)d * c.g.ncp, P, c.g.ncp, Ad);
-    dim3 grid((P + 31) / 32, (ns + 31) / 32, nsplit);
-    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, Ad,
-                       X, P, ns, K, per, part);
+    const int tp = (P + 31) / 32, ts = (ns + 31) / 32, ntile = tp * ts * nsplit;
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(ntile), dim3(256), 0,
+                       c.stream, Ad, X, P, ns, K, per, part, tp, ts, ntile);
     // reduce the split partials into Zd[d] (the E factor is applied later)
     const int n2 = P * ns;
     hipLaunchKernelGGL(k_nl_reduce, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, part, nsplit,
@@ -361,14 +399,14 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
 // over the padded projector index, unsplit (it writes F directly)
 int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
   const int K2 = 2 * c.g.ncp;
-  dim3 grid((ns + 31) / 32, (K2 + 31) / 32, 1);
+  const int ts = (ns + 31) / 32, tk = (K2 + 31) / 32, tot = ts * tk;
   const int per = c.ppad / KS;
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_project<double, double, double>), grid, dim3(256), 0, c.stream, c.Y,
-                       c.At64, ns, K2, c.ppad, per, fnl);
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
+                       c.Y, c.At64, ns, K2, c.ppad, per, fnl, ts, tk, tot);
   else
-    hipLaunchKernelGGL((k_kb_project<double, float, float>), grid, dim3(256), 0, c.stream, c.Y,
-                       c.At32, ns, K2, c.ppad, per, fnl);
+    hipLaunchKernelGGL((k_kb_project<double, float, float>), dim3(tot), dim3(256), 0, c.stream,
+                       c.Y, c.At32, ns, K2, c.ppad, per, fnl, ts, tk, tot);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

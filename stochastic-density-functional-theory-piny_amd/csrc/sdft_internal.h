@@ -106,6 +106,11 @@ struct Ctx {
   int Y_ppad = 0;
   double* red = nullptr;     // reduction scratch
   size_t red_bytes = 0;
+  // Newton-form accumulators for the contract kernel (set for the duration
+  // of a Newton filter call only)
+  int newton_nmu = 0;
+  long long newton_stride = 0;
+  const double* newton_c = nullptr;
   // profiling
   bool profiling = false;
   std::vector<std::string> prof_names;
@@ -188,12 +193,24 @@ struct ContractArgs {
   // step counter of a replayed graph); otherwise sp.ck
   const double* coeffs_dev;
   const int* step_dev;
+  // Newton form: nmu accumulators acc + mu * acc_stride and the coefficient
+  // table newton_c[n * nmu + mu] (device)
+  int nmu;
+  long long acc_stride;
+  const double* newton_c;
   int gx, gy;  // column chunks, pairs
 };
 
 // kModeNoise writes the transformed noise to Xout and to acc (the caller
-// passes the block's chi buffer as acc).
-enum ContractMode { kModeForce = 0, kModeFilter = 1, kModeMoments = 2, kModeNoise = 3 };
+// passes the block's chi buffer as acc).  kModeNewton: P_{n+1} = (H^ - z_n) P_n
+// (sp.p = -scale emean - z_n) and acc_mu += a[n+1][mu] P_{n+1}.
+enum ContractMode {
+  kModeForce = 0,
+  kModeFilter = 1,
+  kModeMoments = 2,
+  kModeNoise = 3,
+  kModeNewton = 4
+};
 
 // KB GEMM arguments of one fragment of a batch (tile counts: tp over
 // projectors, ts over states, tk over the 2 ncp coefficient rows).
@@ -218,6 +235,19 @@ struct KbForceArgs {
   int ns, K2, ppad, ts, tk;
   double* F;
 };
+
+// Workgroups of a launch are dispatched to the eight XCDs round-robin (XCD =
+// index mod 8), each XCD with its own L2.  This maps the hardware index w of
+// a 1-D launch of `total` workgroups to a logical index so that every XCD
+// receives one contiguous range of logical indices (sizes differing by at
+// most one): workgroups that share operands, numbered next to each other,
+// then read them through the same L2.  Bijective on [0, total).
+__device__ inline int xcd_swizzle(int w, int total) {
+  constexpr int X = 8;
+  const int q = total / X, r = total - q * X;
+  const int x = w % X, j = w / X;
+  return x * q + (x < r ? x : r) + j;
+}
 
 // workgroup -> fragment of a batch launch: prefix[f] <= wg < prefix[f + 1]
 __device__ inline int find_frag(const int* __restrict__ prefix, int nfrag, int wg) {
@@ -283,6 +313,9 @@ int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta,
 int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3);
 int energy_ke_device(Ctx& c, const double* X, int nstate, double* out);
 int energy_nl_device(Ctx& c, int nstate_pad, int nstate, double* out);
+int normalize0_device(Ctx& c, double* X);
+int power_update_device(Ctx& c, double* X, const double* F, int mode, double emax,
+                        double* energy_host);
 
 }  // namespace sdft
 

@@ -553,6 +553,26 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
         Xout[base + ncp + j] = fi;
         continue;
       }
+      if (mode == kModeNewton) {
+        // normHNewtonHerm (normh.c:101-150): scale the force, add prefact c
+        const double nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
+        const double ni = g0 ? fi + sp.p * ci : sp.s1 * fi + sp.p * ci;
+        Xout[base + j] = nr;
+        Xout[base + ncp + j] = ni;
+        for (int mu = 0; mu < a.nmu; ++mu) {
+          double* am = acc + mu * a.acc_stride;
+          const double c1 = a.newton_c[(size_t)sp.step * a.nmu + mu];
+          if (sp.first) {
+            const double c0 = a.newton_c[mu];
+            am[base + j] = c0 * cr + c1 * nr;
+            am[base + ncp + j] = c0 * ci + c1 * ni;
+          } else {
+            am[base + j] += c1 * nr;
+            am[base + ncp + j] += c1 * ni;
+          }
+        }
+        continue;
+      }
       double nr, ni;
       if (sp.first) {
         nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
@@ -847,6 +867,9 @@ ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W,
   a.mom_part = mom_part;
   a.coeffs_dev = nullptr;
   a.step_dev = nullptr;
+  a.nmu = c.newton_nmu;
+  a.acc_stride = c.newton_stride;
+  a.newton_c = c.newton_c;
   a.gx = c.g.nchunk;
   a.gy = npairs;
   return a;

@@ -130,3 +130,47 @@ void calcRhoStoHybridCheby(SDFT_STODFT *st) {
                              &st->stoWfUpIm[0][1], st->occNumber / (double)st->numStateStoUp,
                              st->rhoUp, SDFT_RHO_FFTW, 0));
 }
+
+/* filter.c:33-312: one device call for the whole Newton recurrence; the
+ * nmu filtered blocks come back contiguous and are scattered to
+ * stoWfUpRe/Im[imu] */
+void filterNewtonPolyHerm(SDFT_STODFT *st) {
+  const int tot = st->numStateUpProc * st->numCoeff;
+  const int nmu = st->numChemPot > 0 ? st->numChemPot : 1;
+  double *ore = (double *)malloc(sizeof(double) * (size_t)tot * nmu);
+  double *oim = (double *)malloc(sizeof(double) * (size_t)tot * nmu);
+  int imu;
+  if (!ore || !oim) sdft_die("filterNewtonPolyHerm", SDFT_ENOMEM);
+  SDFT_CALL("filterNewtonPolyHerm",
+            sdft_hip_newton_filter(st->ctx, st->numStateUpProc, st->polynormLength, nmu,
+                                   st->expanCoeff, st->sampPoint, st->newtonScale,
+                                   st->energyMean, &st->cre_up[1], &st->cim_up[1], ore, oim,
+                                   st->prec));
+  for (imu = 0; imu < nmu; imu++) {
+    memcpy(&st->stoWfUpRe[imu][1], ore + (size_t)imu * tot, tot * sizeof(double));
+    memcpy(&st->stoWfUpIm[imu][1], oim + (size_t)imu * tot, tot * sizeof(double));
+  }
+  if (st->polynormLength > 1) {
+    memcpy(&st->fcre_up[1], &st->cre_up[1], tot * sizeof(double));
+    memcpy(&st->fcim_up[1], &st->cim_up[1], tot * sizeof(double));
+  }
+  free(ore);
+  free(oim);
+}
+
+static void spectral(SDFT_STODFT *st, int which, double *re, double *im, const char *name) {
+  double e = 0.0;
+  int it = 0;
+  SDFT_CALL(name, sdft_hip_spectral_bound(st->ctx, which, st->energyMax, 1.0e-5, 1000000, re, im,
+                                          &e, &it, st->prec));
+  if (which == 0)
+    st->energyMax = e + 0.1;
+  else
+    st->energyMin = e - 0.1;
+  st->energyMean = 0.5 * (st->energyMin + st->energyMax);
+  st->energyDiff = st->energyMax - st->energyMin;
+}
+
+void genEnergyMax(SDFT_STODFT *st) { spectral(st, 0, st->testWfMaxRe, st->testWfMaxIm, "genEnergyMax"); }
+
+void genEnergyMin(SDFT_STODFT *st) { spectral(st, 1, st->testWfMinRe, st->testWfMinIm, "genEnergyMin"); }

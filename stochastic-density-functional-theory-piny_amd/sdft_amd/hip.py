@@ -42,6 +42,7 @@ EXPORTS = [
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
+    "sdft_hip_spectral_bound", "sdft_hip_newton_filter",
 ]
 
 _LIB = None
@@ -95,6 +96,10 @@ def load(path: str | None = None):
         "sdft_hip_batch_config": ([vp, C.c_int, C.c_int], C.c_int),
         "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
                                           C.c_double], C.c_int),
+        "sdft_hip_spectral_bound": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
+                                     _ip, C.c_int], C.c_int),
+        "sdft_hip_newton_filter": ([vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                    C.c_double, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -197,6 +202,34 @@ class Context:
         _check(self.lib.sdft_hip_hpsi(self.h, re.shape[0], _ptr(re), _ptr(im), _ptr(fre),
                                       _ptr(fim), prec))
         return fre + 1j * fim
+
+    def newton_filter(self, chi, coeffs, samp_point, scale, emean, prec=PREC_FP64):
+        """filterNewtonPolyHerm: coeffs (L, nmu), samp_point (L-1,).  Returns
+        (out (nmu, nstate, ncoef), P_{L-1})."""
+        re, im = _split(chi)
+        co = np.ascontiguousarray(coeffs, dtype=np.float64)
+        if co.ndim == 1:
+            co = co[:, None]
+        L, nmu = co.shape
+        z = np.ascontiguousarray(samp_point, dtype=np.float64)
+        assert z.shape[0] >= L - 1
+        ns, nc = re.shape
+        ore = np.empty((nmu, ns, nc))
+        oim = np.empty((nmu, ns, nc))
+        _check(self.lib.sdft_hip_newton_filter(self.h, ns, L, nmu, _ptr(co), _ptr(z), scale, emean,
+                                               _ptr(re), _ptr(im), _ptr(ore), _ptr(oim), prec))
+        return ore + 1j * oim, re + 1j * im
+
+    def spectral_bound(self, v0, which, emax=0.0, conv=1e-5, max_iter=100000,
+                       prec=PREC_FP64):
+        """genEnergyMax (which 0) / genEnergyMin (which 1) on the device from
+        the start vector v0 (ncoef complex): returns (E, final vector, iterations)."""
+        re, im = _split(np.asarray(v0).reshape(1, -1))
+        e = C.c_double()
+        it = C.c_int()
+        _check(self.lib.sdft_hip_spectral_bound(self.h, which, emax, conv, max_iter, _ptr(re),
+                                                _ptr(im), C.byref(e), C.byref(it), prec))
+        return e.value, (re + 1j * im)[0], it.value
 
     def cheby_coeffs(self, L, emin, emax, mu, beta, fun_flag=2):
         c = np.empty(L)

@@ -46,32 +46,23 @@ def _gdot(a, b):
     return 2.0 * (np.sum(a.real * b.real) + np.sum(a.imag * b.imag)) - a[..., -1].real @ b[..., -1].real
 
 
-def gen_energy_max(ctx, niter=60, seed=7, pad=0.1, prec=_hip.PREC_FP64):
-    """genEnergyMax (calc-spectral-range.c:33-333): power iteration on H for
-    the largest eigenvalue, padded by +0.1 Ha."""
+def gen_energy_max(ctx, niter=100000, seed=7, pad=0.1, conv=1e-5, prec=_hip.PREC_FP64):
+    """genEnergyMax (calc-spectral-range.c:33-333) on the device
+    (sdft_hip_spectral_bound): power iteration on H until the Rayleigh
+    quotient changes by <= conv, padded by +0.1 Ha."""
     from . import systems
-    v = systems.noise_orbitals(ctx.sys, 1, seed)
-    v /= np.sqrt(_gdot(v, v))
-    e = 0.0
-    for _ in range(niter):
-        w = _hpsi(ctx, v, prec)
-        e = _gdot(v, w)
-        v = w / np.sqrt(_gdot(w, w))
+    v = systems.noise_orbitals(ctx.sys, 1, seed)[0]
+    e, _, _ = ctx.spectral_bound(v, 0, conv=conv, max_iter=niter, prec=prec)
     return e + pad
 
 
-def gen_energy_min(ctx, emax, niter=200, seed=8, pad=0.1, prec=_hip.PREC_FP64):
-    """genEnergyMin (calc-spectral-range.c:339-566): power iteration on
-    (E_max - H), E_min = E_max - lambda, padded by -0.1 Ha."""
+def gen_energy_min(ctx, emax, niter=100000, seed=8, pad=0.1, conv=1e-5, prec=_hip.PREC_FP64):
+    """genEnergyMin (calc-spectral-range.c:339-566) on the device: power
+    iteration on (E_max - H); the converged <phi|H|phi> padded by -0.1 Ha."""
     from . import systems
-    v = systems.noise_orbitals(ctx.sys, 1, seed)
-    v /= np.sqrt(_gdot(v, v))
-    lam = 0.0
-    for _ in range(niter):
-        w = emax * v - _hpsi(ctx, v, prec)
-        lam = _gdot(v, w)
-        v = w / np.sqrt(_gdot(w, w))
-    return emax - lam - pad
+    v = systems.noise_orbitals(ctx.sys, 1, seed)[0]
+    e, _, _ = ctx.spectral_bound(v, 1, emax=emax, conv=conv, max_iter=niter, prec=prec)
+    return e - pad
 
 
 @dataclasses.dataclass

@@ -261,3 +261,44 @@ def test_density_generic_layout_and_accumulate(hip, si8_ctx, si8_system):
     acc = rho_f.copy()
     si8_ctx.density(C, 0.4, layout=hip.RHO_FFTW, accumulate_into=acc)
     assert rel(acc, 2.0 * rho_f) < 1e-15
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_spectral_bound_matches_oracle(hip, si8_ctx, si8_system, which):
+    """sdft_hip_spectral_bound (genEnergyMax / genEnergyMin on the device)
+    against the oracle's restatement, a fixed 25 iterations (conv = 0) from
+    the same start vector: energy 1e-11, final vector 1e-10 relative; and a
+    converged run (conv 1e-5) stops where the oracle stops."""
+    from sdft_amd import systems
+    say("case: spectral bound which=%d" % which)
+    v0 = systems.noise_orbitals(si8_system, 1, seed=77)[0]
+    emax = 6.0
+    e_g, v_g, it_g = si8_ctx.spectral_bound(v0, which, emax=emax, conv=0.0, max_iter=25)
+    e_r, v_r, it_r = O.gen_energy_bound(v0, si8_system, which, emax=emax, conv=0.0, max_iter=25)
+    assert it_g == it_r == 25
+    assert abs(e_g - e_r) < 1e-11 * abs(e_r)
+    assert rel(v_g, v_r) < 1e-10
+    if which == 0:
+        e_g, _, it_g = si8_ctx.spectral_bound(v0, 0, conv=1e-5, max_iter=5000)
+        e_r, _, it_r = O.gen_energy_bound(v0, si8_system, 0, conv=1e-5, max_iter=5000)
+        assert it_g == it_r and abs(e_g - e_r) < 1e-9 * abs(e_r)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-4)])
+def test_newton_filter_matches_oracle(hip, si8_ctx, si8_system, prec, tol):
+    """sdft_hip_newton_filter (filterNewtonPolyHerm) against the oracle:
+    three chemical-potential columns, Leja-ordered Chebyshev nodes, L = 20."""
+    from sdft_amd import systems
+    say("case: newton filter prec=%d" % prec)
+    emin, emax = bounds(si8_system)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    L = 20
+    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
+    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 4.0)), z)
+                   for m in (0.1, 0.3, 0.5)], axis=1)
+    chi = systems.noise_orbitals(si8_system, 5, seed=12)
+    out_g, pl_g = si8_ctx.newton_filter(chi, co, z, scale, emean, prec=prec)
+    out_r, pl_r = O.newton_filter(chi, co, z, si8_system, scale, emean)
+    for m in range(3):
+        assert rel(out_g[m], out_r[m]) < tol, (m, rel(out_g[m], out_r[m]))
+    assert rel(pl_g, pl_r) < tol

@@ -25,7 +25,10 @@ class SdftStodft(C.Structure):
                 ("cre_up", _dp), ("cim_up", _dp), ("fcre_up", _dp), ("fcim_up", _dp),
                 ("wfUpRe1", _dp), ("wfUpIm1", _dp),
                 ("stoWfUpRe", C.POINTER(_dp)), ("stoWfUpIm", C.POINTER(_dp)),
-                ("expanCoeff", _dp), ("chebyMomentsUp", _dp), ("chemPot", _dp), ("rhoUp", _dp)]
+                ("expanCoeff", _dp), ("chebyMomentsUp", _dp), ("chemPot", _dp), ("rhoUp", _dp),
+                ("numChemPot", C.c_int), ("newtonScale", C.c_double), ("sampPoint", _dp),
+                ("testWfMaxRe", _dp), ("testWfMaxIm", _dp), ("testWfMinRe", _dp),
+                ("testWfMinIm", _dp)]
 
 
 def one_based(a):
@@ -39,7 +42,7 @@ def shim():
     from sdft_amd import build
     lib = C.CDLL(build.host_lib_path())
     for f in ("filterChebyPolyHerm", "calcCoefForceWrapSCF", "calcChebyMoments", "calcChemPotCheby",
-              "calcRhoStoHybridCheby"):
+              "calcRhoStoHybridCheby", "filterNewtonPolyHerm", "genEnergyMax", "genEnergyMin"):
         getattr(lib, f).argtypes = [C.POINTER(SdftStodft)]
         getattr(lib, f).restype = None
     lib.normHCheby.argtypes = [C.POINTER(SdftStodft), C.c_int, C.c_int]
@@ -106,3 +109,56 @@ def test_shim_filter_normh_mu_density(shim, si8_system):
     rho_r = O.density_real(acc_r, s, 2.0 / B)
     assert np.max(np.abs(rho - rho_r)) < 1e-12 * np.max(np.abs(rho_r))
     ctx.close()
+
+
+def test_shim_newton_filter_and_spectral_range(shim, si8_system):
+    """filterNewtonPolyHerm / genEnergyMax / genEnergyMin through the shim's
+    SDFT_STODFT (1-based arrays) against the oracle."""
+    from oracle import sdft_oracle as O
+    from sdft_amd import hip, stodft, systems
+    print("case: host shim newton + spectral range", flush=True)
+    s = si8_system
+    ctx = hip.Context(s)
+    B, L, nc, nmu = 3, 12, s.ncoef, 2
+    chi = systems.noise_orbitals(s, B, 8)
+    emin, emax = stodft.spectral_bounds(s)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
+    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 4.0)), z)
+                   for m in (0.2, 0.4)], axis=1)
+    cre = np.ascontiguousarray(chi.real.ravel())
+    cim = np.ascontiguousarray(chi.imag.ravel())
+    fre, fim = np.zeros(B * nc), np.zeros(B * nc)
+    outs = [np.zeros(B * nc) for _ in range(2 * nmu)]
+    pre = (_dp * nmu)(*[one_based(outs[m]) for m in range(nmu)])
+    pim = (_dp * nmu)(*[one_based(outs[nmu + m]) for m in range(nmu)])
+    coef = np.ascontiguousarray(co.ravel())
+    zz = np.ascontiguousarray(z[:L - 1])
+    st = SdftStodft()
+    st.ctx = ctx.h.value
+    st.prec = 0
+    st.numStateUpProc, st.numCoeff, st.polynormLength = B, nc, L
+    st.energyMean = emean
+    st.cre_up, st.cim_up = one_based(cre), one_based(cim)
+    st.fcre_up, st.fcim_up = one_based(fre), one_based(fim)
+    st.stoWfUpRe, st.stoWfUpIm = pre, pim
+    st.expanCoeff = coef.ctypes.data_as(_dp)
+    st.numChemPot, st.newtonScale = nmu, scale
+    st.sampPoint = zz.ctypes.data_as(_dp)
+    shim.filterNewtonPolyHerm(C.byref(st))
+    out_r, pl_r = O.newton_filter(chi, co, z, s, scale, emean)
+    for m in range(nmu):
+        got = outs[m].reshape(B, nc) + 1j * outs[nmu + m].reshape(B, nc)
+        assert np.max(np.abs(got - out_r[m])) < 1e-10 * np.max(np.abs(out_r[m]))
+    got_pl = cre.reshape(B, nc) + 1j * cim.reshape(B, nc)
+    assert np.max(np.abs(got_pl - pl_r)) < 1e-10 * np.max(np.abs(pl_r))
+    # spectral range from a given start vector
+    v0 = systems.noise_orbitals(s, 1, seed=19)[0]
+    vre = np.ascontiguousarray(v0.real.copy())
+    vim = np.ascontiguousarray(v0.imag.copy())
+    st.testWfMaxRe, st.testWfMaxIm = vre.ctypes.data_as(_dp), vim.ctypes.data_as(_dp)
+    st.energyMin = emin
+    shim.genEnergyMax(C.byref(st))
+    e_r, v_r, _ = O.gen_energy_bound(v0, s, 0, conv=1e-5)
+    assert abs(st.energyMax - (e_r + 0.1)) < 1e-9 * abs(e_r)
+    assert np.max(np.abs(vre + 1j * vim - v_r)) < 1e-8

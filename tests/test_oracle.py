@@ -418,3 +418,60 @@ def test_real_space_kb_is_hermitian(si8_system):
     lhs = gdot(a[0], hb[0])
     rhs = gdot(ha[0], b[0])
     assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1.0)
+
+
+def test_spectral_bounds_match_dense_eigenvalues(tiny_system):
+    """gen_energy_bound (genEnergyMax / genEnergyMin restated) converges to
+    the extreme eigenvalues of the dense H built from the oracle."""
+    from sdft_amd import systems
+    H = O.dense_hamiltonian(tiny_system)
+    ev = np.linalg.eigvalsh(H)
+    v0 = systems.noise_orbitals(tiny_system, 1, seed=3)[0]
+    emax, _, it = O.gen_energy_bound(v0, tiny_system, 0, conv=1e-12, max_iter=20000)
+    assert abs(emax - ev[-1]) < 1e-6 * abs(ev[-1]), (emax, ev[-1], it)
+    emin, _, it2 = O.gen_energy_bound(v0, tiny_system, 1, emax=emax + 0.1, conv=1e-13,
+                                      max_iter=200000)
+    assert abs(emin - ev[0]) < 1e-5 * max(1.0, abs(ev[0])), (emin, ev[0], it2)
+
+
+def test_newton_filter_on_exact_eigenvector(tiny_system):
+    """Plane wave with constant v, no KB: the Newton filter returns
+    p(y) c with p(y) = sum_n a_n prod_{j<n} (y - z_j), y = scale (lam - emean),
+    for every chemical-potential column."""
+    import copy
+    s = copy.copy(tiny_system)
+    s.vks = np.full(s.ngrid, -0.2)
+    s.proj = np.zeros((0, s.ncoef), dtype=complex)
+    s.proj_energy = np.zeros(0)
+    emin, emax = -0.5, 2.0
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    L = 24
+    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
+    co = np.stack([O.newton_coeffs(lambda x, m=m: 1.0 / (1.0 + np.exp(3.0 * (x - m))), z)
+                   for m in (-0.3, 0.0, 0.4)], axis=1)
+    i = 5
+    c = np.zeros((1, s.ncoef), dtype=complex)
+    c[0, i] = 0.7 - 0.2j
+    out, _ = O.newton_filter(c, co, z, s, scale, emean, with_nl=False)
+    y = scale * (0.5 * s.ak2[i] - 0.2 - emean)
+    for m in range(co.shape[1]):
+        p = co[0, m] + sum(co[n, m] * np.prod(y - z[:n]) for n in range(1, L))
+        assert np.allclose(out[m], p * c, atol=1e-12)
+
+
+def test_newton_filter_matches_chebyshev_filter(tiny_system):
+    """Newton (Leja-ordered Chebyshev nodes) and Chebyshev expansions of the
+    same smooth sqrt(f) agree to their interpolation errors on a real H."""
+    from sdft_amd import systems, stodft
+    s = tiny_system
+    emin, emax = stodft.spectral_bounds(s)
+    scale, emean = 2 / (emax - emin), 0.5 * (emin + emax)
+    mu, beta = 0.5, 3.0
+    L = 48
+    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
+    g = lambda y: np.sqrt(O.fermi(y / scale + emean, mu, beta))  # noqa: E731
+    a = O.newton_coeffs(g, z)
+    chi = systems.noise_orbitals(s, 2, seed=4)
+    out, _ = O.newton_filter(chi, a, z, s, scale, emean)
+    acc, _ = O.filter_cheby(chi, O.cheby_coeffs(L, emin, emax, mu, beta, 2), s, scale, emean)
+    assert np.max(np.abs(out[0] - acc)) < 1e-7 * np.max(np.abs(acc))
