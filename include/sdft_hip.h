@@ -228,17 +228,31 @@ int sdft_hip_batch_destroy(sdft_batch *bat);
 int sdft_hip_batch_filter(sdft_batch *bat, int L, const double *coeffs, const double *scale,
                           const double *emean, int use_graph);
 /* Autotuner: times `steps` Chebyshev steps (after one warm-up step) of each
- * candidate launch configuration -- the split of the KB projection's
- * reduction (1, 2, 4 or 8 slices) and whether the KB products overlap the FFT
- * kernels on a second stream -- on the batch's current data and keeps the
- * fastest.  Writes the chosen split and overlap flag and the per-candidate
- * step times (ms; ncand entries, at most max_cand) when the pointers are
- * non-null.  Returns the number of candidates timed.  Leaves the blocks'
- * T_k buffers modified (the accumulators are untouched). */
-int sdft_hip_batch_autotune(sdft_batch *bat, int steps, int *split_out, int *overlap_out,
-                            int max_cand, double *cand_ms);
-/* Set the launch configuration by hand (split in {1,2,4,8}, overlap 0/1). */
-int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap);
+ * candidate launch configuration -- the plane kernel's b-line tile (0: base
+ * sub-chunk, 1: twice as wide, where it fits LDS), whether the KB products
+ * overlap the FFT kernels on a second stream, and the split of the KB
+ * projection's reduction (1, 2, 4 or 8 slices) -- on the batch's current data
+ * and keeps the fastest.  config_out (nullable, 3 ints) receives split,
+ * overlap, plane tile; cand_ms (nullable) the per-candidate step times in ms
+ * (at most max_cand).  Returns the number of candidates timed (16).  Leaves
+ * the blocks' T_k buffers modified (the accumulators are untouched). */
+int sdft_hip_batch_autotune(sdft_batch *bat, int steps, int *config_out, int max_cand,
+                            double *cand_ms);
+/* Set the launch configuration by hand (split in {1,2,4,8}, overlap 0/1,
+ * plane tile 0/1). */
+int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap, int plane_tile);
+
+/* Launch-configuration tuner for one context (no reference counterpart):
+ * times `steps` H applications of an nstate-orbital block (device noise) in
+ * precision prec for each candidate -- plane-kernel b-line tile (0 base
+ * sub-chunk, 1 twice as wide, where it fits LDS) x KB projection workgroup
+ * target (256, 512, 1024, 2048) -- and keeps the fastest for this context.
+ * config_out (nullable, 2 ints): tile, workgroup target; cand_ms (nullable):
+ * per-candidate ms per H application of the block.  Returns the number of
+ * candidates (8).  sdft_hip_set_config sets the two knobs by hand. */
+int sdft_hip_tune(sdft_ctx *ctx, int nstate, int prec, int steps, int *config_out, int max_cand,
+                  double *cand_ms);
+int sdft_hip_set_config(sdft_ctx *ctx, int plane_tile, int kb_wg_target);
 
 /* Per-kernel timing of the last block_filter call when profiling is on:
  * names[i] / ms[i] / launches[i] for up to max entries; returns the count. */

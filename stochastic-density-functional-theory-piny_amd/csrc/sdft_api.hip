@@ -901,6 +901,90 @@ int sdft_hip_hpsi(sdft_ctx* h, int nstate, const double* cre, const double* cim,
   return rc;
 }
 
+int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out, int max_cand,
+                  double* cand_ms) {
+  if (!h || nstate < 1 || steps < 1 || (prec != 0 && prec != 1)) {
+    set_error("bad tune arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_vks(c))) return rc;
+  Block b;
+  if ((rc = block_alloc(c, nstate, prec, b))) {
+    block_free(b);
+    return rc;
+  }
+  rc = block_noise_impl(b, 1, 0);
+  const bool prof = c.profiling;
+  c.profiling = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (!rc && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)) {
+    set_error("hipEventCreate failed");
+    rc = SDFT_EHIP;
+  }
+  const int targets[4] = {256, 512, 1024, 2048};
+  const int pv0 = c.plane_variant, kt0 = c.kb_wg_target;
+  int best_pv = pv0, best_kt = kt0, ncand = 0;
+  double best = 1e300;
+  StepParams sp{};
+  for (int pv = 0; pv < 2 && !rc; ++pv)
+    for (int ti = 0; ti < 4 && !rc; ++ti) {
+      c.plane_variant = pv;
+      c.kb_wg_target = targets[ti];
+      if ((rc = apply_step(b, kModeForce, b.X0, b.X1, sp, nullptr))) break;  // warm-up
+      if (hipEventRecord(t0, c.stream) != hipSuccess) {
+        rc = SDFT_EHIP;
+        break;
+      }
+      for (int k = 0; k < steps && !rc; ++k)
+        rc = apply_step(b, kModeForce, b.X0, b.X1, sp, nullptr);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(t1, c.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+          hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+        set_error("tune timing failed");
+        rc = SDFT_EHIP;
+        break;
+      }
+      const double per = ms / steps;
+      if (cand_ms && ncand < max_cand) cand_ms[ncand] = per;
+      ++ncand;
+      if (per < best) {
+        best = per;
+        best_pv = pv;
+        best_kt = targets[ti];
+      }
+    }
+  c.profiling = prof;
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  (void)hipStreamSynchronize(c.stream);
+  block_free(b);
+  if (rc) {
+    c.plane_variant = pv0;
+    c.kb_wg_target = kt0;
+    return rc;
+  }
+  c.plane_variant = best_pv;
+  c.kb_wg_target = best_kt;
+  if (config_out) {
+    config_out[0] = best_pv;
+    config_out[1] = best_kt;
+  }
+  return ncand;
+}
+
+int sdft_hip_set_config(sdft_ctx* h, int plane_tile, int kb_wg_target) {
+  if (!h || (plane_tile != 0 && plane_tile != 1) || kb_wg_target < 1) {
+    set_error("bad configuration");
+    return SDFT_EINVAL;
+  }
+  h->c.plane_variant = plane_tile;
+  h->c.kb_wg_target = kb_wg_target;
+  return 0;
+}
+
 int sdft_hip_newton_filter(sdft_ctx* h, int nstate, int L, int nmu, const double* coeffs,
                            const double* samp_point, double scale, double emean, double* cre,
                            double* cim, double* out_re, double* out_im, int prec) {

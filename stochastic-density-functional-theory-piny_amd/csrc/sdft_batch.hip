@@ -66,7 +66,8 @@ struct Frag {
 struct Group {
   int n = 0;
   std::vector<int> frags;
-  size_t lds = 0;
+  size_t lds = 0;    // plane kernel dynamic LDS, tile variant 0
+  size_t lds1 = 0;   // tile variant 1 (0 when it does not fit)
   int tot_exp = 0, tot_plane = 0;
   int* pre_exp = nullptr;    // expand and contract share the (chunk, pair) tiling
   int* pre_plane = nullptr;
@@ -109,6 +110,7 @@ struct Batch {
   std::vector<int> nl_frags;
   int split = 1;       // target slices of the projection's reduction
   bool overlap = true;
+  int plane_variant = 0;  // plane-kernel tile variant (falls back to 0 per group)
   int tot_proj = 0, tot_red = 0, tot_force = 0;
   int* pre_proj = nullptr;
   int* pre_red = nullptr;
@@ -186,7 +188,7 @@ int build_kb_tables(Batch& B) {
     const int P = f.c->nproj, S = f.b->nstate_pad, K2 = 2 * f.c->g.ncp;
     pp.push_back(pp.back() + ((P + 31) / 32) * ((S + 31) / 32) * ns);
     pr.push_back(pr.back() + (P * S + 255) / 256);
-    pf.push_back(pf.back() + ((S + 31) / 32) * ((K2 + 31) / 32));
+    pf.push_back(pf.back() + ((S + 31) / 32) * (K2 / 128));
   }
   B.tot_proj = pp.back();
   B.tot_red = pr.back();
@@ -295,7 +297,7 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
     q.K2 = K2;
     q.ppad = c.ppad;
     q.ts = (S + 31) / 32;
-    q.tk = (K2 + 31) / 32;
+    q.tk = K2 / 128;
     q.F = b.fnl;
     fo.push_back(q);
   }
@@ -329,16 +331,18 @@ int enqueue_step(Batch& B, int variant, int slot) {
   }
   for (Group& g : B.groups)
     if ((rc = launch_batch(0, g.n, B.prec, g.exp[xv], g.pre_exp, (int)g.frags.size(), g.tot_exp,
-                           0, s)))
+                           0, 0, s)))
       return rc;
-  for (Group& g : B.groups)
+  for (Group& g : B.groups) {
+    const int v = (B.plane_variant == 1 && g.lds1 > 0) ? 1 : 0;
     if ((rc = launch_batch(1, g.n, B.prec, g.plane, g.pre_plane, (int)g.frags.size(),
-                           g.tot_plane, g.lds, s)))
+                           g.tot_plane, v ? g.lds1 : g.lds, v, s)))
       return rc;
+  }
   if (nl && B.overlap) SDFT_HIP_CHECK(hipStreamWaitEvent(s, B.ev_join[slot], 0));
   for (Group& g : B.groups)
     if ((rc = launch_batch(2, g.n, B.prec, g.con[variant], g.pre_exp, (int)g.frags.size(),
-                           g.tot_exp, 0, s)))
+                           g.tot_exp, 0, 0, s)))
       return rc;
   hipLaunchKernelGGL(k_counter_advance, dim3(1), dim3(64), 0, s, B.counter);
   SDFT_HIP_CHECK(hipGetLastError());
@@ -507,18 +511,20 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
   if ((rc = upload_vec(B.copy, ca, B.stream))) return fail(rc);
   for (Group& g : B.groups) {
     std::vector<int> pe{0}, pp{0};
-    size_t lds = 0;
+    size_t lds = 0, lds1 = 0;
     for (int fi : g.frags) {
       const Frag& f = B.frags[fi];
       pe.push_back(pe.back() + f.c->g.nchunk * f.npairs);
       pp.push_back(pp.back() + f.c->g.n * f.npairs);
-      lds = std::max(lds, plane_lds_for(g.n, prec, f.c->g.M));
+      lds = std::max(lds, plane_lds_for(g.n, prec, f.c->g.M, 0));
+      lds1 = std::max(lds1, plane_lds_for(g.n, prec, f.c->g.M, 1));
     }
     if (lds > 160 * 1024) {
       set_error("plane buffer exceeds LDS for a fragment grid in this precision");
       return fail(SDFT_EINVAL);
     }
     g.lds = lds;
+    g.lds1 = lds1 <= 160 * 1024 ? lds1 : 0;
     g.tot_exp = pe.back();
     g.tot_plane = pp.back();
     const size_t nf = g.frags.size();
@@ -554,15 +560,16 @@ int sdft_hip_batch_destroy(sdft_batch* h) {
   return 0;
 }
 
-int sdft_hip_batch_config(sdft_batch* h, int split, int overlap) {
+int sdft_hip_batch_config(sdft_batch* h, int split, int overlap, int plane_tile) {
   if (!h || (split != 1 && split != 2 && split != 4 && split != 8) ||
-      (overlap != 0 && overlap != 1)) {
+      (overlap != 0 && overlap != 1) || (plane_tile != 0 && plane_tile != 1)) {
     set_error("bad batch configuration");
     return SDFT_EINVAL;
   }
   Batch& B = h->B;
   B.split = split;
   B.overlap = overlap != 0;
+  B.plane_variant = plane_tile;
   B.graph_valid = false;
   return build_kb_tables(B);
 }
@@ -583,8 +590,8 @@ int sdft_hip_batch_filter(sdft_batch* h, int L, const double* coeffs, const doub
   return run_filter(B, L, use_graph != 0);
 }
 
-int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* split_out, int* overlap_out,
-                            int max_cand, double* cand_ms) {
+int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_cand,
+                            double* cand_ms) {
   if (!h || steps < 1) {
     set_error("bad autotune arguments");
     return SDFT_EINVAL;
@@ -601,14 +608,16 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* split_out, int* overl
   SDFT_HIP_CHECK(hipEventCreate(&t1));
   const int splits[4] = {1, 2, 4, 8};
   double best = 1e300;
-  int best_split = B.split, best_ov = B.overlap ? 1 : 0;
+  int best_split = B.split, best_ov = B.overlap ? 1 : 0, best_pv = B.plane_variant;
   int ncand = 0;
   int rc = 0;
   if ((rc = copy_chi(B))) return rc;
+  for (int pv = 0; pv < 2 && !rc; ++pv)
   for (int ov = 0; ov < 2 && !rc; ++ov) {
     for (int si = 0; si < 4 && !rc; ++si) {
       B.split = splits[si];
       B.overlap = ov != 0;
+      B.plane_variant = pv;
       if ((rc = build_kb_tables(B))) break;
       if ((rc = build_args(B, L, co.data(), sc.data(), em.data()))) break;
       if ((rc = set_counter(B, 2))) break;
@@ -632,6 +641,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* split_out, int* overl
         best = per;
         best_split = splits[si];
         best_ov = ov;
+        best_pv = pv;
       }
     }
   }
@@ -639,12 +649,16 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* split_out, int* overl
   (void)hipEventDestroy(t1);
   B.split = best_split;
   B.overlap = best_ov != 0;
+  B.plane_variant = best_pv;
   B.graph_valid = false;
   int rc2 = build_kb_tables(B);
   if (rc) return rc;
   if (rc2) return rc2;
-  if (split_out) *split_out = best_split;
-  if (overlap_out) *overlap_out = best_ov;
+  if (config_out) {
+    config_out[0] = best_split;
+    config_out[1] = best_ov;
+    config_out[2] = best_pv;
+  }
   return ncand;
 }
 

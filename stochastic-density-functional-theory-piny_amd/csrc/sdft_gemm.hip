@@ -115,13 +115,13 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
     arow[i] = A + (size_t)(p0 + 16 * i + r16) * K + KPL * g;
     xrow[i] = X + (size_t)(s0 + 16 * i + r16) * K + KPL * g;
   }
-  // Software pipeline with D steps in flight: each step's operands are
-  // loaded in their storage types D steps ahead of the MFMAs that consume
-  // them and converted to TI only there, so a wave waits for a load only
-  // when its step comes up.  This wave's steps are step0 + wave + 4n.
-  constexpr int D = sizeof(TI) == 4 ? 4 : 3;
-  TA ra[D][2][KPL];
-  TX rb[D][2][KPL];
+  // Software pipeline: the operands of the next K step are loaded (in their
+  // storage types, converted to TI only at the MFMAs) while the MFMAs of the
+  // current one run.  Deeper rings (3-4 steps in flight) measured slower:
+  // each wave has only a few K steps, so the redundant tail loads they need
+  // for a branch-free loop cost more than the latency they hide.
+  TA ra0[2][KPL], ra1[2][KPL];
+  TX rb0[2][KPL], rb1[2][KPL];
   auto load_raw = [&](int st, TA (&a)[2][KPL], TX (&b)[2][KPL]) {
     const int k0 = st * KS;
 #pragma unroll
@@ -139,29 +139,18 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
         for (int j = 0; j < 2; ++j)
           acc[i][j] = Mfma<TI>::run(TI(a[i][t]), TI(b[j][t]), acc[i][j]);
   };
-  // Every path issues the same loads (a step past this wave's range reloads
-  // the split's last step, unused), so the compiler's wait counters can let
-  // the D - 1 younger steps stay in flight while a step's MFMAs run.
-  const int first = step0 + wave;
-  const int nst = first < step1 ? (step1 - first + 3) / 4 : 0;
-  auto stidx = [&](int m) {
-    const int st = first + 4 * m;
-    return st < step1 ? st : step1 - 1;
-  };
-#pragma unroll
-  for (int u = 0; u < D; ++u) {
-    load_raw(stidx(u), ra[u], rb[u]);
-    __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
-  }
-  for (int n0 = 0; n0 < nst; n0 += D) {
-#pragma unroll
-    for (int u = 0; u < D; ++u) {
-      if (n0 + u < nst) mma_raw(ra[u], rb[u]);
-      load_raw(stidx(n0 + u + D), ra[u], rb[u]);
-      // keep each step's loads in its slot: moved later, they would become
-      // the youngest loads at the next step and force a full wait there
-      __builtin_amdgcn_sched_barrier(0);
-    }
+  int st = step0 + wave;
+  if (st < step1) load_raw(st, ra0, rb0);
+  while (st < step1) {
+    const int nx = st + 4;
+    if (nx < step1) load_raw(nx, ra1, rb1);
+    mma_raw(ra0, rb0);
+    st = nx;
+    if (st >= step1) break;
+    const int nx2 = st + 4;
+    if (nx2 < step1) load_raw(nx2, ra0, rb0);
+    mma_raw(ra1, rb1);
+    st = nx2;
   }
   // sum the four waves' tiles
 #pragma unroll
@@ -181,6 +170,89 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
     const int p = p0 + row, s = s0 + col;
     if (p < P && s < ns) zp[(size_t)p * ns + s] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
   }
+}
+
+// ---- F[s][k] = sum_p Y[s][p] At[k][p] ----------------------------------------
+// The reduction (over the padded projector index, ppad) is short, so it is
+// not split across waves: a workgroup covers 32 rows s x 128 columns k, wave w
+// owns columns 32w .. 32w + 31 of that block and runs the whole reduction
+// (two-stage pipeline as above), and the accumulators are stored directly --
+// no LDS, no barrier.  Both operands have their rows padded to multiples of 32
+// (Y) and 128 (At: 2 ncp) with readable memory.
+template <typename TA, typename TX, typename TI>
+__device__ __forceinline__ void kb_force_block(const TA* __restrict__ Y,
+                                               const TX* __restrict__ At, int ns, int K2,
+                                               int ppad, double* __restrict__ F, int bx, int by) {
+  typedef typename Mfma<TI>::acc_t acc_t;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int s0 = bx * 32, k0 = by * 128 + wave * 32;
+  const int g = lane >> 4, r16 = lane & 15;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
+  const TA* arow[2];
+  const TX* xrow[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    arow[i] = Y + (size_t)(s0 + 16 * i + r16) * ppad + KPL * g;
+    xrow[i] = At + (size_t)(k0 + 16 * i + r16) * ppad + KPL * g;
+  }
+  TA ra0[2][KPL], ra1[2][KPL];
+  TX rb0[2][KPL], rb1[2][KPL];
+  auto load_raw = [&](int st, TA (&a)[2][KPL], TX (&b)[2][KPL]) {
+    const int kk = st * KS;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      load_run<TA>(arow[i] + kk, a[i]);
+      load_run<TX>(xrow[i] + kk, b[i]);
+    }
+  };
+  auto mma_raw = [&](TA (&a)[2][KPL], TX (&b)[2][KPL]) {
+#pragma unroll
+    for (int t = 0; t < KPL; ++t)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = Mfma<TI>::run(TI(a[i][t]), TI(b[j][t]), acc[i][j]);
+  };
+  const int nsteps = ppad / KS;
+  int st = 0;
+  load_raw(0, ra0, rb0);
+  while (st < nsteps) {
+    if (st + 1 < nsteps) load_raw(st + 1, ra1, rb1);
+    mma_raw(ra0, rb0);
+    if (++st >= nsteps) break;
+    if (st + 1 < nsteps) load_raw(st + 1, ra0, rb0);
+    mma_raw(ra1, rb1);
+    ++st;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int s = s0 + 16 * i + Mfma<TI>::row(lane, r);
+      if (s < ns) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) F[(size_t)s * K2 + k0 + 16 * j + r16] = double(acc[i][j][r]);
+      }
+    }
+}
+
+// 1-D launch over ts * (K2 / 128) blocks, XCD-swizzled, s fastest: the
+// blocks sharing one 128-column slab of At sit on one XCD.
+template <typename TA, typename TX, typename TI>
+__global__ __launch_bounds__(256) void k_kb_force(const TA* __restrict__ Y,
+                                                  const TX* __restrict__ At, int ns, int K2,
+                                                  int ppad, double* __restrict__ F, int ts,
+                                                  int total) {
+  const int L = xcd_swizzle(blockIdx.x, total);
+  kb_force_block<TA, TX, TI>(Y, At, ns, K2, ppad, F, L % ts, L / ts);
 }
 
 // 1-D launch over tp * ts * nsplit tiles.  Logical tile L (XCD-swizzled) is
@@ -253,13 +325,12 @@ template <typename TA, typename TI>
 __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __restrict__ args,
                                                         const int* __restrict__ prefix,
                                                         int nfrag) {
-  __shared__ double red[4][32 * 32];
   const int L = xcd_swizzle(blockIdx.x, prefix[nfrag]);
   const int f = find_frag(prefix, nfrag, L);
   const KbForceArgs a = args[f];
   const int local = L - prefix[f];
-  kb_project_tile<double, TA, TI>(a.Y, reinterpret_cast<const TA*>(a.At), a.ns, a.K2, a.ppad,
-                                  a.ppad / KS, a.F, local % a.ts, local / a.ts, 0, red);
+  kb_force_block<double, TA, TI>(a.Y, reinterpret_cast<const TA*>(a.At), a.ns, a.K2, a.ppad, a.F,
+                                 local % a.ts, local / a.ts);
 }
 
 static int ensure(void** p, size_t* have, size_t need) {
@@ -276,14 +347,15 @@ static int ensure(void** p, size_t* have, size_t need) {
   return 0;
 }
 
-// K split of the projection GEMM: about 512 workgroups, at least 16 K steps
-// (four per wave) per split.  Returns the split count, *per = steps per split.
+// K split of the projection GEMM: about c.kb_wg_target workgroups (512 unless
+// tuned), at least 16 K steps (four per wave) per split.  Returns the split
+// count, *per = steps per split.
 int kb_project_split(const Ctx& c, int ns, int* per) {
   const int P = c.nproj;
   const int K = 2 * c.g.ncp;  // multiple of 128
   const int tiles = ((P + 31) / 32) * ((ns + 31) / 32);
   const int nsteps = K / KS;
-  int nsplit = (512 + tiles - 1) / tiles;
+  int nsplit = (c.kb_wg_target + tiles - 1) / tiles;
   if (nsplit > nsteps / 16) nsplit = nsteps / 16;
   if (nsplit < 1) nsplit = 1;
   *per = (nsteps + nsplit - 1) / nsplit;
@@ -395,18 +467,16 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
   return 0;
 }
 
-// F[s][k] = sum_p Y[s][p] At[k][p]: the projection tile with the reduction
-// over the padded projector index, unsplit (it writes F directly)
+// F[s][k] = sum_p Y[s][p] At[k][p] (K2 = 2 ncp is a multiple of 128)
 int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
   const int K2 = 2 * c.g.ncp;
-  const int ts = (ns + 31) / 32, tk = (K2 + 31) / 32, tot = ts * tk;
-  const int per = c.ppad / KS;
+  const int ts = (ns + 31) / 32, tot = ts * (K2 / 128);
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
-                       c.Y, c.At64, ns, K2, c.ppad, per, fnl, ts, tk, tot);
+    hipLaunchKernelGGL((k_kb_force<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
+                       c.Y, c.At64, ns, K2, c.ppad, fnl, ts, tot);
   else
-    hipLaunchKernelGGL((k_kb_project<double, float, float>), dim3(tot), dim3(256), 0, c.stream,
-                       c.Y, c.At32, ns, K2, c.ppad, per, fnl, ts, tk, tot);
+    hipLaunchKernelGGL((k_kb_force<double, float, float>), dim3(tot), dim3(256), 0, c.stream, c.Y,
+                       c.At32, ns, K2, c.ppad, fnl, ts, tot);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -106,6 +106,10 @@ struct Ctx {
   int Y_ppad = 0;
   double* red = nullptr;     // reduction scratch
   size_t red_bytes = 0;
+  // launch configuration (the tuner's pick): plane-kernel tile variant (0
+  // base sub-chunk, 1 double) and the KB projection's target workgroup count
+  int plane_variant = 0;
+  int kb_wg_target = 512;
   // Newton-form accumulators for the contract kernel (set for the duration
   // of a Newton filter call only)
   int newton_nmu = 0;
@@ -213,7 +217,8 @@ enum ContractMode {
 };
 
 // KB GEMM arguments of one fragment of a batch (tile counts: tp over
-// projectors, ts over states, tk over the 2 ncp coefficient rows).
+// projectors, ts over states (32 each); tk over the 2 ncp coefficient rows in
+// 128-row blocks).
 struct KbProjectArgs {
   const void* A;  // [P][K] (double or float)
   const double* X;
@@ -287,11 +292,12 @@ PlaneArgs make_plane_args(const Ctx& c, int prec, void* W);
 ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W, int npairs,
                                 const double* Xcur, double* Xout, double* acc, const double* fnl,
                                 const double* chi, const StepParams& sp, double* mom_part);
-size_t plane_lds_for(int n, int prec, int M);
-// one launch of kind 0 (expand), 1 (plane) or 2 (contract) over the fragments
-// whose argument blocks are args[0..nfrag) (device), all of grid size n
+size_t plane_lds_for(int n, int prec, int M, int variant);
+// one launch of kind 0 (expand), 1 (plane, tile variant `variant`) or 2
+// (contract) over the fragments whose argument blocks are args[0..nfrag)
+// (device), all of grid size n
 int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
-                 int total, size_t lds, hipStream_t s);
+                 int total, size_t lds, int variant, hipStream_t s);
 int chunk_cols_for(int n);
 int round_fft(int n_in);
 

@@ -164,22 +164,28 @@ __global__ __launch_bounds__(kThreads) void k_expand_batch(const ExpandArgs* __r
 // scattered.  The b-lines read and write only the 2K+1 rows |kb| <= K of the
 // c-transformed plane; their inverse last pass, the v_KS multiply and the
 // forward first pass run fused in registers.
-template <typename T, int N>
+// Tile variants of the b-line sub-chunk CB (columns rc transformed together):
+// V = 0 the base width (one radix butterfly per thread and pass at n = 64),
+// V = 1 twice it (half the sub-chunks and barriers, two butterflies per
+// thread); the tuner picks per context or batch.
+template <typename T, int N, int V = 0>
 struct PlaneCfg {
   static constexpr int ML = Cfg<N>::ML;
-  static constexpr int CB = largest_divisor_le(N, ML < (sizeof(T) == 4 ? 32 : 16)
-                                                      ? ML
-                                                      : (sizeof(T) == 4 ? 32 : 16));
+  static constexpr int CB0 = largest_divisor_le(N, ML < (sizeof(T) == 4 ? 32 : 16)
+                                                       ? ML
+                                                       : (sizeof(T) == 4 ? 32 : 16));
+  static constexpr int CB = V == 0 ? CB0 : largest_divisor_le(N, 2 * CB0 < N ? 2 * CB0 : N);
 };
 
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 size_t plane2_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * Cfg<N>::LD + (size_t)PlaneCfg<T, N>::CB * Cfg<N>::LD + N);
+  return sizeof(Cx<T>) *
+         ((size_t)M * Cfg<N>::LD + (size_t)PlaneCfg<T, N, V>::CB * Cfg<N>::LD + N);
 }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
+  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N, V>::CB;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
@@ -245,19 +251,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   }
 }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 __global__ __launch_bounds__(kThreads) void k_plane(PlaneArgs a) {
-  plane_body<T, N>(a, blockIdx.x, blockIdx.y);
+  plane_body<T, N, V>(a, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 __global__ __launch_bounds__(kThreads) void k_plane_batch(const PlaneArgs* __restrict__ args,
                                                           const int* __restrict__ prefix,
                                                           int nfrag) {
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const PlaneArgs a = args[f];
   const int local = blockIdx.x - prefix[f];
-  plane_body<T, N>(a, local % a.gx, local / a.gx);
+  plane_body<T, N, V>(a, local % a.gx, local / a.gx);
 }
 
 // ---- K2, split form: for planes whose M x (n+1) buffer exceeds LDS --------
@@ -875,11 +881,13 @@ ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W,
   return a;
 }
 
-size_t plane_lds_for(int n, int prec, int M) {
+size_t plane_lds_for(int n, int prec, int M, int variant) {
   switch (n) {
-#define SDFT_PL(NN)                                                                 \
-  case NN:                                                                          \
-    return prec == 0 ? plane2_lds_bytes<double, NN>(M) : plane2_lds_bytes<float, NN>(M);
+#define SDFT_PL(NN)                                                                    \
+  case NN:                                                                             \
+    if (variant == 1)                                                                  \
+      return prec == 0 ? plane2_lds_bytes<double, NN, 1>(M) : plane2_lds_bytes<float, NN, 1>(M); \
+    return prec == 0 ? plane2_lds_bytes<double, NN, 0>(M) : plane2_lds_bytes<float, NN, 0>(M);
     SDFT_PL(24) SDFT_PL(32) SDFT_PL(36) SDFT_PL(48) SDFT_PL(64) SDFT_PL(96) SDFT_PL(128)
     SDFT_PL(192)
 #undef SDFT_PL
@@ -892,15 +900,20 @@ size_t plane_lds_for(int n, int prec, int M) {
 // size; `prefix` holds each fragment's first workgroup (size nfrag + 1).
 template <typename T, int N>
 static int batch_launch_impl(int kind, const void* args, const int* prefix, int nfrag, int total,
-                             size_t lds, hipStream_t s) {
+                             size_t lds, int variant, hipStream_t s) {
   if (kind == 0) {
     hipLaunchKernelGGL((k_expand_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
                        reinterpret_cast<const ExpandArgs*>(args), prefix, nfrag);
   } else if (kind == 1) {
-    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_batch<T, N>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_plane_batch<T, N>), dim3(total), dim3(kThreads), lds, s,
-                       reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+    const void* fn = variant == 1 ? reinterpret_cast<const void*>(&k_plane_batch<T, N, 1>)
+                                  : reinterpret_cast<const void*>(&k_plane_batch<T, N, 0>);
+    SDFT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (variant == 1)
+      hipLaunchKernelGGL((k_plane_batch<T, N, 1>), dim3(total), dim3(kThreads), lds, s,
+                         reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+    else
+      hipLaunchKernelGGL((k_plane_batch<T, N, 0>), dim3(total), dim3(kThreads), lds, s,
+                         reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
   } else {
     hipLaunchKernelGGL((k_contract_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
                        reinterpret_cast<const ContractArgs*>(args), prefix, nfrag);
@@ -910,14 +923,14 @@ static int batch_launch_impl(int kind, const void* args, const int* prefix, int 
 }
 
 int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
-                 int total, size_t lds, hipStream_t s) {
+                 int total, size_t lds, int variant, hipStream_t s) {
   if (total <= 0) return 0;
   if (prec == 0) {
     SDFT_SIZE_SWITCH(n, return (batch_launch_impl<double, NN>(kind, args, prefix, nfrag, total,
-                                                               lds, s)));
+                                                               lds, variant, s)));
   } else {
     SDFT_SIZE_SWITCH(n, return (batch_launch_impl<float, NN>(kind, args, prefix, nfrag, total, lds,
-                                                              s)));
+                                                              variant, s)));
   }
   return 0;
 }
@@ -949,13 +962,22 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
 template <typename T, int N>
 static int plane_impl(Ctx& c, void* W, int npairs, void** Wout) {
   const Geometry& g = c.g;
-  const size_t lds = plane2_lds_bytes<T, N>(g.M);
   PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
   dim3 grid(N, npairs);
+  const size_t lds1 = plane2_lds_bytes<T, N, 1>(g.M);
+  if (c.plane_variant == 1 && lds1 <= 160 * 1024) {
+    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, 1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    hipLaunchKernelGGL((k_plane<T, N, 1>), grid, dim3(kThreads), lds1, c.stream, a);
+    SDFT_HIP_CHECK(hipGetLastError());
+    *Wout = W;
+    return 0;
+  }
+  const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
   if (lds <= 160 * 1024) {
-    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N>),
+    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, 0>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_plane<T, N>), grid, dim3(kThreads), lds, c.stream, a);
+    hipLaunchKernelGGL((k_plane<T, N, 0>), grid, dim3(kThreads), lds, c.stream, a);
     SDFT_HIP_CHECK(hipGetLastError());
     *Wout = W;
     return 0;

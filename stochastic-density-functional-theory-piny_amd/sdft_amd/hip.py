@@ -42,7 +42,7 @@ EXPORTS = [
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
-    "sdft_hip_spectral_bound", "sdft_hip_newton_filter",
+    "sdft_hip_spectral_bound", "sdft_hip_newton_filter", "sdft_hip_tune", "sdft_hip_set_config",
 ]
 
 _LIB = None
@@ -92,14 +92,16 @@ def load(path: str | None = None):
         "sdft_hip_batch_create": ([C.c_int, C.POINTER(vp), C.POINTER(vp)], C.c_int),
         "sdft_hip_batch_destroy": ([vp], C.c_int),
         "sdft_hip_batch_filter": ([vp, C.c_int, _dp, _dp, _dp, C.c_int], C.c_int),
-        "sdft_hip_batch_autotune": ([vp, C.c_int, _ip, _ip, C.c_int, _dp], C.c_int),
-        "sdft_hip_batch_config": ([vp, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_batch_autotune": ([vp, C.c_int, _ip, C.c_int, _dp], C.c_int),
+        "sdft_hip_batch_config": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
                                           C.c_double], C.c_int),
         "sdft_hip_spectral_bound": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                                      _ip, C.c_int], C.c_int),
         "sdft_hip_newton_filter": ([vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
                                     C.c_double, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
+        "sdft_hip_tune": ([vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _dp], C.c_int),
+        "sdft_hip_set_config": ([vp, C.c_int, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -202,6 +204,19 @@ class Context:
         _check(self.lib.sdft_hip_hpsi(self.h, re.shape[0], _ptr(re), _ptr(im), _ptr(fre),
                                       _ptr(fim), prec))
         return fre + 1j * fim
+
+    def tune(self, nstate, prec=PREC_FP64, steps=6):
+        """sdft_hip_tune: returns ((plane_tile, kb_wg_target), per-candidate ms)."""
+        cfg = np.zeros(2, dtype=np.int32)
+        ms = np.zeros(16)
+        n = self.lib.sdft_hip_tune(self.h, nstate, prec, steps, cfg.ctypes.data_as(_ip), 16,
+                                   _ptr(ms))
+        if n < 0:
+            _check(n)
+        return (int(cfg[0]), int(cfg[1])), ms[:n].copy()
+
+    def set_config(self, plane_tile, kb_wg_target=512):
+        _check(self.lib.sdft_hip_set_config(self.h, plane_tile, kb_wg_target))
 
     def newton_filter(self, chi, coeffs, samp_point, scale, emean, prec=PREC_FP64):
         """filterNewtonPolyHerm: coeffs (L, nmu), samp_point (L-1,).  Returns
@@ -397,14 +412,13 @@ class Batch:
                                               1 if use_graph else 0))
 
     def autotune(self, steps=8):
-        split = C.c_int(0)
-        ov = C.c_int(0)
-        ms = np.zeros(16)
-        n = self.lib.sdft_hip_batch_autotune(self.h, steps, C.byref(split), C.byref(ov), 16,
-                                             _ptr(ms))
+        """Returns ((split, overlap, plane_tile), per-candidate ms/step)."""
+        cfg = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(32)
+        n = self.lib.sdft_hip_batch_autotune(self.h, steps, cfg.ctypes.data_as(_ip), 32, _ptr(ms))
         if n < 0:
             _check(n)
-        return split.value, ov.value, ms[:n].copy()
+        return tuple(int(x) for x in cfg), ms[:n].copy()
 
-    def config(self, split, overlap):
-        _check(self.lib.sdft_hip_batch_config(self.h, split, overlap))
+    def config(self, split, overlap, plane_tile=0):
+        _check(self.lib.sdft_hip_batch_config(self.h, split, overlap, plane_tile))

@@ -123,21 +123,22 @@ def test_batch_fp32_matches_oracle(hip, frags):
 
 
 def test_batch_autotune_and_configs(hip, frags):
-    say("case: batch autotune + every KB split / overlap configuration, L=9")
+    say("case: batch autotune + every KB split / overlap / plane tile configuration, L=9")
     L = 9
     co, scale, emean = _coeffs(frags, L)
     ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP64)
-    split, ov, ms = bat.autotune(steps=4)
-    assert split in (1, 2, 4, 8) and ov in (0, 1)
-    assert len(ms) == 8 and np.all(ms > 0)
+    (split, ov, pv), ms = bat.autotune(steps=4)
+    assert split in (1, 2, 4, 8) and ov in (0, 1) and pv in (0, 1)
+    assert len(ms) == 16 and np.all(ms > 0)
     refs = [O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])[0]
             for f, (sysm, chi, _, _) in enumerate(frags)]
     for sp in (1, 2, 4, 8):
         for o in (0, 1):
-            bat.config(sp, o)
-            bat.filter(co, scale, emean, use_graph=True)
-            for f in range(len(frags)):
-                assert rel(blocks[f].get(0), refs[f]) < 1e-10, (sp, o, f)
+            for t in (0, 1):
+                bat.config(sp, o, t)
+                bat.filter(co, scale, emean, use_graph=True)
+                for f in range(len(frags)):
+                    assert rel(blocks[f].get(0), refs[f]) < 1e-10, (sp, o, t, f)
     bat.close()
 
 
@@ -155,4 +156,42 @@ def test_batch_rejects_bad_input(hip, frags):
     bat = hip.Batch([b64])
     with pytest.raises(hip.SdftError):
         bat.filter(np.ones((1, 1)), np.ones(1), np.zeros(1))  # L < 2
+    bat.close()
+
+
+def test_batch_mixed_local_only_and_real_space_fragments(hip, frags):
+    """A batch whose fragments mix the three KB forms: G-space table, no
+    projectors (local potential only: the contract takes no NL force) and
+    real-space projectors (device-built table); each against the oracle."""
+    from sdft_amd import systems
+    say("case: batch with local-only and real-space-KB fragments, L=9")
+    L = 9
+    sub = frags[:4]
+    co, scale, emean = _coeffs(sub, L)
+    ctxs, blocks = [], []
+    kinds = ["gspace", "local", "real", "gspace"]
+    rs = {}
+    for i, (sysm, chi, _, _) in enumerate(sub):
+        if kinds[i] == "local":
+            c = hip.Context(sysm, with_nl=False)
+        elif kinds[i] == "real":
+            c = hip.Context(sysm, with_nl=False)
+            rs[i] = systems.real_space_projectors(sysm, 0.45 * sysm.L)
+            c.set_projectors_real(rs[i])
+        else:
+            c = hip.Context(sysm)
+        b = hip.Block(c, chi.shape[0], hip.PREC_FP64)
+        b.set(chi)
+        ctxs.append(c)
+        blocks.append(b)
+    bat = hip.Batch(blocks)
+    bat.filter(co, scale, emean)
+    for i, (sysm, chi, _, _) in enumerate(sub):
+        if kinds[i] == "local":
+            ref, _ = O.filter_cheby(chi, co[i], sysm, scale[i], emean[i], with_nl=False)
+        elif kinds[i] == "real":
+            ref, _ = O.filter_cheby(chi, co[i], sysm, scale[i], emean[i], nl_real=rs[i])
+        else:
+            ref, _ = O.filter_cheby(chi, co[i], sysm, scale[i], emean[i])
+        assert rel(blocks[i].get(0), ref) < 1e-10, (i, kinds[i])
     bat.close()

@@ -302,3 +302,29 @@ def test_newton_filter_matches_oracle(hip, si8_ctx, si8_system, prec, tol):
     for m in range(3):
         assert rel(out_g[m], out_r[m]) < tol, (m, rel(out_g[m], out_r[m]))
     assert rel(pl_g, pl_r) < tol
+
+
+@pytest.fixture(scope="module")
+def si64_system():
+    from sdft_amd import systems
+    return systems.build_system("si64", 10.26, 2, 64)
+
+
+def test_plane_tiles_and_tuner_si64(hip, si64_system):
+    """Both plane-kernel b-line tiles (sdft_hip_set_config) at 64^3 against
+    the oracle (FP64 1e-12, FP32 2e-5), and the tuner returns one of the
+    candidates."""
+    from sdft_amd import systems
+    say("case: plane tiles 0/1 + tuner si64")
+    ctx = hip.Context(si64_system)
+    C = systems.noise_orbitals(si64_system, 4, seed=23)
+    ref = O.hpsi_raw(C, si64_system)
+    for tile in (0, 1):
+        for kt in (256, 2048):
+            ctx.set_config(tile, kt)
+            assert rel(ctx.hpsi_raw(C), ref) < 1e-12, (tile, kt)
+            assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5, (tile, kt)
+    (tile, kt), ms = ctx.tune(8, hip.PREC_FP32, steps=2)
+    assert tile in (0, 1) and kt in (256, 512, 1024, 2048)
+    assert len(ms) == 8 and np.all(ms > 0)
+    assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
