@@ -30,7 +30,8 @@ for p in (REPO, PKG):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6   # vector (and matrix) FP64
+FP32_VALU_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -205,12 +206,15 @@ def main():
         tfl = flops / avg_s / 1e12
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic("k_plane<float, %d>" % wl.n, wl),
-                    "kernel": "k_plane<float,%d> (b/c transforms + v.psi)" % wl.n,
+                    "traffic": measured_traffic("k_plane<float, %d, %d>" % (
+                        wl.n, tuned["fp32"]["plane_tile"]), wl),
+                    "kernel": "k_plane<float,%d,%d> (b/c transforms + v.psi)" % (
+                        wl.n, tuned["fp32"]["plane_tile"]),
                     "avg_launch_ms": round(avg_s * 1e3, 4),
                     "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
                     "achieved_tflops": round(tfl, 2),
-                    "valu_frac": round(tfl / FP64_PEAK_TFLOPS, 4)}
+                    "valu_peak_tflops": FP32_VALU_PEAK_TFLOPS,
+                    "valu_frac": round(tfl / FP32_VALU_PEAK_TFLOPS, 4)}
     else:
         roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": None, "traffic": None, "kernel": dom_name,

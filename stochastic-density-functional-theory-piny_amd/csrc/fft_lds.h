@@ -24,8 +24,10 @@
 
 namespace sdft {
 
+// A complex value, aligned to its size so that loads and stores of one
+// element are single 64-bit (float) or 128-bit (double) accesses.
 template <typename T>
-struct Cx {
+struct alignas(2 * sizeof(T)) Cx {
   T x, y;
 };
 

@@ -207,9 +207,15 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   for (int r0 = 0; r0 < M; r0 += ML) {
     const int nl = M - r0 < ML ? M - r0 : ML;
     const int* rc = rowcol + (size_t)r0 * N;
+    // loads are unconditional (a column outside the sphere reads entry 0)
+    // and masked by a multiply afterwards: one vector load per element, all
+    // of a butterfly's index and data loads in flight together, no
+    // divergent branch (a select would be turned back into a guarded load)
     auto lw = [&](int l, int pos) {
       const int d = rc[l * N + pos];
-      return d >= 0 ? Wp[d] : zero;
+      const Cx<T> x = Wp[d >= 0 ? d : 0];
+      const T m = d >= 0 ? T(1) : T(0);
+      return cx(x.x * m, x.y * m);
     };
     Cx<T>* buf = Pl + r0 * LD;
     auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LD + pos] = x; };
@@ -221,7 +227,9 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   for (int rc0 = 0; rc0 < N; rc0 += CB) {
     auto lp = [&](int l, int pos) {
       const int kb = pos <= N / 2 ? pos : pos - N;
-      return (kb >= -K && kb <= K) ? Pl[(kb + K) * LD + rc0 + l] : zero;
+      const bool in = kb >= -K && kb <= K;
+      const Cx<T> x = Pl[(in ? kb + K : 0) * LD + rc0 + l];
+      return in ? x : zero;
     };
     auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
     auto sq = [&](int l, int pos, Cx<T> x) { Q[l * LD + pos] = x; };
@@ -317,7 +325,9 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
       const int* rc = rowcol + (size_t)r0 * N;
       auto lw = [&](int l, int pos) {
         const int d = rc[l * N + pos];
-        return d >= 0 ? Wp[d] : zero;
+        const Cx<T> x = Wp[d >= 0 ? d : 0];
+        const T m = d >= 0 ? T(1) : T(0);
+        return cx(x.x * m, x.y * m);
       };
       auto sh = [&](int l, int pos, Cx<T> x) {
         const int c = pos - c_lo;
@@ -723,7 +733,9 @@ __global__ __launch_bounds__(kThreads) void k_density_split(
         const int* rc = rowcol + (size_t)r0 * N;
         auto lw = [&](int l, int pos) {
           const int d = rc[l * N + pos];
-          return d >= 0 ? Wp[d] : zero;
+          const Cx<T> x = Wp[d >= 0 ? d : 0];
+          const T m = d >= 0 ? T(1) : T(0);
+          return cx(x.x * m, x.y * m);
         };
         auto sh = [&](int l, int pos, Cx<T> x) {
           const int c = pos - c_lo;
