@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--batch-config", default=None,
                     help="fixed KB split,overlap,plane_tile for the batch (e.g. 2,0,0) instead of the "
                          "autotuner")
+    ap.add_argument("--config", default=None,
+                    help="plane_tile,kb_wg_target,fused_expand for the single-system filter "
+                         "instead of the tuner's pick")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware counters: an 8-term FP32 and FP64 filter of "
                          "the workload, then a 256 MiB plain copy (counter calibration)")
@@ -138,11 +141,16 @@ def main():
     tuned = {}
     for prec, tag in ((hip.PREC_FP32, "fp32"), (hip.PREC_FP64, "fp64")):
         # launch configuration for this precision (plane tile, KB workgroups)
-        cfg, cand = ctx.tune(nst, prec, steps=4)
-        tuned[tag] = {"plane_tile": cfg[0], "kb_wg_target": cfg[1],
+        if args.config:
+            cfg = tuple(int(x) for x in args.config.split(","))
+            ctx.set_config(*cfg)
+            cand = np.zeros(0)
+        else:
+            cfg, cand = ctx.tune(nst, prec, steps=4)
+        tuned[tag] = {"plane_tile": cfg[0], "kb_wg_target": cfg[1], "fused_expand": cfg[2],
                       "candidates_ms_per_step": np.round(cand, 4).tolist()}
-        log("%s tune: plane_tile=%d kb_wg_target=%d candidates %s" % (
-            tag, cfg[0], cfg[1], np.round(cand, 4).tolist()))
+        log("%s tune: plane_tile=%d kb_wg_target=%d fused_expand=%d candidates %s" % (
+            tag, cfg[0], cfg[1], cfg[2], np.round(cand, 4).tolist()))
         blk = hip.Block(ctx, nst, prec)
         blk.set(chi)
         for _ in range(args.warmup):

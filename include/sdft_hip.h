@@ -243,16 +243,20 @@ int sdft_hip_batch_autotune(sdft_batch *bat, int steps, int *config_out, int max
 int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap, int plane_tile);
 
 /* Launch-configuration tuner for one context (no reference counterpart):
- * times `steps` H applications of an nstate-orbital block (device noise) in
- * precision prec for each candidate -- plane-kernel b-line tile (0 base
- * sub-chunk, 1 twice as wide, where it fits LDS) x KB projection workgroup
- * target (256, 512, 1024, 2048) -- and keeps the fastest for this context.
- * config_out (nullable, 2 ints): tile, workgroup target; cand_ms (nullable):
- * per-candidate ms per H application of the block.  Returns the number of
- * candidates (8).  sdft_hip_set_config sets the two knobs by hand. */
+ * times `steps` Chebyshev filter steps of an nstate-orbital block (device
+ * noise, zero coefficients) in precision prec for each candidate --
+ * plane-kernel b-line tile (0 base sub-chunk, 1 twice as wide, where it fits
+ * LDS) x KB projection workgroup target (256, 1024) x fused expand (0: an
+ * expand kernel per step, 1: the contract kernel writes the next step's
+ * a-expansion; offered only when every sphere column is one ka interval) --
+ * and keeps the fastest for this context.  config_out (nullable, 3 ints):
+ * tile, workgroup target, fused expand; cand_ms (nullable): per-candidate ms
+ * per filter step of the block.  Returns the number of candidates (8, or 4
+ * without the fused form).  sdft_hip_set_config sets the three knobs by
+ * hand. */
 int sdft_hip_tune(sdft_ctx *ctx, int nstate, int prec, int steps, int *config_out, int max_cand,
                   double *cand_ms);
-int sdft_hip_set_config(sdft_ctx *ctx, int plane_tile, int kb_wg_target);
+int sdft_hip_set_config(sdft_ctx *ctx, int plane_tile, int kb_wg_target, int fused_expand);
 
 /* Per-kernel timing of the last block_filter call when profiling is on:
  * names[i] / ms[i] / launches[i] for up to max entries; returns the count. */

@@ -154,6 +154,15 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
   }
   order.push_back({0, 0});
   g.ncol = (int)order.size();
+  g.col_intervals = true;
+  for (auto& key : order) {
+    std::vector<int> kas;
+    for (auto& e : cols[key]) kas.push_back(std::get<0>(e));
+    std::sort(kas.begin(), kas.end());
+    const int h = ((int)kas.size() - 1) / 2;
+    for (int t = 0; t < (int)kas.size(); ++t)
+      if (kas[t] != t - h) g.col_intervals = false;
+  }
   g.ncolp = (g.ncol + 3) / 4 * 4;
   std::map<std::pair<int, int>, int> colid;
   for (int dd = 0; dd < g.ncol; ++dd) colid[order[dd]] = dd;
@@ -264,7 +273,7 @@ static void free_ctx(Ctx& c) {
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
                   c.At64,       c.At32,
-                  c.E,          c.W,            c.W2,          c.zpart,        c.Y,
+                  c.E,          c.W,            c.W2,          c.zpart,        c.Y,           c.Y32,
                   c.Z,          c.red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -375,12 +384,16 @@ static int block_alloc(Ctx& c, int nstate, int prec, Block& b) {
   SDFT_HIP_CHECK(hipMemsetAsync(b.acc, 0, bytes, c.stream));
   SDFT_HIP_CHECK(hipMemsetAsync(b.chi, 0, bytes, c.stream));
   SDFT_HIP_CHECK(hipMemsetAsync(b.fnl, 0, bytes, c.stream));
+  if (prec == 1) {
+    if ((rc = dalloc(&b.X32, xbytes / 2))) return rc;
+    SDFT_HIP_CHECK(hipMemsetAsync(b.X32, 0, xbytes / 2, c.stream));
+  }
   b.cur = 0;
   return 0;
 }
 
 static void block_free(Block& b) {
-  void* ptrs[] = {b.X0, b.X1, b.acc, b.chi, b.fnl, b.mom_part};
+  void* ptrs[] = {b.X0, b.X1, b.acc, b.chi, b.fnl, b.mom_part, b.X32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b = Block();
@@ -425,14 +438,37 @@ static int block_fetch(Block& b, const double* src, double* re, double* im) {
   return 0;
 }
 
-// One application of the H sandwich to `cur`; `mode` selects the epilogue.
+__global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = float(x[i]);
+}
+
+// apply_step flags: state a previous step of the same loop left behind, and
+// what this step leaves for the next one.
+enum StepFlags {
+  kHaveW = 1,    // W holds the a-expansion of `cur` (no expand kernel)
+  kEmitW = 2,    // the contract writes the a-expansion of `out` into W
+  kHaveX32 = 4,  // FP32 form: X32 holds `cur` in float (no conversion)
+  kEmitX32 = 8   // FP32 form: the contract writes `out` in float into X32
+};
+
+// One application of the H sandwich to `cur`; `mode` selects the epilogue,
+// `flags` (StepFlags) the state carried between the steps of a loop.
 static int apply_step(Block& b, int mode, const double* cur, double* out, const StepParams& sp,
-                      double* mom_part) {
+                      double* mom_part, int flags = 0) {
   Ctx& c = *b.ctx;
   const int npairs = b.nstate_pad / 2;
   int rc;
   if ((rc = ensure_W(c, w_bytes(c, npairs, b.prec)))) return rc;
-  const double* fnl = nullptr;
+  const void* fnl = nullptr;
+  // the FP32 form's KB products read T in float
+  const bool x32 = b.prec == 1 && c.nproj > 0;
+  if (x32 && !(flags & kHaveX32)) {
+    const size_t n = (size_t)b.nstate_pad * 2 * c.g.ncp;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, cur,
+                       b.X32, n);
+    SDFT_HIP_CHECK(hipGetLastError());
+  }
   // The KB chain runs on its own stream, overlapping the FFT kernels; with
   // profiling on, each kernel's events are recorded on the stream it runs on.
   const bool overlap = c.nproj > 0;
@@ -445,7 +481,8 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
     }
     {
       ProfScope ps(c, "kb_project");
-      rc = launch_nl_project(c, b.prec, cur, b.nstate_pad);
+      rc = launch_nl_project(c, b.prec, x32 ? (const void*)b.X32 : (const void*)cur,
+                             b.nstate_pad);
     }
     if (!rc) {
       ProfScope ps(c, "kb_force");
@@ -458,7 +495,7 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
     if (rc) return rc;
     fnl = b.fnl;
   }
-  {
+  if (!(flags & kHaveW)) {
     ProfScope ps(c, "expand_a");
     if ((rc = launch_expand(c, b.prec, cur, npairs, c.W))) return rc;
   }
@@ -471,10 +508,21 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
   {
     ProfScope ps(c, "contract_a");
     if ((rc = launch_contract(c, b.prec, mode, Wres, npairs, cur, out, b.acc, fnl, b.chi, sp,
-                              mom_part)))
+                              mom_part, (flags & kEmitW) ? c.W : nullptr,
+                              (x32 && (flags & kEmitX32)) ? b.X32 : nullptr)))
       return rc;
   }
   return 0;
+}
+
+// apply_step flags for step k of a loop over 1 <= k < L: from the second
+// step on, W (when the contract may re-expand: interval columns, and the
+// tuner has not switched it off) and X32 hold what the previous contract
+// wrote, and every step but the last writes them for its output.
+static int fuse_flags(const Ctx& c, int k, int L) {
+  int f = (k > 1 ? kHaveX32 : 0) | (k + 1 < L ? kEmitX32 : 0);
+  if (c.g.col_intervals && c.fuse_expand) f |= (k > 1 ? kHaveW : 0) | (k + 1 < L ? kEmitW : 0);
+  return f;
 }
 
 static int check_vks(Ctx& c) {
@@ -567,7 +615,9 @@ static int block_filter_impl(Block& b, int L, const double* coeffs, double scale
     sp.step = k;
     double* cur = bufs[(k - 1) & 1];
     double* out = bufs[k & 1];
-    if ((rc = apply_step(b, kModeFilter, cur, out, sp, moments ? b.mom_part : nullptr))) return rc;
+    if ((rc = apply_step(b, kModeFilter, cur, out, sp, moments ? b.mom_part : nullptr,
+                         fuse_flags(c, k, L))))
+      return rc;
   }
   // T_{L-1} sits in bufs[(L-1) & 1]
   b.cur = (bufs[(L - 1) & 1] == b.X0) ? 0 : 1;
@@ -753,8 +803,8 @@ int sdft_hip_init(const sdft_grid_desc* desc, sdft_ctx** out) {
   }
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream_nl, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_cur, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_nl, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_cur, kSyncEventFlags);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_nl, kSyncEventFlags);
   if (e != hipSuccess) {
     free_ctx(c);
     delete h;
@@ -923,39 +973,63 @@ int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out,
     set_error("hipEventCreate failed");
     rc = SDFT_EHIP;
   }
-  const int targets[4] = {256, 512, 1024, 2048};
+  // candidates: plane tile x KB workgroup target x fused expand (the last
+  // only where the columns allow it), each timed on filter steps k >= 3 of
+  // a loop as block_filter_impl runs it (zero coefficients: the accumulator
+  // is left as it is; a tiny scale keeps T_k bounded)
+  const int targets[2] = {256, 1024};
+  const int nfuse = c.g.col_intervals ? 2 : 1;
   const int pv0 = c.plane_variant, kt0 = c.kb_wg_target;
+  const bool fu0 = c.fuse_expand;
   int best_pv = pv0, best_kt = kt0, ncand = 0;
+  bool best_fu = fu0;
   double best = 1e300;
-  StepParams sp{};
+  const int L = steps + 4;
+  double* bufs[2] = {b.X0, b.X1};
+  auto step_params = [](int k) {
+    StepParams sp{};
+    const double m = (k == 1) ? 1.0 : 2.0, scale = 1e-4;
+    sp.s1 = -scale * 0.25 * m;
+    sp.s2 = -scale * 0.5 * m;
+    sp.first = (k == 1);
+    sp.step = k;
+    return sp;
+  };
   for (int pv = 0; pv < 2 && !rc; ++pv)
-    for (int ti = 0; ti < 4 && !rc; ++ti) {
-      c.plane_variant = pv;
-      c.kb_wg_target = targets[ti];
-      if ((rc = apply_step(b, kModeForce, b.X0, b.X1, sp, nullptr))) break;  // warm-up
-      if (hipEventRecord(t0, c.stream) != hipSuccess) {
-        rc = SDFT_EHIP;
-        break;
+    for (int ti = 0; ti < 2 && !rc; ++ti)
+      for (int fu = 0; fu < nfuse && !rc; ++fu) {
+        c.plane_variant = pv;
+        c.kb_wg_target = targets[ti];
+        c.fuse_expand = fu != 0;
+        for (int k = 1; k <= 2 && !rc; ++k)  // warm-up: the first two steps
+          rc = apply_step(b, kModeFilter, bufs[(k - 1) & 1], bufs[k & 1], step_params(k),
+                          nullptr, fuse_flags(c, k, L));
+        if (rc) break;
+        if (hipEventRecord(t0, c.stream) != hipSuccess) {
+          rc = SDFT_EHIP;
+          break;
+        }
+        for (int k = 3; k < steps + 3 && !rc; ++k)
+          rc = apply_step(b, kModeFilter, bufs[(k - 1) & 1], bufs[k & 1], step_params(k),
+                          nullptr, fuse_flags(c, k, L));
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(t1, c.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+            hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+          set_error("tune timing failed");
+          rc = SDFT_EHIP;
+          break;
+        }
+        const double per = ms / steps;
+        if (cand_ms && ncand < max_cand) cand_ms[ncand] = per;
+        ++ncand;
+        if (per < best) {
+          best = per;
+          best_pv = pv;
+          best_kt = targets[ti];
+          best_fu = fu != 0;
+        }
       }
-      for (int k = 0; k < steps && !rc; ++k)
-        rc = apply_step(b, kModeForce, b.X0, b.X1, sp, nullptr);
-      if (rc) break;
-      float ms = 0.f;
-      if (hipEventRecord(t1, c.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
-          hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
-        set_error("tune timing failed");
-        rc = SDFT_EHIP;
-        break;
-      }
-      const double per = ms / steps;
-      if (cand_ms && ncand < max_cand) cand_ms[ncand] = per;
-      ++ncand;
-      if (per < best) {
-        best = per;
-        best_pv = pv;
-        best_kt = targets[ti];
-      }
-    }
   c.profiling = prof;
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
@@ -964,24 +1038,29 @@ int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out,
   if (rc) {
     c.plane_variant = pv0;
     c.kb_wg_target = kt0;
+    c.fuse_expand = fu0;
     return rc;
   }
   c.plane_variant = best_pv;
   c.kb_wg_target = best_kt;
+  c.fuse_expand = best_fu;
   if (config_out) {
     config_out[0] = best_pv;
     config_out[1] = best_kt;
+    config_out[2] = best_fu ? 1 : 0;
   }
   return ncand;
 }
 
-int sdft_hip_set_config(sdft_ctx* h, int plane_tile, int kb_wg_target) {
-  if (!h || (plane_tile != 0 && plane_tile != 1) || kb_wg_target < 1) {
+int sdft_hip_set_config(sdft_ctx* h, int plane_tile, int kb_wg_target, int fused_expand) {
+  if (!h || (plane_tile != 0 && plane_tile != 1) || kb_wg_target < 1 ||
+      (fused_expand != 0 && fused_expand != 1)) {
     set_error("bad configuration");
     return SDFT_EINVAL;
   }
   h->c.plane_variant = plane_tile;
   h->c.kb_wg_target = kb_wg_target;
+  h->c.fuse_expand = fused_expand != 0;
   return 0;
 }
 
@@ -1037,7 +1116,7 @@ int sdft_hip_newton_filter(sdft_ctx* h, int nstate, int L, int nmu, const double
       sp.c0 = 0.0;
       sp.first = (n == 1);
       sp.step = n;
-      rc = apply_step(b, kModeNewton, X[(n - 1) & 1], X[n & 1], sp, nullptr);
+      rc = apply_step(b, kModeNewton, X[(n - 1) & 1], X[n & 1], sp, nullptr, fuse_flags(c, n, L));
     }
     b.acc = own_acc;
     c.newton_nmu = 0;
@@ -1157,7 +1236,8 @@ int sdft_hip_cheby_moments(sdft_ctx* h, int nstate, int L, double scale, double 
       sp.c0 = 0.0;
       sp.first = (k == 1);
       sp.step = k;
-      rc = apply_step(b, kModeMoments, bufs[(k - 1) & 1], bufs[k & 1], sp, b.mom_part);
+      rc = apply_step(b, kModeMoments, bufs[(k - 1) & 1], bufs[k & 1], sp, b.mom_part,
+                      fuse_flags(c, k, nmom));
     }
     if (!rc) rc = prof_collect(c);
   }

@@ -40,6 +40,7 @@ struct CopyArgs {
   const double* src;
   double* dst;
   long long n;
+  float* dst32;  // FP32 form with projectors: the float copy as well
 };
 
 __global__ void k_copy_batch(const CopyArgs* __restrict__ args, const int* __restrict__ prefix,
@@ -47,7 +48,11 @@ __global__ void k_copy_batch(const CopyArgs* __restrict__ args, const int* __res
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const CopyArgs a = args[f];
   const long long i = (long long)(blockIdx.x - prefix[f]) * 256 + threadIdx.x;
-  if (i < a.n) a.dst[i] = a.src[i];
+  if (i < a.n) {
+    const double x = a.src[i];
+    a.dst[i] = x;
+    if (a.dst32) a.dst32[i] = float(x);
+  }
 }
 
 constexpr int kVariants = 3;  // 0: first step (X0 -> X1), 1: even k (X1 -> X0), 2: odd k >= 3
@@ -110,6 +115,9 @@ struct Batch {
   std::vector<int> nl_frags;
   int split = 1;       // target slices of the projection's reduction
   bool overlap = true;
+  // every fragment's columns are ka intervals: the contract writes the next
+  // step's expansion, and only the first step launches the expand kernel
+  bool fused = true;
   int plane_variant = 0;  // plane-kernel tile variant (falls back to 0 per group)
   int tot_proj = 0, tot_red = 0, tot_force = 0;
   int* pre_proj = nullptr;
@@ -188,7 +196,7 @@ int build_kb_tables(Batch& B) {
     const int P = f.c->nproj, S = f.b->nstate_pad, K2 = 2 * f.c->g.ncp;
     pp.push_back(pp.back() + ((P + 31) / 32) * ((S + 31) / 32) * ns);
     pr.push_back(pr.back() + (P * S + 255) / 256);
-    pf.push_back(pf.back() + ((S + 31) / 32) * (K2 / 128));
+    pf.push_back(pf.back() + ((S + 31) / 32) * ((K2 + kForceCols - 1) / kForceCols));
   }
   B.tot_proj = pp.back();
   B.tot_red = pr.back();
@@ -243,6 +251,8 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
         ContractArgs a = make_contract_args(c, B.prec, kModeFilter, f.W, f.npairs, cur, out,
                                             b.acc, c.nproj > 0 ? b.fnl : nullptr, b.chi, sp,
                                             nullptr);
+        a.Wnext = B.fused ? f.W : nullptr;
+        a.Xout32 = (B.prec == 1 && c.nproj > 0) ? b.X32 : nullptr;
         a.coeffs_dev = B.coeffs + (size_t)fi * L;
         a.step_dev = B.counter;
         co[v].push_back(a);
@@ -269,7 +279,7 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
     for (int v = 0; v < 2; ++v) {
       KbProjectArgs a;
       a.A = A;
-      a.X = X[v];
+      a.X = B.prec == 1 ? (const void*)b.X32 : (const void*)X[v];
       a.P = P;
       a.ns = S;
       a.K = K2;
@@ -297,7 +307,7 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
     q.K2 = K2;
     q.ppad = c.ppad;
     q.ts = (S + 31) / 32;
-    q.tk = K2 / 128;
+    q.tk = (K2 + kForceCols - 1) / kForceCols;
     q.F = b.fnl;
     fo.push_back(q);
   }
@@ -329,10 +339,11 @@ int enqueue_step(Batch& B, int variant, int slot) {
     if ((rc = launch_kb_batch(2, B.prec, B.force, B.pre_force, nnl, B.tot_force, snl))) return rc;
     if (B.overlap) SDFT_HIP_CHECK(hipEventRecord(B.ev_join[slot], snl));
   }
-  for (Group& g : B.groups)
-    if ((rc = launch_batch(0, g.n, B.prec, g.exp[xv], g.pre_exp, (int)g.frags.size(), g.tot_exp,
-                           0, 0, s)))
-      return rc;
+  if (variant == 0 || !B.fused)
+    for (Group& g : B.groups)
+      if ((rc = launch_batch(0, g.n, B.prec, g.exp[xv], g.pre_exp, (int)g.frags.size(),
+                             g.tot_exp, 0, 0, s)))
+        return rc;
   for (Group& g : B.groups) {
     const int v = (B.plane_variant == 1 && g.lds1 > 0) ? 1 : 0;
     if ((rc = launch_batch(1, g.n, B.prec, g.plane, g.pre_plane, (int)g.frags.size(),
@@ -342,7 +353,7 @@ int enqueue_step(Batch& B, int variant, int slot) {
   if (nl && B.overlap) SDFT_HIP_CHECK(hipStreamWaitEvent(s, B.ev_join[slot], 0));
   for (Group& g : B.groups)
     if ((rc = launch_batch(2, g.n, B.prec, g.con[variant], g.pre_exp, (int)g.frags.size(),
-                           g.tot_exp, 0, 0, s)))
+                           g.tot_exp, 0, B.fused ? 1 : 0, s)))
       return rc;
   hipLaunchKernelGGL(k_counter_advance, dim3(1), dim3(64), 0, s, B.counter);
   SDFT_HIP_CHECK(hipGetLastError());
@@ -467,8 +478,8 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
   hipError_t e = hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&B.stream_nl, hipStreamNonBlocking);
   for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-    e = hipEventCreateWithFlags(&B.ev_fork[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_join[i], hipEventDisableTiming);
+    e = hipEventCreateWithFlags(&B.ev_fork[i], kSyncEventFlags);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_join[i], kSyncEventFlags);
   }
   if (e != hipSuccess) return fail(hip_fail(e, "batch stream/event", __FILE__, __LINE__));
 
@@ -480,6 +491,7 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     f.b = &blocks[i]->b;
     f.c = f.b->ctx;
     f.npairs = f.b->nstate_pad / 2;
+    if (!f.c->g.col_intervals) B.fused = false;
     if ((rc = dmalloc(&f.W, w_bytes(*f.c, f.npairs, prec)))) return fail(rc);
     const int P = f.c->nproj, S = f.b->nstate_pad;
     if (P > 0) {
@@ -492,7 +504,8 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
       if ((rc = dmalloc(&f.Z, sizeof(double) * (size_t)P * S))) return fail(rc);
     }
     const long long nelem = (long long)S * 2 * f.c->g.ncp;
-    ca.push_back(CopyArgs{f.b->chi, f.b->X0, nelem});
+    ca.push_back(CopyArgs{f.b->chi, f.b->X0, nelem,
+                          (prec == 1 && f.c->nproj > 0) ? f.b->X32 : nullptr});
     pc.push_back(pc.back() + (int)((nelem + 255) / 256));
     // group by grid size
     const int n = f.c->g.n;
@@ -601,7 +614,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
   // Timed steps k >= 2 with c_k = 0 leave the accumulators as they are; a
   // tiny scale and zero shift keep the recurrence x_{k+1} = A x_k - x_{k-1}
   // bounded (|A| << 2), so T_k stays finite however many steps run.
-  const int L = steps + 3;
+  const int L = steps + 4;
   std::vector<double> co((size_t)nf * L, 0.0), sc(nf, 1e-4), em(nf, 0.0);
   hipEvent_t t0 = nullptr, t1 = nullptr;
   SDFT_HIP_CHECK(hipEventCreate(&t0));
@@ -620,8 +633,11 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
       B.plane_variant = pv;
       if ((rc = build_kb_tables(B))) break;
       if ((rc = build_args(B, L, co.data(), sc.data(), em.data()))) break;
-      if ((rc = set_counter(B, 2))) break;
-      if ((rc = enqueue_step(B, 1, 0))) break;  // warm-up
+      // warm-up: the first step (which expands chi) and one more, so that the
+      // timed steps start from the state a filter has at k = 3
+      if ((rc = set_counter(B, 1))) break;
+      if ((rc = enqueue_step(B, 0, 0))) break;
+      if ((rc = enqueue_step(B, 1, 1))) break;
       hipError_t e = hipEventRecord(t0, B.stream);
       for (int k = 0; k < steps && e == hipSuccess && !rc; ++k)
         rc = enqueue_step(B, (k & 1) ? 1 : 2, k & 1);

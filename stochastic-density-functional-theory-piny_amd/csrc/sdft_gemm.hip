@@ -174,52 +174,58 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
 
 // ---- F[s][k] = sum_p Y[s][p] At[k][p] ----------------------------------------
 // The reduction (over the padded projector index, ppad) is short, so it is
-// not split across waves: a workgroup covers 32 rows s x 128 columns k, wave w
-// owns columns 32w .. 32w + 31 of that block and runs the whole reduction
-// (two-stage pipeline as above), and the accumulators are stored directly --
-// no LDS, no barrier.  Both operands have their rows padded to multiples of 32
-// (Y) and 128 (At: 2 ncp) with readable memory.
-template <typename TA, typename TX, typename TI>
-__device__ __forceinline__ void kb_force_block(const TA* __restrict__ Y,
-                                               const TX* __restrict__ At, int ns, int K2,
-                                               int ppad, double* __restrict__ F, int bx, int by) {
-  typedef typename Mfma<TI>::acc_t acc_t;
+// not split across waves: a workgroup covers 32 rows s x 64 NJ columns k,
+// wave w owns columns 16 NJ w .. 16 NJ (w + 1) - 1 of that block (2 x NJ MFMA
+// blocks) and runs the whole reduction (two-stage pipeline as above), and the
+// accumulators are stored directly -- no LDS, no barrier.  K2 is a multiple
+// of 128, so a wave's columns lie either all below K2 or all above (that
+// wave returns at once).  NJ = 2 keeps the kernel's registers low enough for
+// it to run beside the plane kernel on the other stream (NJ = 4 reuses each
+// Y run twice as often but needs about twice the registers).  Y has its rows padded
+// to a multiple of 32 with readable memory.  All operands and F are in the
+// form's type T (float for FP32: the MFMA's own operand and accumulator type).
+constexpr int kForceNJ = kForceCols / 64;
+
+template <typename T>
+__device__ __forceinline__ void kb_force_block(const T* __restrict__ Y, const T* __restrict__ At,
+                                               int ns, int K2, int ppad, T* __restrict__ F,
+                                               int bx, int by) {
+  constexpr int NJ = kForceNJ;
+  typedef typename Mfma<T>::acc_t acc_t;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int s0 = bx * 32, k0 = by * 128 + wave * 32;
+  const int s0 = bx * 32, k0 = by * (64 * NJ) + wave * (16 * NJ);
+  if (k0 >= K2) return;
   const int g = lane >> 4, r16 = lane & 15;
-  acc_t acc[2][2];
+  acc_t acc[2][NJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = TI(0);
-  const TA* arow[2];
-  const TX* xrow[2];
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
+  const T* arow[2];
+  const T* xrow[NJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    arow[i] = Y + (size_t)(s0 + 16 * i + r16) * ppad + KPL * g;
-    xrow[i] = At + (size_t)(k0 + 16 * i + r16) * ppad + KPL * g;
-  }
-  TA ra0[2][KPL], ra1[2][KPL];
-  TX rb0[2][KPL], rb1[2][KPL];
-  auto load_raw = [&](int st, TA (&a)[2][KPL], TX (&b)[2][KPL]) {
+  for (int i = 0; i < 2; ++i) arow[i] = Y + (size_t)(s0 + 16 * i + r16) * ppad + KPL * g;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) xrow[j] = At + (size_t)(k0 + 16 * j + r16) * ppad + KPL * g;
+  T ra0[2][KPL], ra1[2][KPL];
+  T rb0[NJ][KPL], rb1[NJ][KPL];
+  auto load_raw = [&](int st, T (&a)[2][KPL], T (&b)[NJ][KPL]) {
     const int kk = st * KS;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      load_run<TA>(arow[i] + kk, a[i]);
-      load_run<TX>(xrow[i] + kk, b[i]);
-    }
+    for (int i = 0; i < 2; ++i) load_run<T>(arow[i] + kk, a[i]);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) load_run<T>(xrow[j] + kk, b[j]);
   };
-  auto mma_raw = [&](TA (&a)[2][KPL], TX (&b)[2][KPL]) {
+  auto mma_raw = [&](T (&a)[2][KPL], T (&b)[NJ][KPL]) {
 #pragma unroll
     for (int t = 0; t < KPL; ++t)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = Mfma<TI>::run(TI(a[i][t]), TI(b[j][t]), acc[i][j]);
+        for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::run(a[i][t], b[j][t], acc[i][j]);
   };
   const int nsteps = ppad / KS;
   int st = 0;
@@ -236,23 +242,22 @@ __device__ __forceinline__ void kb_force_block(const TA* __restrict__ Y,
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int s = s0 + 16 * i + Mfma<TI>::row(lane, r);
+      const int s = s0 + 16 * i + Mfma<T>::row(lane, r);
       if (s < ns) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) F[(size_t)s * K2 + k0 + 16 * j + r16] = double(acc[i][j][r]);
+        for (int j = 0; j < NJ; ++j) F[(size_t)s * K2 + k0 + 16 * j + r16] = acc[i][j][r];
       }
     }
 }
 
-// 1-D launch over ts * (K2 / 128) blocks, XCD-swizzled, s fastest: the
-// blocks sharing one 128-column slab of At sit on one XCD.
-template <typename TA, typename TX, typename TI>
-__global__ __launch_bounds__(256) void k_kb_force(const TA* __restrict__ Y,
-                                                  const TX* __restrict__ At, int ns, int K2,
-                                                  int ppad, double* __restrict__ F, int ts,
-                                                  int total) {
+// 1-D launch over ts * ceil(K2 / kForceCols) blocks, XCD-swizzled, s fastest:
+// the blocks sharing one slab of At sit on one XCD.
+template <typename T>
+__global__ __launch_bounds__(256) void k_kb_force(const T* __restrict__ Y, const T* __restrict__ At,
+                                                  int ns, int K2, int ppad, T* __restrict__ F,
+                                                  int ts, int total) {
   const int L = xcd_swizzle(blockIdx.x, total);
-  kb_force_block<TA, TX, TI>(Y, At, ns, K2, ppad, F, L % ts, L / ts);
+  kb_force_block<T>(Y, At, ns, K2, ppad, F, L % ts, L / ts);
 }
 
 // 1-D launch over tp * ts * nsplit tiles.  Logical tile L (XCD-swizzled) is
@@ -283,45 +288,45 @@ __global__ __launch_bounds__(256) void k_kb_project_batch(const KbProjectArgs* _
   const KbProjectArgs a = args[f];
   const int local = L - prefix[f];
   const int bx = local % a.tp, r = local / a.tp;
-  kb_project_tile<TA, double, TI>(reinterpret_cast<const TA*>(a.A), a.X, a.P, a.ns, a.K, a.per,
-                                  a.zpart, bx, r % a.ts, r / a.ts, red);
+  kb_project_tile<TA, TA, TI>(reinterpret_cast<const TA*>(a.A), reinterpret_cast<const TA*>(a.X),
+                              a.P, a.ns, a.K, a.per, a.zpart, bx, r % a.ts, r / a.ts, red);
 }
 
 // Z[p][s] = sum_z part[z][p][s];  Y[s][p] = -2 E_p Z[p][s] (transposed, row
-// stride ppad; the padding columns p >= P are never written and stay zero)
+// stride ppad, in the form's type TY; the padding columns p >= P are never
+// written and stay zero)
+template <typename TY>
 __device__ __forceinline__ void nl_reduce_elem(const double* __restrict__ part, int nsplit, int P,
                                                int ns, int ppad, const double* __restrict__ E,
-                                               double* __restrict__ Z, double* __restrict__ Y,
-                                               int i) {
+                                               double* __restrict__ Z, TY* __restrict__ Y, int i) {
   if (i >= P * ns) return;
   double s = 0.0;
   for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * P * ns + i];
   const int p = i / ns, st = i - p * ns;
   Z[i] = s;
-  if (Y) Y[(size_t)st * ppad + p] = -2.0 * E[p] * s;
+  if (Y) Y[(size_t)st * ppad + p] = TY(-2.0 * E[p] * s);
 }
 
+template <typename TY>
 __global__ void k_nl_reduce(const double* __restrict__ part, int nsplit, int P, int ns, int ppad,
                             const double* __restrict__ E, double* __restrict__ Z,
-                            double* __restrict__ Y) {
-  nl_reduce_elem(part, nsplit, P, ns, ppad, E, Z, Y, blockIdx.x * blockDim.x + threadIdx.x);
+                            TY* __restrict__ Y) {
+  nl_reduce_elem<TY>(part, nsplit, P, ns, ppad, E, Z, Y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+template <typename TY>
 __global__ __launch_bounds__(256) void k_nl_reduce_batch(const KbReduceArgs* __restrict__ args,
                                                          const int* __restrict__ prefix,
                                                          int nfrag) {
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const KbReduceArgs a = args[f];
-  nl_reduce_elem(a.zpart, a.nsplit, a.P, a.ns, a.ppad, a.E, a.Z, a.Y,
-                 (blockIdx.x - prefix[f]) * 256 + threadIdx.x);
+  nl_reduce_elem<TY>(a.zpart, a.nsplit, a.P, a.ns, a.ppad, a.E, a.Z, reinterpret_cast<TY*>(a.Y),
+                     (blockIdx.x - prefix[f]) * 256 + threadIdx.x);
 }
 
-// ---- F[s][k] = sum_p Y[s][p] At[k][p] --------------------------------------
-// The same tile as the projection with Y (states x ppad) as the row operand
-// and At (coefficient rows x ppad) as the column operand: both are read in
-// runs of 8 consecutive projectors per lane, and the K dimension (ppad, a
-// multiple of 32) is not split, so the tile writes F directly.
-template <typename TA, typename TI>
+// ---- batched F[s][k] = sum_p Y[s][p] At[k][p] ------------------------------
+// kb_force_block per fragment: ts x tk blocks of 32 states x kForceCols columns.
+template <typename T>
 __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __restrict__ args,
                                                         const int* __restrict__ prefix,
                                                         int nfrag) {
@@ -329,8 +334,8 @@ __global__ __launch_bounds__(256) void k_kb_force_batch(const KbForceArgs* __res
   const int f = find_frag(prefix, nfrag, L);
   const KbForceArgs a = args[f];
   const int local = L - prefix[f];
-  kb_force_block<double, TA, TI>(a.Y, reinterpret_cast<const TA*>(a.At), a.ns, a.K2, a.ppad, a.F,
-                                 local % a.ts, local / a.ts);
+  kb_force_block<T>(reinterpret_cast<const T*>(a.Y), reinterpret_cast<const T*>(a.At), a.ns, a.K2,
+                    a.ppad, reinterpret_cast<T*>(a.F), local % a.ts, local / a.ts);
 }
 
 static int ensure(void** p, size_t* have, size_t need) {
@@ -362,7 +367,7 @@ int kb_project_split(const Ctx& c, int ns, int* per) {
   return (nsteps + *per - 1) / *per;
 }
 
-int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
+int launch_nl_project(Ctx& c, int prec, const void* X, int ns) {
   const int P = c.nproj;
   const int K = 2 * c.g.ncp;
   int per = 0;
@@ -375,29 +380,34 @@ int launch_nl_project(Ctx& c, int prec, const double* X, int ns) {
   const size_t ybytes = sizeof(double) * (size_t)c.ppad * ((ns + 31) / 32 * 32);
   if (c.Y_bytes < ybytes || c.Y_ppad != c.ppad) {
     if (c.Y) SDFT_HIP_CHECK(hipFree(c.Y));
+    if (c.Y32) SDFT_HIP_CHECK(hipFree(c.Y32));
     if (c.Z) SDFT_HIP_CHECK(hipFree(c.Z));
     c.Y = c.Z = nullptr;
+    c.Y32 = nullptr;
     c.Y_bytes = 0;
-    if (hipMalloc(&c.Y, ybytes) != hipSuccess ||
+    if (hipMalloc(&c.Y, ybytes) != hipSuccess || hipMalloc(&c.Y32, ybytes / 2) != hipSuccess ||
         hipMalloc(&c.Z, sizeof(double) * (size_t)c.ppad * ns) != hipSuccess) {
       set_error("device allocation failed (KB projections)");
       return -3;
     }
     SDFT_HIP_CHECK(hipMemsetAsync(c.Y, 0, ybytes, c.stream));
+    SDFT_HIP_CHECK(hipMemsetAsync(c.Y32, 0, ybytes / 2, c.stream));
     c.Y_bytes = ybytes;
     c.Y_ppad = c.ppad;
   }
   const int tp = (P + 31) / 32, ts = (ns + 31) / 32, tot = tp * ts * nsplit;
-  if (prec == 0)
-    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
-                       c.A64, X, P, ns, K, per, c.zpart, tp, ts, tot);
-  else
-    hipLaunchKernelGGL((k_kb_project<float, double, float>), dim3(tot), dim3(256), 0, c.stream,
-                       c.A32, X, P, ns, K, per, c.zpart, tp, ts, tot);
-  SDFT_HIP_CHECK(hipGetLastError());
   const int nel = P * ns;
-  hipLaunchKernelGGL(k_nl_reduce, dim3((nel + 255) / 256), dim3(256), 0, c.stream, c.zpart, nsplit,
-                     P, ns, c.ppad, c.E, c.Z, c.Y);
+  if (prec == 0) {
+    hipLaunchKernelGGL((k_kb_project<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
+                       c.A64, static_cast<const double*>(X), P, ns, K, per, c.zpart, tp, ts, tot);
+    hipLaunchKernelGGL(k_nl_reduce<double>, dim3((nel + 255) / 256), dim3(256), 0, c.stream,
+                       c.zpart, nsplit, P, ns, c.ppad, c.E, c.Z, c.Y);
+  } else {
+    hipLaunchKernelGGL((k_kb_project<float, float, float>), dim3(tot), dim3(256), 0, c.stream,
+                       c.A32, static_cast<const float*>(X), P, ns, K, per, c.zpart, tp, ts, tot);
+    hipLaunchKernelGGL(k_nl_reduce<float>, dim3((nel + 255) / 256), dim3(256), 0, c.stream,
+                       c.zpart, nsplit, P, ns, c.ppad, c.E, c.Z, c.Y32);
+  }
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -458,8 +468,8 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
                        c.stream, Ad, X, P, ns, K, per, part, tp, ts, ntile);
     // reduce the split partials into Zd[d] (the E factor is applied later)
     const int n2 = P * ns;
-    hipLaunchKernelGGL(k_nl_reduce, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, part, nsplit,
-                       P, ns, c.ppad, c.E, Zd + (size_t)d * n2, (double*)nullptr);
+    hipLaunchKernelGGL(k_nl_reduce<double>, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, part,
+                       nsplit, P, ns, c.ppad, c.E, Zd + (size_t)d * n2, (double*)nullptr);
     SDFT_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_force_sums, dim3(P), dim3(256), 0, c.stream, c.Z, Zd, P, ns, nstate, S);
@@ -467,16 +477,17 @@ int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gin
   return 0;
 }
 
-// F[s][k] = sum_p Y[s][p] At[k][p] (K2 = 2 ncp is a multiple of 128)
-int launch_nl_force(Ctx& c, int prec, int ns, double* fnl) {
+// F[s][k] = sum_p Y[s][p] At[k][p] (K2 = 2 ncp is a multiple of 128), F in
+// the form's type
+int launch_nl_force(Ctx& c, int prec, int ns, void* fnl) {
   const int K2 = 2 * c.g.ncp;
-  const int ts = (ns + 31) / 32, tot = ts * (K2 / 128);
+  const int ts = (ns + 31) / 32, tot = ts * ((K2 + kForceCols - 1) / kForceCols);
   if (prec == 0)
-    hipLaunchKernelGGL((k_kb_force<double, double, double>), dim3(tot), dim3(256), 0, c.stream,
-                       c.Y, c.At64, ns, K2, c.ppad, fnl, ts, tot);
+    hipLaunchKernelGGL((k_kb_force<double>), dim3(tot), dim3(256), 0, c.stream, c.Y, c.At64, ns,
+                       K2, c.ppad, static_cast<double*>(fnl), ts, tot);
   else
-    hipLaunchKernelGGL((k_kb_force<double, float, float>), dim3(tot), dim3(256), 0, c.stream, c.Y,
-                       c.At32, ns, K2, c.ppad, fnl, ts, tot);
+    hipLaunchKernelGGL((k_kb_force<float>), dim3(tot), dim3(256), 0, c.stream, c.Y32, c.At32, ns,
+                       K2, c.ppad, static_cast<float*>(fnl), ts, tot);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -493,14 +504,18 @@ int launch_kb_batch(int kind, int prec, const void* args, const int* prefix, int
       hipLaunchKernelGGL((k_kb_project_batch<float, float>), dim3(total), dim3(256), 0, s,
                          reinterpret_cast<const KbProjectArgs*>(args), prefix, nfrag);
   } else if (kind == 1) {
-    hipLaunchKernelGGL(k_nl_reduce_batch, dim3(total), dim3(256), 0, s,
-                       reinterpret_cast<const KbReduceArgs*>(args), prefix, nfrag);
+    if (prec == 0)
+      hipLaunchKernelGGL(k_nl_reduce_batch<double>, dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbReduceArgs*>(args), prefix, nfrag);
+    else
+      hipLaunchKernelGGL(k_nl_reduce_batch<float>, dim3(total), dim3(256), 0, s,
+                         reinterpret_cast<const KbReduceArgs*>(args), prefix, nfrag);
   } else {
     if (prec == 0)
-      hipLaunchKernelGGL((k_kb_force_batch<double, double>), dim3(total), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_kb_force_batch<double>), dim3(total), dim3(256), 0, s,
                          reinterpret_cast<const KbForceArgs*>(args), prefix, nfrag);
     else
-      hipLaunchKernelGGL((k_kb_force_batch<float, float>), dim3(total), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_kb_force_batch<float>), dim3(total), dim3(256), 0, s,
                          reinterpret_cast<const KbForceArgs*>(args), prefix, nfrag);
   }
   SDFT_HIP_CHECK(hipGetLastError());

@@ -48,6 +48,9 @@ struct Geometry {
   int ncolp = 0;      // padded column stride of W
   int nchunk = 0;     // column chunks (column-pair aligned)
   int chunk_max = 0;  // max columns in a chunk
+  // every column's ka set is one interval [-h, h] (h from its entry count):
+  // the contract kernel can then re-expand its own columns for the next step
+  bool col_intervals = false;
   // host copies
   std::vector<int> perm;        // internal j -> caller index
   std::vector<int> iperm;       // caller index -> internal j
@@ -68,6 +71,12 @@ struct DeviceTables {
   void* tw64 = nullptr;       // [n] double2 exp(+2 pi i m/n)
   void* tw32 = nullptr;       // [n] float2
 };
+
+// Events that only order work between the library's own streams on one
+// device: no timing, and no system-scope fence (nothing is handed to the
+// host or another device through them), so a fork or join costs no
+// system-wide cache writeback.
+constexpr unsigned kSyncEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
 
 struct Ctx {
   int device = 0;
@@ -101,6 +110,7 @@ struct Ctx {
   double* zpart = nullptr;   // split-K partials of Z
   size_t zpart_bytes = 0;
   double* Y = nullptr;       // [nstate_pad][ppad] = -2 E Z, transposed, zero padded
+  float* Y32 = nullptr;      // the same for the FP32 form (its force operand)
   double* Z = nullptr;       // [nproj][nstate_pad]
   size_t Y_bytes = 0;
   int Y_ppad = 0;
@@ -110,6 +120,9 @@ struct Ctx {
   // base sub-chunk, 1 double) and the KB projection's target workgroup count
   int plane_variant = 0;
   int kb_wg_target = 512;
+  // filter loops let the contract kernel write the next step's a-expansion
+  // (when the columns allow it, Geometry::col_intervals); the tuner decides
+  bool fuse_expand = true;
   // Newton-form accumulators for the contract kernel (set for the duration
   // of a Newton filter call only)
   int newton_nmu = 0;
@@ -131,7 +144,12 @@ struct Block {
   double* X1 = nullptr;    // T pong
   double* acc = nullptr;   // filtered accumulator
   double* chi = nullptr;   // copy of the input noise (moments)
-  double* fnl = nullptr;   // NL raw force of the current step
+  // NL raw force of the current step, stored in the form's precision
+  // (double for FP64, float for FP32: the MFMA accumulates in that type)
+  void* fnl = nullptr;
+  // FP32 form: float copy of the T buffer the KB projection reads (its MFMA
+  // operand), rows padded like X0 / X1
+  float* X32 = nullptr;
   double* mom_part = nullptr;  // per-step moment partials
   size_t mom_part_bytes = 0;
   int cur = 0;             // which of X0/X1 holds the current T
@@ -186,7 +204,7 @@ struct ContractArgs {
   const double* Xcur;
   double* Xout;
   double* acc;
-  const double* fnl;
+  const void* fnl;  // in the kernel's precision T
   const double* chi;
   int ldx, ncp, ncoef;
   double invN;
@@ -202,6 +220,15 @@ struct ContractArgs {
   int nmu;
   long long acc_stride;
   const double* newton_c;
+  // when non-null (filter / moments / Newton modes), the kernel also packs
+  // the new vector Xout of its column pairs into its LDS columns, inverse
+  // transforms them along a and writes them to Wnext: the expand of the
+  // next step (needs Geometry::col_intervals)
+  void* Wnext;
+  const int* ent_off;
+  // FP32 form: when non-null, the new vector is also stored here in float
+  // (the next KB projection's operand)
+  float* Xout32;
   int gx, gy;  // column chunks, pairs
 };
 
@@ -218,10 +245,10 @@ enum ContractMode {
 
 // KB GEMM arguments of one fragment of a batch (tile counts: tp over
 // projectors, ts over states (32 each); tk over the 2 ncp coefficient rows in
-// 128-row blocks).
+// kForceCols-row blocks).
 struct KbProjectArgs {
   const void* A;  // [P][K] (double or float)
-  const double* X;
+  const void* X;  // [ns][K] in the same type
   int P, ns, K, per, nsplit, tp, ts;
   double* zpart;
 };
@@ -231,14 +258,18 @@ struct KbReduceArgs {
   int nsplit, P, ns, ppad;
   const double* E;
   double* Z;
-  double* Y;  // [ns][ppad]
+  void* Y;  // [ns][ppad], double (FP64 form) or float (FP32 form)
 };
 
+// columns k of F per workgroup of the force product (four waves of
+// kForceCols / 4)
+constexpr int kForceCols = 128;
+
 struct KbForceArgs {
-  const double* Y;  // [ns][ppad]
-  const void* At;   // [K2][ppad] (double or float)
+  const void* Y;   // [ns][ppad]   (all three in the form's type)
+  const void* At;  // [K2][ppad]
   int ns, K2, ppad, ts, tk;
-  double* F;
+  void* F;         // [ns][K2]
 };
 
 // Workgroups of a launch are dispatched to the eight XCDs round-robin (XCD =
@@ -275,8 +306,9 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W);
 // buffer above the LDS size)
 int launch_plane(Ctx& c, int prec, void* W, int npairs, void** Wout);
 int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
-                    const double* Xcur, double* Xout, double* acc, const double* fnl,
-                    const double* chi, const StepParams& sp, double* mom_part);
+                    const double* Xcur, double* Xout, double* acc, const void* fnl,
+                    const double* chi, const StepParams& sp, double* mom_part,
+                    void* Wnext = nullptr, float* Xout32 = nullptr);
 int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups);
 int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                        int nstate);
@@ -290,20 +322,21 @@ int selftest_fft_dispatch(int n, int nlines, int dir, int prec, const double* in
 ExpandArgs make_expand_args(const Ctx& c, int prec, const double* X, void* W);
 PlaneArgs make_plane_args(const Ctx& c, int prec, void* W);
 ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W, int npairs,
-                                const double* Xcur, double* Xout, double* acc, const double* fnl,
+                                const double* Xcur, double* Xout, double* acc, const void* fnl,
                                 const double* chi, const StepParams& sp, double* mom_part);
 size_t plane_lds_for(int n, int prec, int M, int variant);
 // one launch of kind 0 (expand), 1 (plane, tile variant `variant`) or 2
-// (contract) over the fragments whose argument blocks are args[0..nfrag)
-// (device), all of grid size n
+// (contract; variant 1: with the fused expand) over the fragments whose
+// argument blocks are args[0..nfrag) (device), all of grid size n
 int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
                  int total, size_t lds, int variant, hipStream_t s);
 int chunk_cols_for(int n);
 int round_fft(int n_in);
 
 // ---- GEMM (sdft_gemm.hip) -----------------------------------------------
-int launch_nl_project(Ctx& c, int prec, const double* X, int nstate_pad);
-int launch_nl_force(Ctx& c, int prec, int nstate_pad, double* fnl);
+// X / fnl in the form's type (double for prec 0, float for prec 1)
+int launch_nl_project(Ctx& c, int prec, const void* X, int nstate_pad);
+int launch_nl_force(Ctx& c, int prec, int nstate_pad, void* fnl);
 int kb_project_split(const Ctx& c, int ns, int* per);
 int launch_kb_batch(int kind, int prec, const void* args, const int* prefix, int nfrag,
                     int total, hipStream_t s);

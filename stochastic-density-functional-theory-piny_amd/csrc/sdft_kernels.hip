@@ -472,13 +472,16 @@ __global__ __launch_bounds__(kThreads) void k_noise_plane(Cx<double>* __restrict
 }
 
 // ---- K3: forward FFT along a + unpack + force / recurrence ---------------
-template <typename T, int N>
+// E = true compiles the fused expand (a.Wnext); E = false leaves it out, so
+// the kernel without it keeps its lower register count.
+template <typename T, int N, bool E>
 __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, int q) {
   constexpr int CH = Cfg<N>::CH, LD = Cfg<N>::LD, ML = Cfg<N>::ML;
   __shared__ Cx<T> buf[CH * LD];
   __shared__ Cx<T> tw[N];
   __shared__ double red[3 * (kThreads / 64)];
-  const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
+  // W may be the buffer Wnext writes (the fused expand): no __restrict__
+  const Cx<T>* W = reinterpret_cast<const Cx<T>*>(a.W);
   const int ncolp = a.ncolp;
   const int* __restrict__ chunk_col = a.chunk_col;
   const int* __restrict__ chunk_coef = a.chunk_coef;
@@ -489,7 +492,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
   const double* __restrict__ Xcur = a.Xcur;
   double* __restrict__ Xout = a.Xout;
   double* __restrict__ acc = a.acc;
-  const double* __restrict__ fnl = a.fnl;
+  const T* __restrict__ fnl = reinterpret_cast<const T*>(a.fnl);
   const double* __restrict__ chi = a.chi;
   const int ldx = a.ldx, ncp = a.ncp, ncoef = a.ncoef, mode = a.mode;
   const double invN = a.invN;
@@ -514,11 +517,16 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
   fft_lines<T, N, -1, ML, kThreads>(buf, nc, LD, tw, tid);
 
   double dots[3] = {0.0, 0.0, 0.0};
+  Cx<T>* Wn = reinterpret_cast<Cx<T>*>(a.Wnext);
+  const bool emit =
+      E && Wn != nullptr && (mode == kModeFilter || mode == kModeMoments || mode == kModeNewton);
   const int j0 = chunk_coef[chunk], j1 = chunk_coef[chunk + 1];
   for (int j = j0 + tid; j < j1; j += kThreads) {
     const int pp = coef_plus[j], pm = coef_minus[j];
-    const Cx<T> t = buf[((pp >> 8) - c0) * LD + (pp & 255)];
-    const Cx<T> tc = buf[((pm >> 8) - c0) * LD + (pm & 255)];
+    Cx<T>* bp = &buf[((pp >> 8) - c0) * LD + (pp & 255)];
+    Cx<T>* bm = &buf[((pm >> 8) - c0) * LD + (pm & 255)];
+    const Cx<T> t = *bp;
+    const Cx<T> tc = *bm;
     const double tr = double(t.x) * invN, ti = double(t.y) * invN;
     const double tcr = double(tc.x) * invN, tci = double(tc.y) * invN;
     const bool g0 = (j == ncoef - 1);
@@ -549,14 +557,15 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
       }
       continue;
     }
+    double nv[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
     for (int o = 0; o < 2; ++o) {
       const size_t base = (size_t)(2 * q + o) * ldx;
       const double cr = Xcur[base + j], ci = Xcur[base + ncp + j];
       double fr = f[o][0], fi = f[o][1];
       if (fnl) {
-        fr += fnl[base + j];
-        fi += fnl[base + ncp + j];
+        fr += double(fnl[base + j]);
+        fi += double(fnl[base + ncp + j]);
       }
       if (!g0) {
         fr -= 2.0 * k2 * cr;
@@ -569,10 +578,11 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
         Xout[base + ncp + j] = fi;
         continue;
       }
+      double nr, ni;
       if (mode == kModeNewton) {
         // normHNewtonHerm (normh.c:101-150): scale the force, add prefact c
-        const double nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
-        const double ni = g0 ? fi + sp.p * ci : sp.s1 * fi + sp.p * ci;
+        nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
+        ni = g0 ? fi + sp.p * ci : sp.s1 * fi + sp.p * ci;
         Xout[base + j] = nr;
         Xout[base + ncp + j] = ni;
         for (int mu = 0; mu < a.nmu; ++mu) {
@@ -587,39 +597,55 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
             am[base + ncp + j] += c1 * ni;
           }
         }
-        continue;
-      }
-      double nr, ni;
-      if (sp.first) {
-        nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
-        ni = g0 ? fi : sp.s1 * fi + sp.p * ci;
       } else {
-        const double pr = Xout[base + j], pi = Xout[base + ncp + j];
-        nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr - pr;
-        ni = g0 ? fi : sp.s1 * fi + sp.p * ci - pi;
-      }
-      Xout[base + j] = nr;
-      Xout[base + ncp + j] = ni;
-      if (mode == kModeFilter) {
         if (sp.first) {
-          acc[base + j] = sp.c0 * cr + sp.ck * nr;
-          acc[base + ncp + j] = sp.c0 * ci + sp.ck * ni;
+          nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr;
+          ni = g0 ? fi : sp.s1 * fi + sp.p * ci;
         } else {
-          acc[base + j] += sp.ck * nr;
-          acc[base + ncp + j] += sp.ck * ni;
+          const double pr = Xout[base + j], pi = Xout[base + ncp + j];
+          nr = (g0 ? sp.s2 : sp.s1) * fr + sp.p * cr - pr;
+          ni = g0 ? fi : sp.s1 * fi + sp.p * ci - pi;
+        }
+        Xout[base + j] = nr;
+        Xout[base + ncp + j] = ni;
+        if (mode == kModeFilter) {
+          if (sp.first) {
+            acc[base + j] = sp.c0 * cr + sp.ck * nr;
+            acc[base + ncp + j] = sp.c0 * ci + sp.ck * ni;
+          } else {
+            acc[base + j] += sp.ck * nr;
+            acc[base + ncp + j] += sp.ck * ni;
+          }
+        }
+        if (mom_part) {
+          // Gamma dot as coded at calc-chempot-chebyshev.c:725-729:
+          // 2 sum re*re + 2 sum im*im - re(0)*re(0)
+          const double wr = g0 ? 1.0 : 2.0;
+          const double xr = chi[base + j], xi = chi[base + ncp + j];
+          dots[0] += wr * xr * nr + 2.0 * xi * ni;
+          if (sp.first)
+            dots[1] += wr * cr * cr + 2.0 * ci * ci;
+          else
+            dots[1] += wr * nr * cr + 2.0 * ni * ci;
+          dots[2] += wr * nr * nr + 2.0 * ni * ni;
         }
       }
-      if (mom_part) {
-        // Gamma dot as coded at calc-chempot-chebyshev.c:725-729:
-        // 2 sum re*re + 2 sum im*im - re(0)*re(0)
-        const double wr = g0 ? 1.0 : 2.0;
-        const double xr = chi[base + j], xi = chi[base + ncp + j];
-        dots[0] += wr * xr * nr + 2.0 * xi * ni;
-        if (sp.first)
-          dots[1] += wr * cr * cr + 2.0 * ci * ci;
-        else
-          dots[1] += wr * nr * cr + 2.0 * ni * ci;
-        dots[2] += wr * nr * nr + 2.0 * ni * ni;
+      nv[o][0] = nr;
+      nv[o][1] = ni;
+      if (a.Xout32) {
+        a.Xout32[base + j] = float(nr);
+        a.Xout32[base + ncp + j] = float(ni);
+      }
+    }
+    if (emit) {
+      // the Gamma pair pack of the new pair (as in k_expand) into the two
+      // positions this coefficient was read from: no other thread touches them
+      const double ar = nv[0][0], ai = nv[0][1], br = nv[1][0], bi = nv[1][1];
+      if (g0) {
+        *bp = cx(T(ar), T(br));
+      } else {
+        *bp = cx(T(ar - bi), T(ai + br));
+        *bm = cx(T(ar + bi), T(br - ai));
       }
     }
   }
@@ -632,21 +658,41 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
       mom_part[row * 3 + 2] = dots[2];
     }
   }
+  if (emit) {
+    // positions outside the sphere (|ka| > h of the column, h from its entry
+    // count) become zero: disjoint from the positions written above
+    const int* __restrict__ ent_off = a.ent_off;
+    const Cx<T> zero = cx(T(0), T(0));
+    for (int e = tid; e < nc * N; e += kThreads) {
+      const int dl = e / N, ia = e - dl * N;
+      const int h = (ent_off[c0 + dl + 1] - ent_off[c0 + dl] - 1) >> 1;
+      const int ka = ia <= N / 2 ? ia : ia - N;
+      if (ka > h || ka < -h) buf[dl * LD + ia] = zero;
+    }
+    __syncthreads();
+    fft_lines<T, N, +1, ML, kThreads>(buf, nc, LD, tw, tid);
+    Cx<T>* Wq2 = Wn + (size_t)q * N * ncolp;
+    for (int e = tid; e < nc * N; e += kThreads) {
+      const int ra = e / nc;
+      const int dl = e - ra * nc;
+      Wq2[(size_t)ra * ncolp + c0 + dl] = buf[dl * LD + ra];
+    }
+  }
 }
 
-template <typename T, int N>
+template <typename T, int N, bool E>
 __global__ __launch_bounds__(kThreads) void k_contract(ContractArgs a) {
-  contract_body<T, N>(a, blockIdx.x, blockIdx.y);
+  contract_body<T, N, E>(a, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int N>
+template <typename T, int N, bool E>
 __global__ __launch_bounds__(kThreads) void k_contract_batch(const ContractArgs* __restrict__ args,
                                                              const int* __restrict__ prefix,
                                                              int nfrag) {
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const ContractArgs a = args[f];
   const int local = blockIdx.x - prefix[f];
-  contract_body<T, N>(a, local % a.gx, local / a.gx);
+  contract_body<T, N, E>(a, local % a.gx, local / a.gx);
 }
 
 // ---- density: |psi1|^2 + |psi2|^2 summed over the pairs of a group -------
@@ -860,7 +906,7 @@ PlaneArgs make_plane_args(const Ctx& c, int prec, void* W) {
 }
 
 ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W, int npairs,
-                                const double* Xcur, double* Xout, double* acc, const double* fnl,
+                                const double* Xcur, double* Xout, double* acc, const void* fnl,
                                 const double* chi, const StepParams& sp, double* mom_part) {
   ContractArgs a;
   a.W = W;
@@ -888,6 +934,9 @@ ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W,
   a.nmu = c.newton_nmu;
   a.acc_stride = c.newton_stride;
   a.newton_c = c.newton_c;
+  a.Wnext = nullptr;
+  a.ent_off = c.t.ent_off;
+  a.Xout32 = nullptr;
   a.gx = c.g.nchunk;
   a.gy = npairs;
   return a;
@@ -926,8 +975,11 @@ static int batch_launch_impl(int kind, const void* args, const int* prefix, int 
     else
       hipLaunchKernelGGL((k_plane_batch<T, N, 0>), dim3(total), dim3(kThreads), lds, s,
                          reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+  } else if (variant == 1) {  // contract with the fused expand
+    hipLaunchKernelGGL((k_contract_batch<T, N, true>), dim3(total), dim3(kThreads), 0, s,
+                       reinterpret_cast<const ContractArgs*>(args), prefix, nfrag);
   } else {
-    hipLaunchKernelGGL((k_contract_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((k_contract_batch<T, N, false>), dim3(total), dim3(kThreads), 0, s,
                        reinterpret_cast<const ContractArgs*>(args), prefix, nfrag);
   }
   SDFT_HIP_CHECK(hipGetLastError());
@@ -1030,26 +1082,34 @@ int launch_plane(Ctx& c, int prec, void* W, int npairs, void** Wout) {
 
 template <typename T, int N>
 static int contract_impl(Ctx& c, int mode, const void* W, int npairs, const double* Xcur,
-                         double* Xout, double* acc, const double* fnl, const double* chi,
-                         const StepParams& sp, double* mom_part, const void* tw) {
-  (void)tw;
-  const ContractArgs a = make_contract_args(c, prec_of<T>(), mode, W, npairs, Xcur, Xout, acc,
-                                            fnl, chi, sp, mom_part);
+                         double* Xout, double* acc, const void* fnl, const double* chi,
+                         const StepParams& sp, double* mom_part, void* Wnext, float* Xout32) {
+  ContractArgs a = make_contract_args(c, prec_of<T>(), mode, W, npairs, Xcur, Xout, acc, fnl,
+                                      chi, sp, mom_part);
+  a.Wnext = Wnext;
+  a.Xout32 = Xout32;
   dim3 grid(a.gx, npairs);
-  hipLaunchKernelGGL((k_contract<T, N>), grid, dim3(kThreads), 0, c.stream, a);
+  if (Wnext)
+    hipLaunchKernelGGL((k_contract<T, N, true>), grid, dim3(kThreads), 0, c.stream, a);
+  else
+    hipLaunchKernelGGL((k_contract<T, N, false>), grid, dim3(kThreads), 0, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
 int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs, const double* Xcur,
-                    double* Xout, double* acc, const double* fnl, const double* chi,
-                    const StepParams& sp, double* mom_part) {
+                    double* Xout, double* acc, const void* fnl, const double* chi,
+                    const StepParams& sp, double* mom_part, void* Wnext, float* Xout32) {
+  if (Wnext && !c.g.col_intervals) {
+    set_error("internal error: fused expand needs interval columns");
+    return SDFT_EINVAL_CODE;
+  }
   if (prec == 0) {
     SDFT_SIZE_SWITCH(c.g.n, return (contract_impl<double, NN>(c, mode, W, npairs, Xcur, Xout, acc,
-                                                              fnl, chi, sp, mom_part, c.t.tw64)));
+                                                              fnl, chi, sp, mom_part, Wnext, Xout32)));
   } else {
     SDFT_SIZE_SWITCH(c.g.n, return (contract_impl<float, NN>(c, mode, W, npairs, Xcur, Xout, acc,
-                                                             fnl, chi, sp, mom_part, c.t.tw32)));
+                                                             fnl, chi, sp, mom_part, Wnext, Xout32)));
   }
   return 0;
 }

@@ -101,7 +101,7 @@ def load(path: str | None = None):
         "sdft_hip_newton_filter": ([vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
                                     C.c_double, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
         "sdft_hip_tune": ([vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _dp], C.c_int),
-        "sdft_hip_set_config": ([vp, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_set_config": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -206,17 +206,18 @@ class Context:
         return fre + 1j * fim
 
     def tune(self, nstate, prec=PREC_FP64, steps=6):
-        """sdft_hip_tune: returns ((plane_tile, kb_wg_target), per-candidate ms)."""
-        cfg = np.zeros(2, dtype=np.int32)
+        """sdft_hip_tune: returns ((plane_tile, kb_wg_target, fused_expand),
+        per-candidate ms per filter step)."""
+        cfg = np.zeros(3, dtype=np.int32)
         ms = np.zeros(16)
         n = self.lib.sdft_hip_tune(self.h, nstate, prec, steps, cfg.ctypes.data_as(_ip), 16,
                                    _ptr(ms))
         if n < 0:
             _check(n)
-        return (int(cfg[0]), int(cfg[1])), ms[:n].copy()
+        return (int(cfg[0]), int(cfg[1]), int(cfg[2])), ms[:n].copy()
 
-    def set_config(self, plane_tile, kb_wg_target=512):
-        _check(self.lib.sdft_hip_set_config(self.h, plane_tile, kb_wg_target))
+    def set_config(self, plane_tile, kb_wg_target=512, fused_expand=1):
+        _check(self.lib.sdft_hip_set_config(self.h, plane_tile, kb_wg_target, fused_expand))
 
     def newton_filter(self, chi, coeffs, samp_point, scale, emean, prec=PREC_FP64):
         """filterNewtonPolyHerm: coeffs (L, nmu), samp_point (L-1,).  Returns

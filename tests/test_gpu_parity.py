@@ -324,7 +324,30 @@ def test_plane_tiles_and_tuner_si64(hip, si64_system):
             ctx.set_config(tile, kt)
             assert rel(ctx.hpsi_raw(C), ref) < 1e-12, (tile, kt)
             assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5, (tile, kt)
-    (tile, kt), ms = ctx.tune(8, hip.PREC_FP32, steps=2)
-    assert tile in (0, 1) and kt in (256, 512, 1024, 2048)
+    (tile, kt, fu), ms = ctx.tune(8, hip.PREC_FP32, steps=2)
+    assert tile in (0, 1) and kt in (256, 1024) and fu in (0, 1)
     assert len(ms) == 8 and np.all(ms > 0)
     assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-4)])
+def test_filter_fused_and_separate_expand(hip, si8_system, si8_ctx, prec, tol):
+    """The filter with the contract kernel writing the next step's
+    a-expansion (fused) and with an expand kernel per step give the oracle's
+    filterChebyPolyHerm; in FP64 the two forms agree to rounding."""
+    from sdft_amd import stodft, systems
+    say("case: filter fused vs separate expand prec=%d" % prec)
+    emin, emax = stodft.spectral_bounds(si8_system)
+    L = 17
+    co = O.cheby_coeffs(L, emin, emax, 0.3, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si8_system, 6, seed=31)
+    ref, _ = O.filter_cheby(chi, co, si8_system, scale, emean)
+    out = {}
+    for fu in (0, 1):
+        si8_ctx.set_config(0, 512, fu)
+        out[fu], _ = si8_ctx.cheby_filter(chi, co, scale, emean, prec=prec)
+        assert rel(out[fu], ref) < tol, (fu, rel(out[fu], ref))
+    si8_ctx.set_config(0, 512, 1)
+    if prec == 0:
+        assert rel(out[0], out[1]) < 1e-13
