@@ -145,6 +145,14 @@ struct Dft<T, 4, DIR> {
     v[2] = a - c;
     v[3] = b - d;
   }
+  // the same with v[1] = v[2] = 0 (not read)
+  SDFT_HD static inline void run_zin(Cx<T>* v) {
+    const Cx<T> a = v[0], d = mul_di<T, DIR>(v[3]);
+    v[0] = a + v[3];
+    v[1] = a - d;
+    v[2] = a - v[3];
+    v[3] = a + d;
+  }
 };
 
 template <typename T, int DIR>
@@ -154,6 +162,18 @@ struct Dft<T, 8, DIR> {
     Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
     Dft<T, 4, DIR>::run(e);
     Dft<T, 4, DIR>::run(o);
+    combine(v, e, o);
+  }
+  // the same with v[2..5] = 0 (not read): both radix-4 halves have their
+  // middle inputs zero
+  SDFT_HD static inline void run_zin(Cx<T>* v) {
+    Cx<T> e[4] = {v[0], v[0], v[0], v[6]};
+    Cx<T> o[4] = {v[1], v[1], v[1], v[7]};
+    Dft<T, 4, DIR>::run_zin(e);
+    Dft<T, 4, DIR>::run_zin(o);
+    combine(v, e, o);
+  }
+  SDFT_HD static inline void combine(Cx<T>* v, const Cx<T>* e, const Cx<T>* o) {
     const T h = T(0.70710678118654752440084436210485);
     // w^1 = (1 + DIR i)/sqrt2, w^2 = DIR i, w^3 = (-1 + DIR i)/sqrt2
     Cx<T> o1 = o[1], o2 = o[2], o3 = o[3];
@@ -172,6 +192,17 @@ struct Dft<T, 8, DIR> {
 };
 
 // ---- one Stockham pass over lines in LDS ---------------------------------
+// Pruning of a line whose nonzero spectrum is |k| < N/4 (the plane-wave
+// sphere on a grid with n > 4K): in the first pass (NS = 1) of a radix R
+// divisible by 4, butterfly j's inputs j + r N/R with R/4 <= r < 3R/4 lie in
+// [N/4, 3N/4) and are zero (ZIN: not loaded, pruned radix-R DFT); in the last
+// pass (NS = N/R) its outputs there are not needed (ZOUT: not stored, and the
+// compiler drops their arithmetic).
+template <int R>
+constexpr bool mid_index(int r) {
+  return R % 4 == 0 && r >= R / 4 && r < 3 * R / 4;
+}
+
 // tw[m] = exp(+2 pi i m / N), m < N (the table for the line length).
 template <typename T, int N, int DIR, int PASS, int MAXLINES, int NT>
 SDFT_DEV inline void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
@@ -252,13 +283,15 @@ constexpr int ns_sched(int N, int i, bool rev) {
 // the same LDS buffer.  LINE_FAST maps consecutive threads to consecutive
 // lines (same butterfly), otherwise to consecutive butterflies of a line.
 template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
-          class LF, class SF>
+          bool ZIN = false, bool ZOUT = false, class LF, class SF>
 SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
   constexpr int R = radix_sched(N, PASS, REV);
   constexpr int NB = N / R;
   constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
   constexpr int NS = ns_sched(N, PASS, REV);
   constexpr int STEP = N / (NS * R);
+  static_assert(!ZIN || (NS == 1 && R % 4 == 0 && (R == 4 || R == 8)), "ZIN: first pass, radix 4/8");
+  static_assert(!ZOUT || (NS * R == N && R % 4 == 0), "ZOUT: last pass, radix 4/8");
   const int total = nlines * NB;
   Cx<T> v[MAXB][R];
 #pragma unroll
@@ -268,7 +301,8 @@ SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, S
       const int line = LINE_FAST ? idx % nlines : idx / NB;
       const int j = LINE_FAST ? idx / nlines : idx - line * NB;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lf(line, j + r * NB);
+      for (int r = 0; r < R; ++r)
+        if (!(ZIN && mid_index<R>(r))) v[b][r] = lf(line, j + r * NB);
     }
   }
   __syncthreads();
@@ -286,10 +320,14 @@ SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, S
           v[b][r] = DIR > 0 ? cmul(v[b][r], w) : cmulc(v[b][r], w);
         }
       }
-      Dft<T, R, DIR>::run(v[b]);
+      if constexpr (ZIN)
+        Dft<T, R, DIR>::run_zin(v[b]);
+      else
+        Dft<T, R, DIR>::run(v[b]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) sf(line, base + r * NS, v[b][r]);
+      for (int r = 0; r < R; ++r)
+        if (!(ZOUT && mid_index<R>(r))) sf(line, base + r * NS, v[b][r]);
     }
   }
   __syncthreads();
@@ -297,22 +335,25 @@ SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, S
 
 // Passes FIRST..LAST (inclusive) of one transform: the first loads through
 // lf0, the last stores through sf1, the ones between go through the LDS lines
-// buf + line * ld.
+// buf + line * ld.  ZIN / ZOUT prune the first / last pass (see fft_pass_io).
 template <typename T, int N, int DIR, bool REV, int FIRST, int LAST, int MAXLINES, int NT,
-          bool LINE_FAST, class LF, class SF>
+          bool LINE_FAST, bool ZIN = false, bool ZOUT = false, class LF, class SF>
 SDFT_DEV inline void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* tw, int tid,
                                    LF& lf0, SF& sf1) {
   auto lb = [&](int line, int pos) { return buf[line * ld + pos]; };
   auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + pos] = x; };
   if constexpr (FIRST == LAST) {
-    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lf0, sf1);
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, ZOUT>(nlines, tw, tid, lf0,
+                                                                          sf1);
   } else {
-    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lf0, sb);
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, false>(nlines, tw, tid, lf0,
+                                                                           sb);
     if constexpr (FIRST + 1 < LAST) {
       fft_passes_io<T, N, DIR, REV, FIRST + 1, LAST - 1, MAXLINES, NT, LINE_FAST>(
           buf, ld, nlines, tw, tid, lb, sb);
     }
-    fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST>(nlines, tw, tid, lb, sf1);
+    fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST, false, ZOUT>(nlines, tw, tid, lb,
+                                                                           sf1);
   }
 }
 

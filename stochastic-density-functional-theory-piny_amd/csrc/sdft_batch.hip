@@ -73,6 +73,7 @@ struct Group {
   std::vector<int> frags;
   size_t lds = 0;    // plane kernel dynamic LDS, tile variant 0
   size_t lds1 = 0;   // tile variant 1 (0 when it does not fit)
+  bool prune = true;  // every fragment's sphere inside |k| < n/4 (pruned plane passes)
   int tot_exp = 0, tot_plane = 0;
   int* pre_exp = nullptr;    // expand and contract share the (chunk, pair) tiling
   int* pre_plane = nullptr;
@@ -347,7 +348,7 @@ int enqueue_step(Batch& B, int variant, int slot) {
   for (Group& g : B.groups) {
     const int v = (B.plane_variant == 1 && g.lds1 > 0) ? 1 : 0;
     if ((rc = launch_batch(1, g.n, B.prec, g.plane, g.pre_plane, (int)g.frags.size(),
-                           g.tot_plane, v ? g.lds1 : g.lds, v, s)))
+                           g.tot_plane, v ? g.lds1 : g.lds, v | (g.prune ? 2 : 0), s)))
       return rc;
   }
   if (nl && B.overlap) SDFT_HIP_CHECK(hipStreamWaitEvent(s, B.ev_join[slot], 0));
@@ -531,6 +532,7 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
       pp.push_back(pp.back() + f.c->g.n * f.npairs);
       lds = std::max(lds, plane_lds_for(g.n, prec, f.c->g.M, 0));
       lds1 = std::max(lds1, plane_lds_for(g.n, prec, f.c->g.M, 1));
+      if (!plane_prunable(f.c->g.K, g.n)) g.prune = false;
     }
     if (lds > 160 * 1024) {
       set_error("plane buffer exceeds LDS for a fragment grid in this precision");

@@ -325,8 +325,12 @@ ContractArgs make_contract_args(const Ctx& c, int prec, int mode, const void* W,
                                 const double* Xcur, double* Xout, double* acc, const void* fnl,
                                 const double* chi, const StepParams& sp, double* mom_part);
 size_t plane_lds_for(int n, int prec, int M, int variant);
-// one launch of kind 0 (expand), 1 (plane, tile variant `variant`) or 2
-// (contract; variant 1: with the fused expand) over the fragments whose
+// the plane kernel's pruned first / last passes need the sphere inside
+// |k| < n/4 on every axis
+inline bool plane_prunable(int K, int n) { return 4 * K < n; }
+// one launch of kind 0 (expand), 1 (plane; variant bit 0 the tile, bit 1
+// the pruned passes) or 2 (contract; variant 1: with the fused expand) over
+// the fragments whose
 // argument blocks are args[0..nfrag) (device), all of grid size n
 int launch_batch(int kind, int n, int prec, const void* args, const int* prefix, int nfrag,
                  int total, size_t lds, int variant, hipStream_t s);

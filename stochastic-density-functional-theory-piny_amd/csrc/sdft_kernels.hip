@@ -167,7 +167,12 @@ __global__ __launch_bounds__(kThreads) void k_expand_batch(const ExpandArgs* __r
 // Tile variants of the b-line sub-chunk CB (columns rc transformed together):
 // V = 0 the base width (one radix butterfly per thread and pass at n = 64),
 // V = 1 twice it (half the sub-chunks and barriers, two butterflies per
-// thread); the tuner picks per context or batch.
+// thread); the tuner picks per context or batch.  PR = true (for 4K < n, as
+// on every density grid of the reference, n >= 4K + 1) prunes the first pass
+// of the two inverse transforms (c and b: inputs with |k| >= n/4 are zero)
+// and the last pass of the two forward ones (outputs with |k| >= n/4 are
+// not kept); the c-forward runs in reversed radix order so that its last
+// pass is the radix-4/8 one.
 template <typename T, int N, int V = 0>
 struct PlaneCfg {
   static constexpr int ML = Cfg<N>::ML;
@@ -183,7 +188,7 @@ size_t plane2_lds_bytes(int M) {
          ((size_t)M * Cfg<N>::LD + (size_t)PlaneCfg<T, N, V>::CB * Cfg<N>::LD + N);
 }
 
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool PR>
 __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N, V>::CB;
   constexpr int P = n_passes(N);
@@ -219,7 +224,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     };
     Cx<T>* buf = Pl + r0 * LD;
     auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LD + pos] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lw, sb);
+    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false, PR, false>(buf, LD, nl, tw, tid,
+                                                                             lw, sb);
   }
 
   // along b, CB columns rc at a time
@@ -239,10 +245,12 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       if (kb >= -K && kb <= K) Pl[(kb + K) * LD + rc0 + l] = x;
     };
     if constexpr (P >= 2)
-      fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true>(Q, LD, CB, tw, tid, lp, sq);
+      fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true, PR, false>(Q, LD, CB, tw, tid,
+                                                                              lp, sq);
     fft_fused_vmul<T, N, CB, kThreads, true>(CB, tw, tid, lq, vf, sq);
     if constexpr (P >= 2)
-      fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true>(Q, LD, CB, tw, tid, lq, sp);
+      fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true, false, PR>(Q, LD, CB, tw, tid,
+                                                                             lq, sp);
   }
 
   // forward along c: LDS -> W
@@ -255,23 +263,24 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       const int d = rc[l * N + pos];
       if (d >= 0) Wp[d] = x;
     };
-    fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(buf, LD, nl, tw, tid, lb, sw);
+    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, false, false, PR>(buf, LD, nl, tw, tid,
+                                                                           lb, sw);
   }
 }
 
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool PR>
 __global__ __launch_bounds__(kThreads) void k_plane(PlaneArgs a) {
-  plane_body<T, N, V>(a, blockIdx.x, blockIdx.y);
+  plane_body<T, N, V, PR>(a, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool PR>
 __global__ __launch_bounds__(kThreads) void k_plane_batch(const PlaneArgs* __restrict__ args,
                                                           const int* __restrict__ prefix,
                                                           int nfrag) {
   const int f = find_frag(prefix, nfrag, blockIdx.x);
   const PlaneArgs a = args[f];
   const int local = blockIdx.x - prefix[f];
-  plane_body<T, N, V>(a, local % a.gx, local / a.gx);
+  plane_body<T, N, V, PR>(a, local % a.gx, local / a.gx);
 }
 
 // ---- K2, split form: for planes whose M x (n+1) buffer exceeds LDS --------
@@ -959,22 +968,31 @@ size_t plane_lds_for(int n, int prec, int M, int variant) {
 
 // ---- batched launches: one launch per kernel over the fragments of one grid
 // size; `prefix` holds each fragment's first workgroup (size nfrag + 1).
+template <typename T, int N, int V, bool PR>
+static int plane_batch_launch(const void* args, const int* prefix, int nfrag, int total,
+                              size_t lds, hipStream_t s) {
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_batch<T, N, V, PR>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_plane_batch<T, N, V, PR>), dim3(total), dim3(kThreads), lds, s,
+                     reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+  return 0;
+}
+
 template <typename T, int N>
 static int batch_launch_impl(int kind, const void* args, const int* prefix, int nfrag, int total,
                              size_t lds, int variant, hipStream_t s) {
   if (kind == 0) {
     hipLaunchKernelGGL((k_expand_batch<T, N>), dim3(total), dim3(kThreads), 0, s,
                        reinterpret_cast<const ExpandArgs*>(args), prefix, nfrag);
-  } else if (kind == 1) {
-    const void* fn = variant == 1 ? reinterpret_cast<const void*>(&k_plane_batch<T, N, 1>)
-                                  : reinterpret_cast<const void*>(&k_plane_batch<T, N, 0>);
-    SDFT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (variant == 1)
-      hipLaunchKernelGGL((k_plane_batch<T, N, 1>), dim3(total), dim3(kThreads), lds, s,
-                         reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
-    else
-      hipLaunchKernelGGL((k_plane_batch<T, N, 0>), dim3(total), dim3(kThreads), lds, s,
-                         reinterpret_cast<const PlaneArgs*>(args), prefix, nfrag);
+  } else if (kind == 1) {  // variant: bit 0 tile, bit 1 pruned passes
+    int rc = 0;
+    switch (variant & 3) {
+      case 0: rc = plane_batch_launch<T, N, 0, false>(args, prefix, nfrag, total, lds, s); break;
+      case 1: rc = plane_batch_launch<T, N, 1, false>(args, prefix, nfrag, total, lds, s); break;
+      case 2: rc = plane_batch_launch<T, N, 0, true>(args, prefix, nfrag, total, lds, s); break;
+      default: rc = plane_batch_launch<T, N, 1, true>(args, prefix, nfrag, total, lds, s); break;
+    }
+    if (rc) return rc;
   } else if (variant == 1) {  // contract with the fused expand
     hipLaunchKernelGGL((k_contract_batch<T, N, true>), dim3(total), dim3(kThreads), 0, s,
                        reinterpret_cast<const ContractArgs*>(args), prefix, nfrag);
@@ -1023,28 +1041,32 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
   return 0;
 }
 
+template <typename T, int N, int V, bool PR>
+static int plane_launch(Ctx& c, const PlaneArgs& a, size_t lds, int npairs) {
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, V, PR>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_plane<T, N, V, PR>), dim3(N, npairs), dim3(kThreads), lds, c.stream, a);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <typename T, int N>
 static int plane_impl(Ctx& c, void* W, int npairs, void** Wout) {
   const Geometry& g = c.g;
   PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
   dim3 grid(N, npairs);
+  const bool pr = plane_prunable(g.K, N);
   const size_t lds1 = plane2_lds_bytes<T, N, 1>(g.M);
   if (c.plane_variant == 1 && lds1 <= 160 * 1024) {
-    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, 1>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
-    hipLaunchKernelGGL((k_plane<T, N, 1>), grid, dim3(kThreads), lds1, c.stream, a);
-    SDFT_HIP_CHECK(hipGetLastError());
     *Wout = W;
-    return 0;
+    return pr ? plane_launch<T, N, 1, true>(c, a, lds1, npairs)
+              : plane_launch<T, N, 1, false>(c, a, lds1, npairs);
   }
   const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
   if (lds <= 160 * 1024) {
-    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, 0>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_plane<T, N, 0>), grid, dim3(kThreads), lds, c.stream, a);
-    SDFT_HIP_CHECK(hipGetLastError());
     *Wout = W;
-    return 0;
+    return pr ? plane_launch<T, N, 0, true>(c, a, lds, npairs)
+              : plane_launch<T, N, 0, false>(c, a, lds, npairs);
   }
   const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
   if (lds2 > 160 * 1024) {
