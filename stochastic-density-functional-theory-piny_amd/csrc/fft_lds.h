@@ -404,6 +404,21 @@ SDFT_DEV inline void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf
   __syncthreads();
 }
 
+// fft_lines with a choice of thread mapping (LINE_FAST: consecutive threads
+// on consecutive lines; otherwise on consecutive butterflies of one line).
+template <typename T, int N, int DIR, int MAXLINES, int NT, bool LINE_FAST>
+SDFT_DEV inline void fft_lines_map(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+  if constexpr (!LINE_FAST) {
+    fft_lines<T, N, DIR, MAXLINES, NT>(buf, nlines, ld, tw, tid);
+  } else {
+    constexpr int P = n_passes(N);
+    auto lb = [&](int line, int pos) { return buf[line * ld + pos]; };
+    auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + pos] = x; };
+    fft_passes_io<T, N, DIR, false, 0, P - 1, MAXLINES, NT, true>(buf, ld, nlines, tw, tid, lb,
+                                                                  sb);
+  }
+}
+
 // Any number of lines: batches of MAXLINES.
 template <typename T, int N, int DIR, int MAXLINES, int NT>
 SDFT_DEV inline void fft_lines_batched(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {

@@ -56,6 +56,16 @@ int chunk_cols_for(int n) {
 }
 
 
+// Thread mapping of the passes over lines that sit next to each other in
+// LDS (c rows of a plane, a columns of a chunk): with consecutive threads on
+// consecutive butterflies of one line, a first pass stores with a stride of R
+// elements, which for 16-byte (double) elements puts a line's butterflies in
+// one LDS bank; consecutive threads on consecutive lines (stride n + 1)
+// avoid that.  For float the line-major order keeps the global gathers of
+// the c rows denser and measured faster.
+template <typename T>
+constexpr bool kLineFast = sizeof(T) == 8;
+
 template <typename T, int N>
 size_t plane_lds_bytes(int M) {
   return sizeof(Cx<T>) * ((size_t)M * Cfg<N>::LD + (size_t)Cfg<N>::CB * Cfg<N>::LD + N);
@@ -133,7 +143,7 @@ __device__ __forceinline__ void expand_body(const ExpandArgs& a, int chunk, int 
     buf[(d - c0) * LD + ia] = cx(T(zr), T(zi));
   }
   __syncthreads();
-  fft_lines<T, N, +1, ML, kThreads>(buf, nc, LD, tw, tid);
+  fft_lines_map<T, N, +1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
   Cx<T>* Wq = W + (size_t)q * N * ncolp;
   for (int e = tid; e < nc * N; e += kThreads) {
     const int ra = e / nc;
@@ -224,8 +234,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     };
     Cx<T>* buf = Pl + r0 * LD;
     auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LD + pos] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false, PR, false>(buf, LD, nl, tw, tid,
-                                                                             lw, sb);
+    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, kLineFast<T>, PR, false>(
+        buf, LD, nl, tw, tid, lw, sb);
   }
 
   // along b, CB columns rc at a time
@@ -263,8 +273,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       const int d = rc[l * N + pos];
       if (d >= 0) Wp[d] = x;
     };
-    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, false, false, PR>(buf, LD, nl, tw, tid,
-                                                                           lb, sw);
+    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, kLineFast<T>, false, PR>(
+        buf, LD, nl, tw, tid, lb, sw);
   }
 }
 
@@ -523,7 +533,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
     buf[dl * LD + ra] = Wq[(size_t)ra * ncolp + c0 + dl];
   }
   __syncthreads();
-  fft_lines<T, N, -1, ML, kThreads>(buf, nc, LD, tw, tid);
+  fft_lines_map<T, N, -1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
 
   double dots[3] = {0.0, 0.0, 0.0};
   Cx<T>* Wn = reinterpret_cast<Cx<T>*>(a.Wnext);
@@ -679,7 +689,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
       if (ka > h || ka < -h) buf[dl * LD + ia] = zero;
     }
     __syncthreads();
-    fft_lines<T, N, +1, ML, kThreads>(buf, nc, LD, tw, tid);
+    fft_lines_map<T, N, +1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
     Cx<T>* Wq2 = Wn + (size_t)q * N * ncolp;
     for (int e = tid; e < nc * N; e += kThreads) {
       const int ra = e / nc;
