@@ -200,6 +200,9 @@ def main():
     log("FP32 vs FP64: density L1/e %.3e, energy/atom %.3e Ha (E64 %.6f, Ne %.6f)" % (
         dens_l1, de_atom, e64, ne64))
 
+    # v_KS from the density (Hartree + external + LDA on the device)
+    ks = ks_bench(ctx, sysm, rho64, not args.no_cpu_baseline)
+
     # roofline of the dominant kernel (FP32 form)
     kt32 = results["fp32"]["kernels"]
     dom = max(kt32.items(), key=lambda kv: kv[1][0])
@@ -268,8 +271,35 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fragment_batch": batch,
+        "ks_potential": ks,
     }
     print(json.dumps(line), flush=True)
+
+
+def ks_bench(ctx, sysm, rho, with_cpu, reps=20):
+    """sdft_hip_ks_potential on the workload's own FP64 density: wall time
+    per call (host density in, v_KS set on the device and copied out, the
+    four energies), and the FP64 numpy oracle's time for one call."""
+    from sdft_amd import systems
+    dens = systems.density_sphere(sysm)
+    vext = systems.vloc_density(sysm, dens)
+    ctx.ks_init(dens)
+    ctx.ks_potential(rho, vext, sysm.volume, xc=0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vks, en = ctx.ks_potential(rho, vext, sysm.volume, xc=0)
+    dt = (time.perf_counter() - t0) / reps
+    out = {"ms_per_call": round(1e3 * dt, 3), "density_sphere_ncoef": dens.ncoef, "xc": "pz_lda",
+           "energies": {k: float(v) for k, v in en.items()}}
+    if with_cpu:
+        from oracle import sdft_oracle as O
+        t1 = time.perf_counter()
+        ref, _ = O.ks_potential(rho, vext, dens, sysm.volume, 0)
+        out["cpu_oracle_ms"] = round(1e3 * (time.perf_counter() - t1), 1)
+        out["max_rel_diff_vs_oracle"] = float(np.max(np.abs(vks - ref)) / np.max(np.abs(ref)))
+    ctx.set_vks(sysm.vks)
+    log("ks_potential: %.3f ms/call (density sphere %d coefficients)" % (1e3 * dt, dens.ncoef))
+    return out
 
 
 def _build_fragment(spec):

@@ -34,6 +34,10 @@ extern "C" {
 #define SDFT_RHO_FFTW 0    /* density in FFTW order (a,b,c), c fastest */
 #define SDFT_RHO_GENERIC 1 /* density in the generic PINY order, a fastest */
 
+#define SDFT_XC_PZ_LDA 0   /* Perdew-Zunger LDA (vxc_typ "pz_lda") */
+#define SDFT_XC_PW_LDA 1   /* Perdew-Wang 1992 LDA ("pw_lda") */
+#define SDFT_XC_PADE_LDA 2 /* Goedecker-Teter-Hutter Pade LDA ("pade_lda") */
+
 typedef struct sdft_ctx sdft_ctx;     /* one system: grid, G-set, v_KS, KB table */
 typedef struct sdft_block sdft_block; /* a device-resident block of orbitals */
 
@@ -277,6 +281,28 @@ int sdft_hip_selftest_fft(int n, int nlines, int dir, int prec, const double *in
  * device: the known-size reference against which hardware byte counters of
  * the filter kernels are scaled.  No reference counterpart. */
 int sdft_hip_selftest_copy(long long nbytes, int reps);
+
+/* ---- Kohn-Sham potential from the density (closing the SCF loop) ------
+ * sdft_hip_ks_init: the density plane-wave half sphere (PINY's large density
+ * grid package cp_para_fft_pkg3d_lg: |G|^2 <= 4 x the wave-function cutoff,
+ * G = 0 last) on the context's grid; desc->map / map_conj / ak2_kinetic are
+ * its +G / -G FFTW indices and |G|^2, n1..n3 must equal the context's.
+ * sdft_hip_ks_potential: cpGetVksStodft
+ * (stodft/reduced-energy/cp-energy-ee-rho-stodft.c:644-900) after
+ * calcRhoStoRecipFullg (stodft/density.c:699-870) for a periodic cell (iperd
+ * = 3) without the dual grid: rho (n1 n2 n3 doubles in `layout`, as
+ * sdft_hip_density returns it: sum rho / N = N_e) is transformed to the
+ * sphere, v(G) = vext(G) + 4 pi rho(G) / (|G|^2 V) (vext: ncoef doubles re /
+ * im in the sphere's order, plane-wave amplitudes; G = 0: vext(0)), v(G) ->
+ * v(r), and the LDA potential `xc` (SDFT_XC_*) of rho / V is added.  The
+ * result becomes the context's v_KS (as sdft_hip_set_vks, without a host
+ * copy); energies (nullable, 4 doubles): E_Hartree, E_ext, E_xc and the
+ * muxc sum as cpGetVksStodft / excpot_* accumulate them; vks_out (nullable):
+ * v_KS in `layout`. */
+int sdft_hip_ks_init(sdft_ctx *ctx, const sdft_grid_desc *dens);
+int sdft_hip_ks_potential(sdft_ctx *ctx, const double *rho, int layout, const double *vext_re,
+                          const double *vext_im, double volume, int xc, double *energies,
+                          double *vks_out);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,16 @@ typedef struct SDFT_STODFT {
   /* spectral range start / result vectors (STODFTCOEFPOS testWfMax/Min,
    * 0-based [numCoeff]) */
   double *testWfMaxRe, *testWfMaxIm, *testWfMinRe, *testWfMinIm;
+  /* Kohn-Sham potential (cpGetVksStodft): the context's density sphere set
+   * by sdft_hip_ks_init; vextr / vexti the local external potential on it
+   * (CPSCR_LOC vextr_loc / vexti_loc, 1-based [ncoef_l]), the cell volume,
+   * the LDA form (SDFT_XC_*), and on return the STAT_AVG energies and
+   * (nullable, 0-based [n1*n2*n3], FFTW order) v_ks_up */
+  double *vextr, *vexti;
+  double volume;
+  int vxcType;
+  double ehart, eext, exc, muxc;
+  double *vKsUp;
 } SDFT_STODFT;
 
 /* stodft/filter.c:318-706: acc = sum_k c_k T_k chi; cre_up left = T_{L-1} chi. */
@@ -78,6 +88,11 @@ void filterNewtonPolyHerm(SDFT_STODFT *st);
  * energyMin = E - 0.1 (eigValMin's E), energyConv 1e-5. */
 void genEnergyMax(SDFT_STODFT *st);
 void genEnergyMin(SDFT_STODFT *st);
+/* stodft/reduced-energy/energy-wrapper-scf.c:584-745 (calcKSPot ->
+ * cpGetVksStodft, iperd = 3, no dual grid, LDA) from rhoUp (as
+ * calcRhoStoHybridCheby leaves it): the context's v_KS is replaced on the
+ * device; ehart / eext / exc / muxc and vKsUp (if set) are written. */
+void calcKSPot(SDFT_STODFT *st);
 
 #ifdef __cplusplus
 }

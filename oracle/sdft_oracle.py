@@ -26,6 +26,10 @@ Parity status of each function (how it is pinned; see tests/):
   density_real                              : sum(rho)/N = occ <chi|chi>/Nsto
       identity and the brute-force real-space sum
   energy_ke / energy_nl / nl_forces         : closed forms and FD derivatives
+  xc_lda / ks_potential                     : published correlation energies
+      (Ceperley-Alder points), exchange closed form, PZ branch continuity,
+      v_xc = d(rho eps_xc)/d rho by finite differences, Hartree energy and
+      potential of a plane-wave density in closed form
 """
 from __future__ import annotations
 
@@ -554,3 +558,104 @@ def dense_hamiltonian(sysm, with_nl=True):
     C = from_real_vec(E, nc)
     H = to_real_vec(hpsi(C, sysm, with_nl))
     return H.T
+
+
+# ---------------------------------------------------------------------------
+# Kohn-Sham potential from the density (cpGetVksStodft, LDA, periodic)
+# ---------------------------------------------------------------------------
+
+XC_PZ_LDA, XC_PW_LDA, XC_PADE_LDA = 0, 1, 2
+
+
+def xc_lda(rho, kind):
+    """LDA exchange-correlation of the physical density rho (e/bohr^3) on a
+    grid: returns (v_xc, e_density, mu_density) with v_xc the potential,
+    e_density = rho * eps_xc and mu_density the v_xc * rho term that
+    excpot_* sum into muxc.  Restates excpot_pz_lda
+    (energy/cp/xc_functionals.c:268-362: Perdew-Zunger, rho <= 0 clamped to
+    1e-10), excpot_pw_lda (:368-480: Perdew-Wang 1992, same clamp) and
+    excpot_pade_lda (:32-101: Goedecker-Teter-Hutter Pade fit, no clamp, no
+    muxc contribution), without the LYP switch (lyp_fact = 1)."""
+    rho = np.asarray(rho, dtype=np.float64)
+    if kind == XC_PADE_LDA:
+        a = (0.4581652932831429, 2.217058676663745, 0.7405551735357053, 0.01968227878617998)
+        b = (1.0, 4.504130959426697, 1.110667363742916, 0.02359291751427506)
+        rs = np.cbrt(3.0 / (4.0 * math.pi * rho))
+        num = ((a[3] * rs + a[2]) * rs + a[1]) * rs + a[0]
+        den = (((b[3] * rs + b[2]) * rs + b[1]) * rs + b[0]) * rs
+        drs = -rs / (3.0 * rho)
+        num_p = (3.0 * a[3] * rs + 2.0 * a[2]) * rs + a[1]
+        den_p = ((4.0 * b[3] * rs + 3.0 * b[2]) * rs + 2.0 * b[1]) * rs + b[0]
+        dfact = ((den * num_p - den_p * num) / (den * den)) * drs
+        v = -(dfact * rho + num / den)
+        return v, -(num / den) * rho, np.zeros_like(rho)
+    r = np.where(rho <= 0.0, 1.0e-10, rho)
+    rs = np.cbrt(3.0 / (4.0 * math.pi * r))
+    sq = np.sqrt(rs)
+    vx = -np.cbrt(3.0 * r / math.pi)
+    if kind == XC_PZ_LDA:
+        g, b1, b2 = -0.14230, 1.05290, 0.33340
+        A, B, C, D = 0.03110, -0.0480, 0.00200, -0.01160
+        hi = rs >= 1.0
+        cf1 = 1.0 + b1 * sq + b2 * rs
+        cf2 = 1.0 + (7.0 / 6.0) * b1 * sq + (4.0 / 3.0) * b2 * rs
+        ec_hi = g / cf1
+        mu_hi = ec_hi * (cf2 / cf1)
+        lr = np.log(rs)
+        ec_lo = A * lr + B + C * rs * lr + D * rs
+        mu_lo = A * lr + (B - A / 3.0) + (2.0 / 3.0) * C * rs * lr + (1.0 / 3.0) * (2.0 * D - C) * rs
+        ec = np.where(hi, ec_hi, ec_lo)
+        mu = np.where(hi, mu_hi, mu_lo)
+    elif kind == XC_PW_LDA:
+        a, al1 = 0.03109070, 0.213700
+        be = (7.59570, 3.58760, 1.63820, 0.492940)
+        cf1 = 2.0 * a * (be[0] * sq + be[1] * rs + be[2] * rs ** 1.5 + be[3] * rs * rs)
+        cf2 = 2.0 * a * (0.5 * be[0] / sq + be[1] + 1.5 * be[2] * sq + 2.0 * be[3] * rs)
+        lg = np.log(1.0 + 1.0 / cf1)
+        opa = 1.0 + al1 * rs
+        ec = -2.0 * a * opa * lg
+        mu = -2.0 * a * (1.0 + 2.0 * al1 * rs / 3.0) * lg + (-2.0 * a) * rs * opa / (3.0 * cf1 * (1.0 + cf1)) * cf2
+    else:
+        raise ValueError("unknown LDA kind")
+    v = vx + mu
+    return v, (0.75 * vx + ec) * r, v * r
+
+
+def ks_potential(rho_raw, vext, dens, volume, kind):
+    """v_KS(r) (FFTW order) and the energies from the stochastic density.
+
+    rho_raw: the density as the filter path produces it (sum rho / N = N_e,
+    i.e. the physical density times the cell volume), FFTW order.  dens: the
+    density plane-wave half sphere (k, map, map_conj, ak2; G = 0 last);
+    vext: v_ext(G) on it (plane-wave amplitudes).  Restates
+    calcRhoStoRecipFullg (stodft/density.c:699-870: pack rho, r->G FFT,
+    unpack the sphere, divide rho(r) by the volume) and cpGetVksStodft
+    (stodft/reduced-energy/cp-energy-ee-rho-stodft.c:644-900, iperd = 3, no
+    dual grid): Q(G) = (1/N) sum_r rho_raw e^{-iG.r}, v(G) = vext(G) +
+    4 pi Q(G) / (|G|^2 V) for G != 0 and vext(0) at G = 0; E_H = sum over
+    the half sphere of 4 pi |Q|^2 / (|G|^2 V); E_ext = 2 sum_half Re(vext
+    conj(Q)) + vext(0) Q(0); v(r) = sum_G v(G) e^{iG.r} over the full sphere;
+    then the LDA potential of rho_raw / V is added (excpot_*).  Returns
+    (v_ks, dict(eh, eext, exc, muxc, Q))."""
+    n = tuple(int(x) for x in dens.n)
+    N = n[0] * n[1] * n[2]
+    rho_raw = np.asarray(rho_raw, dtype=np.float64).reshape(n)
+    F = np.fft.fftn(rho_raw).ravel() / N
+    Q = F[dens.map]
+    nc = Q.shape[0]
+    ak2 = np.asarray(dens.ak2, dtype=np.float64)
+    gh = np.zeros(nc)
+    gh[:-1] = 4.0 * math.pi / (ak2[:-1] * volume)
+    v = np.asarray(vext, dtype=np.complex128) + gh * Q
+    eh = float(np.sum(gh[:-1] * np.abs(Q[:-1]) ** 2))
+    eext = float(2.0 * np.sum((vext[:-1] * np.conj(Q[:-1])).real) + (vext[-1] * np.conj(Q[-1])).real)
+    full = np.zeros(N, dtype=np.complex128)
+    full[dens.map[:-1]] = v[:-1]
+    full[dens.map_conj[:-1]] = np.conj(v[:-1])
+    full[dens.map[-1]] = v[-1].real
+    v_r = (np.fft.ifftn(full.reshape(n)) * N).real.ravel()
+    rho_phys = rho_raw.ravel() / volume
+    vxc, ed, md = xc_lda(rho_phys, kind)
+    vscale = volume / N
+    out = dict(eh=eh, eext=eext, exc=float(ed.sum() * vscale), muxc=float(md.sum() * vscale), Q=Q)
+    return v_r + vxc, out

@@ -268,7 +268,12 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
 }
 
 static void free_ctx(Ctx& c) {
-  void* ptrs[] = {c.t.col_pos,  c.t.rowcol,     c.t.ent_off,   c.t.ent_j,      c.t.ent_b,
+  if (c.dens) {
+    free_ctx(*c.dens);
+    delete c.dens;
+    c.dens = nullptr;
+  }
+  void* ptrs[] = {c.ks_field, c.ks_rho, c.ks_vhe, c.ks_part, c.ks_io,c.t.col_pos,  c.t.rowcol,     c.t.ent_off,   c.t.ent_j,      c.t.ent_b,
                   c.t.chunk_col,
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
@@ -1440,6 +1445,133 @@ int sdft_hip_kernel_times(sdft_ctx* h, int max, const char** names, double* ms, 
     if (launches) launches[i] = c.prof_launch[i];
   }
   return n;
+}
+
+// ---- Kohn-Sham potential from the density ----------------------------------
+int sdft_hip_ks_init(sdft_ctx* h, const sdft_grid_desc* d) {
+  if (!h || !d) {
+    set_error("null argument");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  if (d->n1 != c.g.n || d->n2 != c.g.n || d->n3 != c.g.n) {
+    set_error("the density sphere must be on the context's grid");
+    return SDFT_EINVAL;
+  }
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (c.dens) {
+    free_ctx(*c.dens);
+    delete c.dens;
+    c.dens = nullptr;
+  }
+  Ctx* dc = new Ctx();
+  dc->device = c.device;
+  hipError_t e = hipStreamCreateWithFlags(&dc->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete dc;
+    return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
+  }
+  int rc = build_geometry(*dc, d);
+  if (rc) {
+    free_ctx(*dc);
+    delete dc;
+    return rc;
+  }
+  c.dens = dc;
+  const size_t N = (size_t)c.g.N;
+  const size_t nb = std::max((N + 255) / 256, (size_t)(dc->g.ncoef + 255) / 256);
+  if (!c.ks_field && (rc = dalloc(&c.ks_field, 2 * sizeof(double) * N))) return rc;
+  if (!c.ks_rho && (rc = dalloc(&c.ks_rho, sizeof(double) * N))) return rc;
+  if (!c.ks_vhe && (rc = dalloc(&c.ks_vhe, sizeof(double) * N))) return rc;
+  if (!c.ks_io && (rc = dalloc(&c.ks_io, sizeof(double) * N))) return rc;
+  if (c.ks_part_n < 4 * nb) {
+    if (c.ks_part) SDFT_HIP_CHECK(hipFree(c.ks_part));
+    c.ks_part = nullptr;
+    c.ks_part_n = 0;
+    if ((rc = dalloc(&c.ks_part, sizeof(double) * 4 * nb))) return rc;
+    c.ks_part_n = 4 * nb;
+  }
+  return 0;
+}
+
+int sdft_hip_ks_potential(sdft_ctx* h, const double* rho, int layout, const double* vext_re,
+                          const double* vext_im, double volume, int xc, double* energies,
+                          double* vks_out) {
+  if (!h || !rho || !vext_re || !vext_im || !(volume > 0.0) || (layout != SDFT_RHO_FFTW &&
+      layout != SDFT_RHO_GENERIC) || xc < SDFT_XC_PZ_LDA || xc > SDFT_XC_PADE_LDA) {
+    set_error("bad Kohn-Sham potential arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  if (!c.dens) {
+    set_error("density sphere not set (call sdft_hip_ks_init first)");
+    return SDFT_ESTATE;
+  }
+  Ctx& dc = *c.dens;
+  const long long N = c.g.N;
+  const int n = c.g.n;
+  int rc;
+  // the context's v_KS is rewritten below: its queued filter work comes first
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (!c.vks64 && (rc = dalloc(&c.vks64, sizeof(double) * N))) return rc;
+  if (!c.vks32 && (rc = dalloc(&c.vks32, sizeof(float) * N))) return rc;
+  hipStream_t s = dc.stream;
+  // rho -> the sphere: Q(G) = (1/N) sum_r rho e^{-iG.r} (row 0 of X0)
+  SDFT_HIP_CHECK(hipMemcpyAsync(c.ks_io, rho, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  if ((rc = launch_ks_field(s, c.ks_io, layout, n, N, volume, c.ks_field, c.ks_rho))) return rc;
+  if ((rc = ensure_W(dc, w_bytes(dc, 1, 0)))) return rc;
+  Block b;
+  rc = block_alloc(dc, 1, 0, b);
+  if (!rc) rc = launch_grid_plane(dc, dc.W, 1, c.ks_field);
+  StepParams sp{};
+  if (!rc)
+    rc = launch_contract(dc, 0, kModeNoise, dc.W, 1, b.X0, b.X0, b.chi, nullptr, nullptr, sp,
+                         nullptr);
+  // v(G) = vext + Hartree (row 0 of X1; row 1 stays zero)
+  if (!rc) rc = block_put(b, b.X1, vext_re, vext_im);
+  int nb1 = 0, nb2 = 0;
+  if (!rc)
+    rc = launch_ks_hartree(s, b.X0, b.X1, dc.t.ak2, dc.g.ncoef, dc.g.ncp, volume, c.ks_part,
+                           &nb1);
+  // v(G) -> v(r): expand, then the c / b transforms with the real part kept
+  if (!rc) rc = launch_expand(dc, 0, b.X1, 1, dc.W);
+  if (!rc) rc = launch_density(dc, dc.W, 1, c.ks_vhe, 1, true);
+  // + LDA exchange-correlation -> the context's v_KS
+  if (!rc)
+    rc = launch_ks_xc(s, c.ks_vhe, c.ks_rho, N, xc, c.vks64, c.vks32, c.ks_part + 2 * nb1, &nb2);
+  std::vector<double> part;
+  if (!rc) {
+    part.resize(2 * (size_t)(nb1 + nb2));
+    SDFT_HIP_CHECK(hipMemcpyAsync(part.data(), c.ks_part, sizeof(double) * part.size(),
+                                  hipMemcpyDeviceToHost, s));
+    if (vks_out) {
+      rc = launch_ks_out(s, c.vks64, layout, n, N, c.ks_io);
+      if (!rc)
+        SDFT_HIP_CHECK(hipMemcpyAsync(vks_out, c.ks_io, sizeof(double) * N,
+                                      hipMemcpyDeviceToHost, s));
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  block_free(b);
+  if (rc) return rc;
+  c.have_vks = true;
+  if (energies) {
+    double eh = 0.0, ee = 0.0, ex = 0.0, mx = 0.0;
+    for (int i = 0; i < nb1; ++i) {
+      eh += part[2 * i];
+      ee += part[2 * i + 1];
+    }
+    for (int i = nb1; i < nb1 + nb2; ++i) {
+      ex += part[2 * i];
+      mx += part[2 * i + 1];
+    }
+    const double vscale = volume / (double)N;
+    energies[0] = eh;
+    energies[1] = ee;
+    energies[2] = ex * vscale;
+    energies[3] = mx * vscale;
+  }
+  return 0;
 }
 
 int sdft_hip_selftest_copy(long long nbytes, int reps) {

@@ -128,6 +128,15 @@ struct Ctx {
   int newton_nmu = 0;
   long long newton_stride = 0;
   const double* newton_c = nullptr;
+  // Kohn-Sham potential from the density: the density-sphere geometry (its
+  // own context on the same grid, owned) and its device buffers
+  Ctx* dens = nullptr;
+  void* ks_field = nullptr;     // [N] complex: the density as a field
+  double* ks_rho = nullptr;     // [N] physical density, FFTW order
+  double* ks_vhe = nullptr;     // [N] v_H + v_ext in real space
+  double* ks_part = nullptr;    // per-block energy partials
+  double* ks_io = nullptr;      // [N] the caller's density / v_KS staging
+  size_t ks_part_n = 0;
   // profiling
   bool profiling = false;
   std::vector<std::string> prof_names;
@@ -309,7 +318,19 @@ int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
                     const double* Xcur, double* Xout, double* acc, const void* fnl,
                     const double* chi, const StepParams& sp, double* mom_part,
                     void* Wnext = nullptr, float* Xout32 = nullptr);
-int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups);
+// re = true: a single pair (ngroups = 1) whose real part is written to
+// rho_part (the real field sum_G c1(G) e^{iG.r}) instead of |z|^2
+int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups,
+                   bool re = false);
+// Kohn-Sham potential kernels (sdft_ks.hip)
+int launch_ks_field(hipStream_t s, const double* rho, int layout, int n, long long N,
+                    double vol, void* field, double* rho_fftw);
+int launch_ks_hartree(hipStream_t s, const double* Q, double* V, const double* ak2, int ncoef,
+                      int ncp, double vol, double* part, int* nblocks);
+int launch_ks_xc(hipStream_t s, const double* vhe, const double* rho, long long N, int kind,
+                 double* vks, float* vks32, double* part, int* nblocks);
+int launch_ks_out(hipStream_t s, const double* vks, int layout, int n, long long N,
+                  double* out);
 int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                        int nstate);
 int launch_grid_plane(Ctx& c, void* W, int npairs, const void* field);

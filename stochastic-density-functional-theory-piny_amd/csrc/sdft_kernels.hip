@@ -715,7 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_contract_batch(const ContractArgs*
 }
 
 // ---- density: |psi1|^2 + |psi2|^2 summed over the pairs of a group -------
-template <typename T, int N>
+template <typename T, int N, bool RE>
 __global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ W, int ncolp,
                                                       int ncol, const int* __restrict__ col_pos,
                                                       int M, const Cx<T>* __restrict__ twg,
@@ -752,9 +752,13 @@ __global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ 
       for (int e = tid; e < CB * N; e += kThreads) {
         const int rb = e / CB, l = e - rb * CB;
         const Cx<T> z = Q[l * LD + rb];
-        const double s = double(z.x) * double(z.x) + double(z.y) * double(z.y);
         const size_t idx = (size_t)rb * N + rc0 + l;
-        rp[idx] = first ? s : rp[idx] + s;
+        if (RE) {
+          rp[idx] = double(z.x);
+        } else {
+          const double s = double(z.x) * double(z.x) + double(z.y) * double(z.y);
+          rp[idx] = first ? s : rp[idx] + s;
+        }
       }
       __syncthreads();
     }
@@ -769,7 +773,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ 
 // pair and column half, the inverse c-transform of the M rows (through Q,
 // recomputed per half) keeps the half's columns, and its b-lines accumulate
 // |psi1|^2 + |psi2|^2 straight from the last pass's outputs.
-template <typename T, int N>
+template <typename T, int N, bool RE>
 __global__ __launch_bounds__(kThreads) void k_density_split(
     const Cx<T>* __restrict__ W, int ncolp, const int* __restrict__ rowcol, int M,
     const Cx<T>* __restrict__ twg, double* __restrict__ rho_part, int npairs, int ngroups) {
@@ -815,9 +819,13 @@ __global__ __launch_bounds__(kThreads) void k_density_split(
           return (kb >= -K && kb <= K) ? Ph[(kb + K) * LDH + rc0 + l] : zero;
         };
         auto sr = [&](int l, int pos, Cx<T> z) {
-          const double s2 = double(z.x) * double(z.x) + double(z.y) * double(z.y);
           const size_t idx = (size_t)pos * N + c_lo + rc0 + l;
-          rp[idx] = first ? s2 : rp[idx] + s2;
+          if (RE) {
+            rp[idx] = double(z.x);
+          } else {
+            const double s2 = double(z.x) * double(z.x) + double(z.y) * double(z.y);
+            rp[idx] = first ? s2 : rp[idx] + s2;
+          }
         };
         fft_passes_io<T, N, +1, false, 0, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lp, sr);
       }
@@ -1146,7 +1154,7 @@ int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs, const
   return 0;
 }
 
-template <typename T, int N>
+template <typename T, int N, bool RE>
 static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups,
                         const void* tw) {
   const Geometry& g = c.g;
@@ -1157,27 +1165,37 @@ static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int
       set_error("plane buffer exceeds LDS for this grid in this precision");
       return SDFT_EINVAL_CODE;
     }
-    SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_split<T, N>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-    hipLaunchKernelGGL((k_density_split<T, N>), dim3(N, ngroups), dim3(kThreads), lds2, c.stream,
+    SDFT_HIP_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_density_split<T, N, RE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL((k_density_split<T, N, RE>), dim3(N, ngroups), dim3(kThreads), lds2,
+                       c.stream,
                        reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M,
                        reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
     SDFT_HIP_CHECK(hipGetLastError());
     return 0;
   }
-  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density<T, N>),
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density<T, N, RE>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid(N, ngroups);
-  hipLaunchKernelGGL((k_density<T, N>), grid, dim3(kThreads), lds, c.stream,
+  hipLaunchKernelGGL((k_density<T, N, RE>), grid, dim3(kThreads), lds, c.stream,
                      reinterpret_cast<const Cx<T>*>(W), g.ncolp, g.ncol, c.t.col_pos, g.M,
                      reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups) {
-  SDFT_SIZE_SWITCH(c.g.n, return (density_impl<double, NN>(c, W, npairs, rho_part, ngroups,
-                                                           c.t.tw64)));
+int launch_density(Ctx& c, const void* W, int npairs, double* rho_part, int ngroups, bool re) {
+  if (re) {
+    if (npairs != 1 || ngroups != 1) {
+      set_error("internal error: real field output takes one pair");
+      return SDFT_EINVAL_CODE;
+    }
+    SDFT_SIZE_SWITCH(c.g.n, return (density_impl<double, NN, true>(c, W, npairs, rho_part,
+                                                                   ngroups, c.t.tw64)));
+  }
+  SDFT_SIZE_SWITCH(c.g.n, return (density_impl<double, NN, false>(c, W, npairs, rho_part,
+                                                                  ngroups, c.t.tw64)));
   return 0;
 }
 

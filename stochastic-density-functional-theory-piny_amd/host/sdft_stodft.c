@@ -124,6 +124,17 @@ void calcChemPotCheby(SDFT_STODFT *st) {
   printf("Correct Chemical Potential is %.16lg\n", mu);
 }
 
+void calcKSPot(SDFT_STODFT *st) {
+  double en[4];
+  SDFT_CALL("calcKSPot", sdft_hip_ks_potential(st->ctx, st->rhoUp, SDFT_RHO_FFTW, &st->vextr[1],
+                                               &st->vexti[1], st->volume, st->vxcType, en,
+                                               st->vKsUp));
+  st->ehart = en[0];
+  st->eext = en[1];
+  st->exc = en[2];
+  st->muxc = en[3];
+}
+
 void calcRhoStoHybridCheby(SDFT_STODFT *st) {
   SDFT_CALL("calcRhoStoHybridCheby",
             sdft_hip_density(st->ctx, st->numStateUpProc, &st->stoWfUpRe[0][1],

@@ -18,6 +18,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 RHO_FFTW = 0
 RHO_GENERIC = 1
+XC_PZ_LDA, XC_PW_LDA, XC_PADE_LDA = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -43,6 +44,7 @@ EXPORTS = [
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
     "sdft_hip_spectral_bound", "sdft_hip_newton_filter", "sdft_hip_tune", "sdft_hip_set_config",
+    "sdft_hip_ks_init", "sdft_hip_ks_potential",
 ]
 
 _LIB = None
@@ -102,6 +104,9 @@ def load(path: str | None = None):
                                     C.c_double, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
         "sdft_hip_tune": ([vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _dp], C.c_int),
         "sdft_hip_set_config": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_ks_init": ([vp, C.POINTER(GridDesc)], C.c_int),
+        "sdft_hip_ks_potential": ([vp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp],
+                                  C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -215,6 +220,31 @@ class Context:
         if n < 0:
             _check(n)
         return (int(cfg[0]), int(cfg[1]), int(cfg[2])), ms[:n].copy()
+
+    def ks_init(self, dens):
+        """sdft_hip_ks_init: the density plane-wave sphere (systems.DensitySphere)."""
+        self._dmap = np.ascontiguousarray(dens.map, dtype=np.int32)
+        self._dmapc = np.ascontiguousarray(dens.map_conj, dtype=np.int32)
+        self._dak2 = np.ascontiguousarray(dens.ak2, dtype=np.float64)
+        d = GridDesc(dens.n[0], dens.n[1], dens.n[2], dens.ncoef,
+                     self._dmap.ctypes.data_as(_ip), self._dmapc.ctypes.data_as(_ip),
+                     self._dak2.ctypes.data_as(_dp))
+        _check(self.lib.sdft_hip_ks_init(self.h, C.byref(d)))
+
+    def ks_potential(self, rho, vext, volume, xc=0, layout=RHO_FFTW):
+        """sdft_hip_ks_potential: v_KS from the density (becomes the context's
+        v_KS).  Returns (v_KS in `layout`, dict(eh, eext, exc, muxc))."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        if rho.size != self.sys.ngrid or np.size(vext) != len(getattr(self, "_dak2", ())):
+            raise SdftError("ks_potential: rho must have n^3 points and vext one value per "
+                            "density-sphere coefficient (after ks_init)")
+        vr = np.ascontiguousarray(np.real(vext), dtype=np.float64)
+        vi = np.ascontiguousarray(np.imag(vext), dtype=np.float64)
+        en = np.zeros(4)
+        out = np.empty_like(rho)
+        _check(self.lib.sdft_hip_ks_potential(self.h, _ptr(rho), layout, _ptr(vr), _ptr(vi),
+                                              float(volume), xc, _ptr(en), _ptr(out)))
+        return out, dict(eh=en[0], eext=en[1], exc=en[2], muxc=en[3])
 
     def set_config(self, plane_tile, kb_wg_target=512, fused_expand=1):
         _check(self.lib.sdft_hip_set_config(self.h, plane_tile, kb_wg_target, fused_expand))

@@ -357,6 +357,48 @@ def build_system(name: str, a0: float, rep: int, n: int, g2max: float = 12.0,
                   proj_l=pl, proj_m=pm, h_kb=h_kb)
 
 
+@dataclasses.dataclass
+class DensitySphere:
+    """The density plane-wave half sphere |G|^2 <= 4 g2max (PINY's large
+    density grid package): the G-set of rho(G), v_H(G) and v_ext(G)."""
+    n: tuple
+    k: np.ndarray
+    map: np.ndarray
+    map_conj: np.ndarray
+    ak2: np.ndarray
+    gvec: np.ndarray
+
+    @property
+    def ncoef(self) -> int:
+        return int(self.k.shape[0])
+
+
+def density_sphere(sysm: System, g2max: float = 12.0) -> DensitySphere:
+    """Half sphere of the density cutoff (4x the wave-function cutoff in
+    energy) on the system's grid, G = 0 last, with its FFTW-order maps."""
+    k = half_sphere(sysm.L, 4.0 * g2max)
+    if np.any(2 * np.abs(k).max(axis=0) >= np.array(sysm.n)):
+        raise ValueError("grid too small for the density sphere")
+    m, mc = fftw_maps(k, sysm.n)
+    gvec = k.astype(np.float64) * (2.0 * math.pi / sysm.L)
+    return DensitySphere(n=tuple(sysm.n), k=k, map=m, map_conj=mc, ak2=(gvec ** 2).sum(axis=1),
+                         gvec=gvec)
+
+
+def vloc_density(sysm: System, dens: DensitySphere) -> np.ndarray:
+    """v_ext(G) on the density sphere: the erf-screened Coulomb local
+    potential of the atoms, -4 pi Z / (V G^2) exp(-G^2 rc^2 / 4) S(G), zero
+    at G = 0 (the same local potential ks_potential puts into v_KS)."""
+    V = sysm.volume
+    S = _structure_factor(dens.k.astype(np.float64), sysm.L, sysm.pos)
+    g2 = dens.ak2
+    out = np.zeros(dens.ncoef, dtype=np.complex128)
+    nz = g2 > 0
+    out[nz] = (-4.0 * math.pi * sysm.z_val / (V * g2[nz]) * np.exp(-g2[nz] * sysm.rc_loc ** 2 / 4.0)
+               * S[nz])
+    return out
+
+
 def noise_orbitals(sysm: System, nstate: int, seed: int, first: int = 0) -> np.ndarray:
     """Stochastic orbitals chi_s(G), complex (nstate, ncoef), G=0 last.
 

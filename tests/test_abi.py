@@ -50,7 +50,7 @@ def test_host_shim_exports_reference_entry_points(lib_path):
     names = sorted(set(re.findall(r"\b(?:void|double)\s+(\w+)\s*\(SDFT_STODFT", src)))
     assert names == sorted(["filterChebyPolyHerm", "normHCheby", "calcCoefForceWrapSCF",
                             "calcChebyCoeff", "calcChebyMoments", "calcNumElecCheby",
-                            "calcChemPotCheby", "calcRhoStoHybridCheby",
+                            "calcChemPotCheby", "calcRhoStoHybridCheby", "calcKSPot",
                             "filterNewtonPolyHerm", "genEnergyMax", "genEnergyMin"])
     host = ctypes.CDLL(build.host_lib_path())
     assert all(hasattr(host, n) for n in names)

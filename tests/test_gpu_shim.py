@@ -28,7 +28,10 @@ class SdftStodft(C.Structure):
                 ("expanCoeff", _dp), ("chebyMomentsUp", _dp), ("chemPot", _dp), ("rhoUp", _dp),
                 ("numChemPot", C.c_int), ("newtonScale", C.c_double), ("sampPoint", _dp),
                 ("testWfMaxRe", _dp), ("testWfMaxIm", _dp), ("testWfMinRe", _dp),
-                ("testWfMinIm", _dp)]
+                ("testWfMinIm", _dp),
+                ("vextr", _dp), ("vexti", _dp), ("volume", C.c_double), ("vxcType", C.c_int),
+                ("ehart", C.c_double), ("eext", C.c_double), ("exc", C.c_double),
+                ("muxc", C.c_double), ("vKsUp", _dp)]
 
 
 def one_based(a):
@@ -42,7 +45,8 @@ def shim():
     from sdft_amd import build
     lib = C.CDLL(build.host_lib_path())
     for f in ("filterChebyPolyHerm", "calcCoefForceWrapSCF", "calcChebyMoments", "calcChemPotCheby",
-              "calcRhoStoHybridCheby", "filterNewtonPolyHerm", "genEnergyMax", "genEnergyMin"):
+              "calcRhoStoHybridCheby", "filterNewtonPolyHerm", "genEnergyMax", "genEnergyMin",
+              "calcKSPot"):
         getattr(lib, f).argtypes = [C.POINTER(SdftStodft)]
         getattr(lib, f).restype = None
     lib.normHCheby.argtypes = [C.POINTER(SdftStodft), C.c_int, C.c_int]
@@ -162,3 +166,36 @@ def test_shim_newton_filter_and_spectral_range(shim, si8_system):
     e_r, v_r, _ = O.gen_energy_bound(v0, s, 0, conv=1e-5)
     assert abs(st.energyMax - (e_r + 0.1)) < 1e-9 * abs(e_r)
     assert np.max(np.abs(vre + 1j * vim - v_r)) < 1e-8
+
+
+def test_shim_ks_potential(shim, si8_system):
+    """calcKSPot through the shim: density from calcRhoStoHybridCheby's
+    convention, 1-based vext, energies and v_ks_up against the oracle, and
+    the context's H then uses the new potential."""
+    import dataclasses
+    from sdft_amd import hip, systems
+    print("case: host shim calcKSPot si8", flush=True)
+    s = si8_system
+    ctx = hip.Context(s)
+    dens = systems.density_sphere(s)
+    ctx.ks_init(dens)
+    vext = systems.vloc_density(s, dens)
+    chi = systems.noise_orbitals(s, 4, seed=9)
+    rho = O.density_real(chi, s, 0.5)
+    ref, en = O.ks_potential(rho, vext, dens, s.volume, O.XC_PW_LDA)
+    vr = np.ascontiguousarray(vext.real)
+    vi = np.ascontiguousarray(vext.imag)
+    vks = np.zeros(s.ngrid)
+    st = SdftStodft(ctx=ctx.h, prec=0, rhoUp=rho.ctypes.data_as(_dp), vextr=one_based(vr),
+                    vexti=one_based(vi), volume=s.volume, vxcType=O.XC_PW_LDA,
+                    vKsUp=vks.ctypes.data_as(_dp))
+    shim.calcKSPot(C.byref(st))
+    assert np.max(np.abs(vks - ref)) < 1e-10 * np.max(np.abs(ref))
+    for k, v in (("eh", st.ehart), ("eext", st.eext), ("exc", st.exc), ("muxc", st.muxc)):
+        assert abs(v - en[k]) <= 1e-10 * abs(en[k]), k
+    s2 = dataclasses.replace(s, vks=ref)
+    c4 = systems.noise_orbitals(s, 4, seed=2)
+    f = ctx.hpsi_raw(c4)
+    fr = O.hpsi_raw(c4, s2)
+    assert np.max(np.abs(f - fr)) < 1e-11 * np.max(np.abs(fr))
+    ctx.close()

@@ -475,3 +475,70 @@ def test_newton_filter_matches_chebyshev_filter(tiny_system):
     out, _ = O.newton_filter(chi, a, z, s, scale, emean)
     acc, _ = O.filter_cheby(chi, O.cheby_coeffs(L, emin, emax, mu, beta, 2), s, scale, emean)
     assert np.max(np.abs(out[0] - acc)) < 1e-7 * np.max(np.abs(acc))
+
+
+# ---- Kohn-Sham potential from the density -----------------------------------
+
+CA_EC = {1.0: -0.0600, 2.0: -0.0448, 5.0: -0.0281, 10.0: -0.0186}  # Ceperley-Alder (Ha)
+
+
+@pytest.mark.parametrize("kind", [O.XC_PZ_LDA, O.XC_PW_LDA, O.XC_PADE_LDA])
+def test_lda_correlation_matches_ceperley_alder(kind):
+    """eps_xc - eps_x of each LDA at rs = 1, 2, 5, 10 against the
+    Ceperley-Alder correlation energies that all three parametrise (the
+    exchange part enters through the closed form -(3/4)(3 rho/pi)^(1/3), so a
+    wrong exchange term fails here too)."""
+    rs = np.array(sorted(CA_EC))
+    rho = 3.0 / (4.0 * math.pi * rs ** 3)
+    _, e, _ = O.xc_lda(rho, kind)
+    ex = -0.75 * np.cbrt(3.0 * rho / math.pi)
+    ec = e / rho - ex
+    assert np.all(np.abs(ec - np.array([CA_EC[r] for r in rs])) < 1.0e-3), ec
+
+
+@pytest.mark.parametrize("kind", [O.XC_PZ_LDA, O.XC_PW_LDA, O.XC_PADE_LDA])
+def test_lda_potential_is_energy_derivative(kind):
+    """v_xc = d(rho eps_xc)/d rho (central differences) on both PZ branches."""
+    rho = np.array([1.0e-3, 0.01, 0.05, 0.2, 0.9, 3.0])
+    v, _, _ = O.xc_lda(rho, kind)
+    h = 1.0e-6 * rho
+    _, ep, _ = O.xc_lda(rho + h, kind)
+    _, em, _ = O.xc_lda(rho - h, kind)
+    assert np.allclose(v, (ep - em) / (2.0 * h), rtol=1e-6, atol=1e-9)
+
+
+def test_pz_branches_meet_at_rs_one():
+    """The two Perdew-Zunger branches join at rs = 1 (to the 1e-4 of the
+    published parametrisation), energy and potential."""
+    rho1 = 3.0 / (4.0 * math.pi)
+    d = 1.0e-9
+    v, e, _ = O.xc_lda(np.array([rho1 * (1 - d), rho1 * (1 + d)]), O.XC_PZ_LDA)
+    assert abs(e[0] - e[1]) / rho1 < 1.0e-4 and abs(v[0] - v[1]) < 1.0e-3
+
+
+def test_ks_potential_plane_wave_density(si8_system):
+    """rho = rho0 + a cos(G0.r): Hartree energy V pi a^2 / G0^2, Hartree
+    potential 4 pi a cos(G0.r) / G0^2, external energy of vext = b at +-G0
+    equal to b a V (closed forms of the periodic Poisson problem)."""
+    from sdft_amd import systems
+    s = si8_system
+    dens = systems.density_sphere(s)
+    n = s.n[0]
+    V = s.volume
+    idx = np.arange(n)
+    ia, ib, ic = np.meshgrid(idx, idx, idx, indexing="ij")
+    k0 = np.array([1, 2, 0])
+    g0 = 2.0 * math.pi * k0 / s.L
+    ph = 2.0 * math.pi * (k0[0] * ia + k0[1] * ib + k0[2] * ic) / n
+    rho0, a, b = 0.03, 0.004, 0.25
+    rho_phys = (rho0 + a * np.cos(ph)).ravel()
+    j = int(np.nonzero((dens.k == k0).all(axis=1))[0][0])
+    vext = np.zeros(dens.ncoef, dtype=np.complex128)
+    vext[j] = b
+    vks, en = O.ks_potential(rho_phys * V, vext, dens, V, O.XC_PZ_LDA)
+    g2 = float(g0 @ g0)
+    assert abs(en["eh"] - V * math.pi * a * a / g2) < 1e-12 * V
+    assert abs(en["eext"] - b * a * V) < 1e-12 * V
+    vxc, _, _ = O.xc_lda(rho_phys, O.XC_PZ_LDA)
+    vh = 4.0 * math.pi * a * np.cos(ph).ravel() / g2
+    assert np.allclose(vks - vxc, vh + 2.0 * b * np.cos(ph).ravel(), atol=1e-12)
