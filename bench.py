@@ -212,15 +212,17 @@ def main():
     M = 2 * int(np.abs(sysm.k).max()) + 1
     bytes_, flops = plane_kernel_model(sysm, ncol, nst // 2, 8, 4, M)
     roofline = None
+    # the plane kernel's instantiation: tile variant, pruned passes when 4K < n
+    pruned = "true" if 4 * int(np.abs(sysm.k).max()) < wl.n else "false"
     if dom_name == "plane_bc_vpsi":
         gbs = bytes_ / avg_s / 1e9
         tfl = flops / avg_s / 1e12
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic("k_plane<float, %d, %d>" % (
-                        wl.n, tuned["fp32"]["plane_tile"]), wl),
-                    "kernel": "k_plane<float,%d,%d> (b/c transforms + v.psi)" % (
-                        wl.n, tuned["fp32"]["plane_tile"]),
+                    "traffic": measured_traffic("k_plane<float, %d, %d, %s>" % (
+                        wl.n, tuned["fp32"]["plane_tile"], pruned), wl),
+                    "kernel": "k_plane<float,%d,%d,%s> (b/c transforms + v.psi)" % (
+                        wl.n, tuned["fp32"]["plane_tile"], pruned),
                     "avg_launch_ms": round(avg_s * 1e3, 4),
                     "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
                     "achieved_tflops": round(tfl, 2),
