@@ -300,6 +300,17 @@ int sdft_hip_selftest_copy(long long nbytes, int reps);
  * muxc sum as cpGetVksStodft / excpot_* accumulate them; vks_out (nullable):
  * v_KS in `layout`. */
 int sdft_hip_ks_init(sdft_ctx *ctx, const sdft_grid_desc *dens);
+/* Local pseudopotential Hellmann-Feynman forces from the density of the last
+ * sdft_hip_ks_potential call: the local part of calcEnergyForce
+ * (stodft/calc-nuclei-force.c:254 -> calcLocExtPostScf,
+ * stodft/reduced-energy/energy-wrapper-post-scf.c:140-220 -> control_ewd_loc),
+ * without the ion-ion Ewald term.  With v_ext(G) = sum_a f_type(a)(G)
+ * e^{-iG.R_a}: F_a = -dE_ext/dR_a.  pos [natm][3] (bohr), atom_type [natm]
+ * in [0, ntype), vloc [ntype][ncoef] the real form factors f(G) and gvec
+ * [ncoef][3] the cartesian G, both in the density sphere's order; forces
+ * [natm][3] (Ha/bohr). */
+int sdft_hip_local_forces(sdft_ctx *ctx, int natm, const double *pos, const int *atom_type,
+                          int ntype, const double *vloc, const double *gvec, double *forces);
 int sdft_hip_ks_potential(sdft_ctx *ctx, const double *rho, int layout, const double *vext_re,
                           const double *vext_im, double volume, int xc, double *energies,
                           double *vks_out);

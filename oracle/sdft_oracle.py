@@ -26,6 +26,8 @@ Parity status of each function (how it is pinned; see tests/):
   density_real                              : sum(rho)/N = occ <chi|chi>/Nsto
       identity and the brute-force real-space sum
   energy_ke / energy_nl / nl_forces         : closed forms and FD derivatives
+  local_forces                              : central differences of E_ext
+      (ks_potential's external energy) w.r.t. the atom positions
   xc_lda / ks_potential                     : published correlation energies
       (Ceperley-Alder points), exchange closed form, PZ branch continuity,
       v_xc = d(rho eps_xc)/d rho by finite differences, Hartree energy and
@@ -659,3 +661,23 @@ def ks_potential(rho_raw, vext, dens, volume, kind):
     vscale = volume / N
     out = dict(eh=eh, eext=eext, exc=float(ed.sum() * vscale), muxc=float(md.sum() * vscale), Q=Q)
     return v_r + vxc, out
+
+
+def local_forces(Q, dens, pos, vloc_type, atom_type):
+    """Local pseudopotential Hellmann-Feynman forces from rho(G): the local
+    part of calcEnergyForce (stodft/calc-nuclei-force.c:254 ->
+    calcLocExtPostScf, reduced-energy/energy-wrapper-post-scf.c:140-220 ->
+    control_ewd_loc, without the ion-ion Ewald term).  With v_ext(G) =
+    sum_a f_type(a)(G) e^{-iG.R_a} and E_ext = sum_G Re(v_ext(G) conj Q(G)),
+    F_a = -dE_ext/dR_a = 2 sum_{half sphere, G != 0} G f(G) (cos(G.R_a) Im Q
+    + sin(G.R_a) Re Q).  Q: rho(G) on the density sphere (ks_potential's
+    out['Q']), vloc_type: (ntype, ncoef) real form factors, pos (natm, 3)."""
+    G = np.asarray(dens.gvec)[:-1]
+    q = np.asarray(Q)[:-1]
+    F = np.zeros((pos.shape[0], 3))
+    for a in range(pos.shape[0]):
+        f = np.asarray(vloc_type)[atom_type[a]][:-1]
+        ph = G @ pos[a]
+        t = f * (np.cos(ph) * q.imag + np.sin(ph) * q.real)
+        F[a] = 2.0 * (G * t[:, None]).sum(axis=0)
+    return F

@@ -273,7 +273,7 @@ static void free_ctx(Ctx& c) {
     delete c.dens;
     c.dens = nullptr;
   }
-  void* ptrs[] = {c.ks_field, c.ks_rho, c.ks_vhe, c.ks_part, c.ks_io,c.t.col_pos,  c.t.rowcol,     c.t.ent_off,   c.t.ent_j,      c.t.ent_b,
+  void* ptrs[] = {c.ks_field, c.ks_rho, c.ks_vhe, c.ks_part, c.ks_io, c.ks_q,c.t.col_pos,  c.t.rowcol,     c.t.ent_off,   c.t.ent_j,      c.t.ent_b,
                   c.t.chunk_col,
                   c.t.chunk_coef, c.t.coef_plus, c.t.coef_minus, c.t.ak2,       c.t.tw64,
                   c.t.tw32,     c.vks64,        c.vks32,       c.A64,          c.A32,
@@ -1484,6 +1484,9 @@ int sdft_hip_ks_init(sdft_ctx* h, const sdft_grid_desc* d) {
   if (!c.ks_rho && (rc = dalloc(&c.ks_rho, sizeof(double) * N))) return rc;
   if (!c.ks_vhe && (rc = dalloc(&c.ks_vhe, sizeof(double) * N))) return rc;
   if (!c.ks_io && (rc = dalloc(&c.ks_io, sizeof(double) * N))) return rc;
+  if (c.ks_q) SDFT_HIP_CHECK(hipFree(c.ks_q));
+  c.ks_q = nullptr;
+  if ((rc = dalloc(&c.ks_q, sizeof(double) * 2 * dc->g.ncp))) return rc;
   if (c.ks_part_n < 4 * nb) {
     if (c.ks_part) SDFT_HIP_CHECK(hipFree(c.ks_part));
     c.ks_part = nullptr;
@@ -1527,6 +1530,10 @@ int sdft_hip_ks_potential(sdft_ctx* h, const double* rho, int layout, const doub
   if (!rc)
     rc = launch_contract(dc, 0, kModeNoise, dc.W, 1, b.X0, b.X0, b.chi, nullptr, nullptr, sp,
                          nullptr);
+  // rho(G) kept for sdft_hip_local_forces
+  if (!rc)
+    SDFT_HIP_CHECK(hipMemcpyAsync(c.ks_q, b.X0, sizeof(double) * 2 * dc.g.ncp,
+                                  hipMemcpyDeviceToDevice, s));
   // v(G) = vext + Hartree (row 0 of X1; row 1 stays zero)
   if (!rc) rc = block_put(b, b.X1, vext_re, vext_im);
   int nb1 = 0, nb2 = 0;
@@ -1555,6 +1562,7 @@ int sdft_hip_ks_potential(sdft_ctx* h, const double* rho, int layout, const doub
   block_free(b);
   if (rc) return rc;
   c.have_vks = true;
+  c.ks_have_q = true;
   if (energies) {
     double eh = 0.0, ee = 0.0, ex = 0.0, mx = 0.0;
     for (int i = 0; i < nb1; ++i) {
@@ -1572,6 +1580,62 @@ int sdft_hip_ks_potential(sdft_ctx* h, const double* rho, int layout, const doub
     energies[3] = mx * vscale;
   }
   return 0;
+}
+
+int sdft_hip_local_forces(sdft_ctx* h, int natm, const double* pos, const int* atom_type,
+                          int ntype, const double* vloc, const double* gvec, double* forces) {
+  if (!h || natm < 1 || ntype < 1 || !pos || !atom_type || !vloc || !gvec || !forces) {
+    set_error("bad local force arguments");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  if (!c.dens || !c.ks_have_q) {
+    set_error("no density on the device (call sdft_hip_ks_potential first)");
+    return SDFT_ESTATE;
+  }
+  for (int a = 0; a < natm; ++a)
+    if (atom_type[a] < 0 || atom_type[a] >= ntype) {
+      set_error("atom type out of range");
+      return SDFT_EINVAL;
+    }
+  Ctx& dc = *c.dens;
+  const int nc = dc.g.ncoef, ncp = dc.g.ncp;
+  std::vector<double> g((size_t)3 * ncp, 0.0), f((size_t)ntype * ncp, 0.0);
+  for (int j = 0; j < nc; ++j) {
+    const int src = dc.g.perm[j];
+    for (int d = 0; d < 3; ++d) g[(size_t)d * ncp + j] = gvec[(size_t)src * 3 + d];
+    for (int t = 0; t < ntype; ++t) f[(size_t)t * ncp + j] = vloc[(size_t)t * nc + src];
+  }
+  double *dg = nullptr, *df = nullptr, *dp = nullptr, *dF = nullptr;
+  int* dt = nullptr;
+  int rc = dalloc(&dg, sizeof(double) * g.size());
+  if (!rc) rc = dalloc(&df, sizeof(double) * f.size());
+  if (!rc) rc = dalloc(&dp, sizeof(double) * 3 * natm);
+  if (!rc) rc = dalloc(&dF, sizeof(double) * 3 * natm);
+  if (!rc) rc = dalloc(&dt, sizeof(int) * natm);
+  hipStream_t s = dc.stream;
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = hipMemcpyAsync(dg, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(df, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dp, pos, sizeof(double) * 3 * natm, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dt, atom_type, sizeof(int) * natm, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "local force upload", __FILE__, __LINE__);
+  }
+  if (!rc) rc = launch_local_forces(s, c.ks_q, ncp, nc, dg, df, dp, dt, natm, dF);
+  if (!rc) {
+    e = hipMemcpyAsync(forces, dF, sizeof(double) * 3 * natm, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "local force download", __FILE__, __LINE__);
+  }
+  (void)hipStreamSynchronize(s);
+  void* ptrs[] = {dg, df, dp, dF, dt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  return rc;
 }
 
 int sdft_hip_selftest_copy(long long nbytes, int reps) {

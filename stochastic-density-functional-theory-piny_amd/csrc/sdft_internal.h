@@ -136,6 +136,8 @@ struct Ctx {
   double* ks_vhe = nullptr;     // [N] v_H + v_ext in real space
   double* ks_part = nullptr;    // per-block energy partials
   double* ks_io = nullptr;      // [N] the caller's density / v_KS staging
+  double* ks_q = nullptr;       // rho(G) of the last call (density sphere, internal order)
+  bool ks_have_q = false;
   size_t ks_part_n = 0;
   // profiling
   bool profiling = false;
@@ -331,6 +333,9 @@ int launch_ks_xc(hipStream_t s, const double* vhe, const double* rho, long long 
                  double* vks, float* vks32, double* part, int* nblocks);
 int launch_ks_out(hipStream_t s, const double* vks, int layout, int n, long long N,
                   double* out);
+int launch_local_forces(hipStream_t s, const double* Q, int ncp, int ncoef, const double* g,
+                        const double* vloc, const double* pos, const int* type, int natm,
+                        double* F);
 int launch_noise_plane(Ctx& c, void* W, int npairs, unsigned long long seed, int first,
                        int nstate);
 int launch_grid_plane(Ctx& c, void* W, int npairs, const void* field);

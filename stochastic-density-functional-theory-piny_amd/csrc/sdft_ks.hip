@@ -178,6 +178,55 @@ __global__ void k_ks_out(const double* __restrict__ vks, int layout, int n, long
   out[o] = vks[i];
 }
 
+// Local pseudopotential Hellmann-Feynman force on atom a = blockIdx.x: with
+// v_ext(G) = sum_a f_type(a)(G) e^{-iG.R_a} and E_ext = sum_G Re(v_ext conj Q),
+// F_a = -dE/dR_a = 2 sum_{G != 0, half sphere} G f(G) (cos(G.R_a) Im Q +
+// sin(G.R_a) Re Q) (the local part of calcEnergyForce,
+// stodft/calc-nuclei-force.c:254 -> control_ewd_loc).  g: [3][ncp] cartesian
+// G, vloc: [ntype][ncp], both in the internal coefficient order.
+__global__ __launch_bounds__(256) void k_local_forces(const double* __restrict__ Q, int ncp,
+                                                      int ncoef, const double* __restrict__ g,
+                                                      const double* __restrict__ vloc,
+                                                      const double* __restrict__ pos,
+                                                      const int* __restrict__ type,
+                                                      double* __restrict__ F) {
+  __shared__ double sh[3][256];
+  const int a = blockIdx.x;
+  const double rx = pos[3 * a], ry = pos[3 * a + 1], rz = pos[3 * a + 2];
+  const double* f = vloc + (size_t)type[a] * ncp;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < ncoef - 1; j += 256) {
+    const double gx = g[j], gy = g[ncp + j], gz = g[2 * ncp + j];
+    double sn, cs;
+    sincos(gx * rx + gy * ry + gz * rz, &sn, &cs);
+    const double t = f[j] * (cs * Q[ncp + j] + sn * Q[j]);
+    acc[0] += t * gx;
+    acc[1] += t * gy;
+    acc[2] += t * gz;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[3 * a + d] = 2.0 * sh[d][0];
+}
+
+int launch_local_forces(hipStream_t s, const double* Q, int ncp, int ncoef, const double* g,
+                        const double* vloc, const double* pos, const int* type, int natm,
+                        double* F) {
+  hipLaunchKernelGGL(k_local_forces, dim3(natm), dim3(256), 0, s, Q, ncp, ncoef, g, vloc, pos,
+                     type, F);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_ks_field(hipStream_t s, const double* rho, int layout, int n, long long N,
                     double vol, void* field, double* rho_fftw) {
   hipLaunchKernelGGL(k_ks_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, rho, layout,

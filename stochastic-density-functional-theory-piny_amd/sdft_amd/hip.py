@@ -44,7 +44,7 @@ EXPORTS = [
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
     "sdft_hip_spectral_bound", "sdft_hip_newton_filter", "sdft_hip_tune", "sdft_hip_set_config",
-    "sdft_hip_ks_init", "sdft_hip_ks_potential",
+    "sdft_hip_ks_init", "sdft_hip_ks_potential", "sdft_hip_local_forces",
 ]
 
 _LIB = None
@@ -105,6 +105,7 @@ def load(path: str | None = None):
         "sdft_hip_tune": ([vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _dp], C.c_int),
         "sdft_hip_set_config": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "sdft_hip_ks_init": ([vp, C.POINTER(GridDesc)], C.c_int),
+        "sdft_hip_local_forces": ([vp, C.c_int, _dp, _ip, C.c_int, _dp, _dp, _dp], C.c_int),
         "sdft_hip_ks_potential": ([vp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp],
                                   C.c_int),
     }
@@ -245,6 +246,22 @@ class Context:
         _check(self.lib.sdft_hip_ks_potential(self.h, _ptr(rho), layout, _ptr(vr), _ptr(vi),
                                               float(volume), xc, _ptr(en), _ptr(out)))
         return out, dict(eh=en[0], eext=en[1], exc=en[2], muxc=en[3])
+
+    def local_forces(self, pos, atom_type, vloc, gvec):
+        """sdft_hip_local_forces on the density of the last ks_potential call:
+        (natm, 3) local pseudopotential forces."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        at = np.ascontiguousarray(atom_type, dtype=np.int32)
+        vl = np.ascontiguousarray(vloc, dtype=np.float64)
+        gv = np.ascontiguousarray(gvec, dtype=np.float64)
+        if vl.ndim != 2 or vl.shape[1] != len(getattr(self, "_dak2", ())) or gv.shape != (
+                vl.shape[1], 3) or pos.shape != (at.shape[0], 3):
+            raise SdftError("local_forces: shapes (natm,3), (natm,), (ntype,ncoef), (ncoef,3)")
+        out = np.zeros_like(pos)
+        _check(self.lib.sdft_hip_local_forces(self.h, pos.shape[0], _ptr(pos),
+                                              at.ctypes.data_as(_ip), vl.shape[0], _ptr(vl),
+                                              _ptr(gv), _ptr(out)))
+        return out
 
     def set_config(self, plane_tile, kb_wg_target=512, fused_expand=1):
         _check(self.lib.sdft_hip_set_config(self.h, plane_tile, kb_wg_target, fused_expand))

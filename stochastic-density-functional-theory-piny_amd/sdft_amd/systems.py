@@ -385,6 +385,18 @@ def density_sphere(sysm: System, g2max: float = 12.0) -> DensitySphere:
                          gvec=gvec)
 
 
+def vloc_form_factor(sysm: System, dens: DensitySphere) -> np.ndarray:
+    """Per-atom local form factor on the density sphere (one atom type):
+    f(G) = -4 pi Z / (V G^2) exp(-G^2 rc^2 / 4), zero at G = 0, so that
+    v_ext(G) = f(G) sum_a exp(-i G.R_a)."""
+    g2 = dens.ak2
+    out = np.zeros((1, dens.ncoef))
+    nz = g2 > 0
+    out[0, nz] = -4.0 * math.pi * sysm.z_val / (sysm.volume * g2[nz]) * np.exp(
+        -g2[nz] * sysm.rc_loc ** 2 / 4.0)
+    return out
+
+
 def vloc_density(sysm: System, dens: DensitySphere) -> np.ndarray:
     """v_ext(G) on the density sphere: the erf-screened Coulomb local
     potential of the atoms, -4 pi Z / (V G^2) exp(-G^2 rc^2 / 4) S(G), zero

@@ -120,3 +120,33 @@ def test_ks_rejects_bad_calls(hip):
     with pytest.raises(hip.SdftError):
         ctx.ks_potential(rho, np.zeros(dens.ncoef, complex), -1.0)
     ctx.close()
+
+
+@pytest.mark.parametrize("name,rep,n", [("si8", 1, 32), ("si64", 2, 64)])
+def test_local_forces_match_oracle(hip, name, rep, n):
+    """Local pseudopotential forces from the device density vs the oracle
+    (itself pinned by finite differences of E_ext), 1e-10 relative to the
+    largest component."""
+    from sdft_amd import systems
+    say("case: local forces %s" % name)
+    s = systems.build_system(name, 10.26, rep, n)
+    dens = systems.density_sphere(s)
+    vext = systems.vloc_density(s, dens)
+    rho = _density(s)
+    _, en = O.ks_potential(rho, vext, dens, s.volume, 0)
+    ff = systems.vloc_form_factor(s, dens)
+    types = np.zeros(s.natm, dtype=np.int32)
+    ref = O.local_forces(en["Q"], dens, s.pos, ff, types)
+    ctx = hip.Context(s, with_nl=False)
+    ctx.ks_init(dens)
+    with pytest.raises(hip.SdftError):  # no density on the device yet
+        ctx.local_forces(s.pos, types, ff, dens.gvec)
+    ctx.ks_potential(rho, vext, s.volume, xc=0)
+    F = ctx.local_forces(s.pos, types, ff, dens.gvec)
+    assert rel(F, ref) < 1e-10, rel(F, ref)
+    # two atom types (the same form factor scaled) exercise the type table
+    types2 = (np.arange(s.natm) % 2).astype(np.int32)
+    ff2 = np.concatenate([ff, 0.5 * ff])
+    ref2 = O.local_forces(en["Q"], dens, s.pos, ff2, types2)
+    assert rel(ctx.local_forces(s.pos, types2, ff2, dens.gvec), ref2) < 1e-10
+    ctx.close()

@@ -542,3 +542,31 @@ def test_ks_potential_plane_wave_density(si8_system):
     vxc, _, _ = O.xc_lda(rho_phys, O.XC_PZ_LDA)
     vh = 4.0 * math.pi * a * np.cos(ph).ravel() / g2
     assert np.allclose(vks - vxc, vh + 2.0 * b * np.cos(ph).ravel(), atol=1e-12)
+
+
+def test_local_forces_are_minus_energy_gradient(si8_system):
+    """local_forces = -dE_ext/dR_a by central differences of ks_potential's
+    external energy (the density held fixed, v_ext rebuilt at displaced
+    atoms)."""
+    import dataclasses
+    from sdft_amd import systems
+    s = si8_system
+    dens = systems.density_sphere(s)
+    chi = systems.noise_orbitals(s, 4, seed=17)
+    rho = O.density_real(chi, s, 0.5)
+    vext = systems.vloc_density(s, dens)
+    _, en = O.ks_potential(rho, vext, dens, s.volume, O.XC_PZ_LDA)
+    ff = systems.vloc_form_factor(s, dens)
+    F = O.local_forces(en["Q"], dens, s.pos, ff, np.zeros(s.natm, dtype=int))
+    h = 1.0e-4
+    for a, d in ((0, 0), (3, 1), (5, 2)):
+        e = []
+        for sgn in (1, -1):
+            p2 = s.pos.copy()
+            p2[a, d] += sgn * h
+            s2 = dataclasses.replace(s, pos=p2)
+            _, en2 = O.ks_potential(rho, systems.vloc_density(s2, dens), dens, s.volume,
+                                    O.XC_PZ_LDA)
+            e.append(en2["eext"])
+        fd = -(e[0] - e[1]) / (2.0 * h)
+        assert abs(F[a, d] - fd) < 1e-6 * max(1.0, abs(fd)), (a, d, F[a, d], fd)
