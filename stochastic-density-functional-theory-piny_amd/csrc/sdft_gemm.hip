@@ -376,8 +376,11 @@ int launch_nl_project(Ctx& c, int prec, const void* X, int ns) {
                   sizeof(double) * (size_t)nsplit * P * ns);
   if (rc) return rc;
   // Y is [ns][ppad] with zero padding columns: (re)allocated and cleared
-  // whenever it grows or the padded projector count changes
-  const size_t ybytes = sizeof(double) * (size_t)c.ppad * ((ns + 31) / 32 * 32);
+  // whenever it grows or the padded projector count changes.  Z ([P][ns],
+  // written below for every state) is allocated with the same padded row
+  // count, so it holds P * ns doubles whenever Y needs no reallocation.
+  const int ns_pad = (ns + 31) / 32 * 32;
+  const size_t ybytes = sizeof(double) * (size_t)c.ppad * ns_pad;
   if (c.Y_bytes < ybytes || c.Y_ppad != c.ppad) {
     if (c.Y) SDFT_HIP_CHECK(hipFree(c.Y));
     if (c.Y32) SDFT_HIP_CHECK(hipFree(c.Y32));
@@ -386,7 +389,7 @@ int launch_nl_project(Ctx& c, int prec, const void* X, int ns) {
     c.Y32 = nullptr;
     c.Y_bytes = 0;
     if (hipMalloc(&c.Y, ybytes) != hipSuccess || hipMalloc(&c.Y32, ybytes / 2) != hipSuccess ||
-        hipMalloc(&c.Z, sizeof(double) * (size_t)c.ppad * ns) != hipSuccess) {
+        hipMalloc(&c.Z, sizeof(double) * (size_t)c.ppad * ns_pad) != hipSuccess) {
       set_error("device allocation failed (KB projections)");
       return -3;
     }
