@@ -5,4 +5,5 @@ Submodules:
   hip     : ctypes binding of libsdft_hip.so (the C ABI in include/sdft_hip.h)
   stodft  : the reference's filter-path operations over the C ABI
   build   : in-tree build of the HIP and host libraries
+  pinyfiles : the reference's checkpoint, density and fragment-orbital text files
 """

@@ -64,3 +64,16 @@ def test_last_error_without_gpu(lib_path):
     lib.sdft_hip_init.restype = ctypes.c_int
     assert lib.sdft_hip_init(None, None) == -1
     assert b"null" in lib.sdft_hip_last_error()
+
+
+def test_host_library_exports_file_formats(lib_path):
+    """include/sdft_io.h: every declared sdft_io_* entry point is exported by
+    libsdft_host.so and bound in sdft_amd.pinyfiles."""
+    from sdft_amd import build, pinyfiles
+    src = open(os.path.join(REPO, "include", "sdft_io.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(sdft_io_\w+)\s*\(", src)))
+    assert names == sorted(pinyfiles.EXPORTS)
+    host = ctypes.CDLL(build.host_lib_path())
+    missing = [n for n in names if not hasattr(host, n)]
+    assert not missing, missing

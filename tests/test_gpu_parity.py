@@ -76,6 +76,21 @@ def test_hpsi_fp32_si8(hip, si8_ctx, si8_system):
     assert rel(f32, f_ref) < 2e-5
 
 
+def test_hpsi_state_counts_grow_and_shrink(hip, si8_system):
+    """One context applied to blocks of 2, 8, 40, 3 and 33 states in turn:
+    the KB projection buffers (split partials, Z = A X, the padded Y rows)
+    follow the state count, both where they are reallocated (8 -> 40) and
+    where the earlier allocation is reused (2 -> 8, 40 -> 3 -> 33)."""
+    from sdft_amd import systems
+    ctx = hip.Context(si8_system)
+    for i, nstate in enumerate((2, 8, 40, 3, 33)):
+        say(f"case: hpsi si8 state-count sequence nstate={nstate}")
+        chi = systems.noise_orbitals(si8_system, nstate, seed=40 + i)
+        ref = O.hpsi_raw(chi, si8_system)
+        assert rel(ctx.hpsi_raw(chi, hip.PREC_FP64), ref) < 1e-12, nstate
+        assert rel(ctx.hpsi_raw(chi, hip.PREC_FP32), ref) < 2e-5, nstate
+
+
 def test_hpsi_local_only(hip, si8_system):
     from sdft_amd import systems
     say("case: hpsi local-only si8")
@@ -261,6 +276,30 @@ def test_density_generic_layout_and_accumulate(hip, si8_ctx, si8_system):
     acc = rho_f.copy()
     si8_ctx.density(C, 0.4, layout=hip.RHO_FFTW, accumulate_into=acc)
     assert rel(acc, 2.0 * rho_f) < 1e-15
+
+
+def test_device_density_through_piny_files(hip, si8_ctx, si8_system, tmp_path):
+    """The device density (generic order) written as PINY's final-density
+    file and density checkpoint (host C) reads back to 16 digits: the
+    checkpoint holds rho * volume, as stodft/checkpointIO.c:100-106 writes
+    it; the filtered orbitals go through a frag-MO file the same way."""
+    from sdft_amd import pinyfiles, systems
+    say("case: device density -> PINY files")
+    C = systems.noise_orbitals(si8_system, 4, seed=71)
+    rho = si8_ctx.density(C, 0.5, layout=hip.RHO_GENERIC)
+    assert rel(rho, O.density_real(C, si8_system, 0.5).reshape(
+        (si8_system.n[0],) * 3).transpose(2, 1, 0).ravel()) < 1e-12
+    fin = tmp_path / "density-final"
+    pinyfiles.density_write(None, fin, rho)
+    assert rel(pinyfiles.density_read(fin, rho.size), rho) < 1e-15
+    vol = 270.0
+    ck = tmp_path / "density-checkpoint"
+    pinyfiles.checkpoint_write(ck, 3, 4, vol, rho, [rho], [])
+    got = pinyfiles.checkpoint_read(ck, 4, rho.size)
+    assert got["iscf"] == 3 and rel(got["rho_up"], rho * vol) < 1e-15
+    mo = tmp_path / "frag-MO-0"
+    pinyfiles.frag_mo_write(mo, [C])
+    assert rel(pinyfiles.frag_mo_read(mo, [C.shape])[0], C) < 1e-15
 
 
 @pytest.mark.parametrize("which", [0, 1])
