@@ -61,10 +61,14 @@ int chunk_cols_for(int n) {
 // consecutive butterflies of one line, a first pass stores with a stride of R
 // elements, which for 16-byte (double) elements puts a line's butterflies in
 // one LDS bank; consecutive threads on consecutive lines (stride n + 1)
-// avoid that.  For float the line-major order keeps the global gathers of
-// the c rows denser and measured faster.
+// avoid that.  For the plane's c rows in float the line-major order keeps
+// the global gathers of the rows denser and measured faster (kLineFast);
+// the a columns of the expand / contract chunks are transformed in LDS only,
+// and take the line-fast order in both precisions (kLineFastA).
 template <typename T>
 constexpr bool kLineFast = sizeof(T) == 8;
+template <typename T>
+constexpr bool kLineFastA = true;
 
 template <typename T, int N>
 size_t plane_lds_bytes(int M) {
@@ -143,7 +147,7 @@ __device__ __forceinline__ void expand_body(const ExpandArgs& a, int chunk, int 
     buf[(d - c0) * LD + ia] = cx(T(zr), T(zi));
   }
   __syncthreads();
-  fft_lines_map<T, N, +1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
+  fft_lines_map<T, N, +1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
   Cx<T>* Wq = W + (size_t)q * N * ncolp;
   for (int e = tid; e < nc * N; e += kThreads) {
     const int ra = e / nc;
@@ -533,7 +537,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
     buf[dl * LD + ra] = Wq[(size_t)ra * ncolp + c0 + dl];
   }
   __syncthreads();
-  fft_lines_map<T, N, -1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
+  fft_lines_map<T, N, -1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
 
   double dots[3] = {0.0, 0.0, 0.0};
   Cx<T>* Wn = reinterpret_cast<Cx<T>*>(a.Wnext);
@@ -689,7 +693,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
       if (ka > h || ka < -h) buf[dl * LD + ia] = zero;
     }
     __syncthreads();
-    fft_lines_map<T, N, +1, ML, kThreads, kLineFast<T>>(buf, nc, LD, tw, tid);
+    fft_lines_map<T, N, +1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
     Cx<T>* Wq2 = Wn + (size_t)q * N * ncolp;
     for (int e = tid; e < nc * N; e += kThreads) {
       const int ra = e / nc;
