@@ -333,15 +333,29 @@ SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, S
   __syncthreads();
 }
 
+// LDS slot of element pos of a line in the padded layout: one unused slot
+// after every 8 elements.  A radix-8 first pass over lines mapped line-major
+// (8 consecutive threads on the 8 butterflies of a line) stores element
+// 8 j + r from thread j; unpadded, those 8 threads hit 2 bank pairs (4-way
+// conflicts for 8-byte elements); padded (slot 9 j + r, line stride 72
+// slots) the threads of 2 (or 4) consecutive lines, 16 (or 32) threads, hit
+// distinct bank pairs.
+template <bool PAD8>
+SDFT_HD inline int lds_slot(int pos) {
+  return PAD8 ? pos + (pos >> 3) : pos;
+}
+
 // Passes FIRST..LAST (inclusive) of one transform: the first loads through
 // lf0, the last stores through sf1, the ones between go through the LDS lines
-// buf + line * ld.  ZIN / ZOUT prune the first / last pass (see fft_pass_io).
+// buf + line * ld (padded slots with PAD8).  ZIN / ZOUT prune the first /
+// last pass (see fft_pass_io).
 template <typename T, int N, int DIR, bool REV, int FIRST, int LAST, int MAXLINES, int NT,
-          bool LINE_FAST, bool ZIN = false, bool ZOUT = false, class LF, class SF>
+          bool LINE_FAST, bool ZIN = false, bool ZOUT = false, bool PAD8 = false, class LF,
+          class SF>
 SDFT_DEV inline void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* tw, int tid,
                                    LF& lf0, SF& sf1) {
-  auto lb = [&](int line, int pos) { return buf[line * ld + pos]; };
-  auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + pos] = x; };
+  auto lb = [&](int line, int pos) { return buf[line * ld + lds_slot<PAD8>(pos)]; };
+  auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + lds_slot<PAD8>(pos)] = x; };
   if constexpr (FIRST == LAST) {
     fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, ZOUT>(nlines, tw, tid, lf0,
                                                                           sf1);
@@ -349,8 +363,8 @@ SDFT_DEV inline void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* 
     fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, false>(nlines, tw, tid, lf0,
                                                                            sb);
     if constexpr (FIRST + 1 < LAST) {
-      fft_passes_io<T, N, DIR, REV, FIRST + 1, LAST - 1, MAXLINES, NT, LINE_FAST>(
-          buf, ld, nlines, tw, tid, lb, sb);
+      fft_passes_io<T, N, DIR, REV, FIRST + 1, LAST - 1, MAXLINES, NT, LINE_FAST, false, false,
+                    PAD8>(buf, ld, nlines, tw, tid, lb, sb);
     }
     fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST, false, ZOUT>(nlines, tw, tid, lb,
                                                                            sf1);

@@ -196,15 +196,26 @@ struct PlaneCfg {
   static constexpr int CB = V == 0 ? CB0 : largest_divisor_le(N, 2 * CB0 < N ? 2 * CB0 : N);
 };
 
+// The c rows of the plane buffer in float at n = 64 use the padded slot
+// layout (lds_slot<true>, row stride 72): their first passes run line-major
+// (for coalesced W gathers) and store with a stride of 8 elements.
+template <typename T, int N>
+struct PlaneRows {
+  static constexpr bool PAD = sizeof(T) == 4 && N == 64;
+  static constexpr int LDC = PAD ? N + N / 8 : Cfg<N>::LD;
+};
+
 template <typename T, int N, int V = 0>
 size_t plane2_lds_bytes(int M) {
-  return sizeof(Cx<T>) *
-         ((size_t)M * Cfg<N>::LD + (size_t)PlaneCfg<T, N, V>::CB * Cfg<N>::LD + N);
+  return sizeof(Cx<T>) * ((size_t)M * PlaneRows<T, N>::LDC +
+                          (size_t)PlaneCfg<T, N, V>::CB * Cfg<N>::LD + N);
 }
 
 template <typename T, int N, int V, bool PR>
 __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N, V>::CB;
+  constexpr bool PAD = PlaneRows<T, N>::PAD;
+  constexpr int LDC = PlaneRows<T, N>::LDC;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
@@ -212,8 +223,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   const int* __restrict__ rowcol = a.rowcol;
   const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
   const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
-  Cx<T>* Pl = reinterpret_cast<Cx<T>*>(smem_raw);
-  Cx<T>* Q = Pl + M * LD;
+  Cx<T>* Pl = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows, stride LDC
+  Cx<T>* Q = Pl + M * LDC;
   Cx<T>* tw = Q + CB * LD;
   const int tid = threadIdx.x;
   const int K = (M - 1) / 2;
@@ -236,10 +247,10 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       const T m = d >= 0 ? T(1) : T(0);
       return cx(x.x * m, x.y * m);
     };
-    Cx<T>* buf = Pl + r0 * LD;
-    auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LD + pos] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, kLineFast<T>, PR, false>(
-        buf, LD, nl, tw, tid, lw, sb);
+    Cx<T>* buf = Pl + r0 * LDC;
+    auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LDC + lds_slot<PAD>(pos)] = x; };
+    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, kLineFast<T>, PR, false, PAD>(
+        buf, LDC, nl, tw, tid, lw, sb);
   }
 
   // along b, CB columns rc at a time
@@ -248,7 +259,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     auto lp = [&](int l, int pos) {
       const int kb = pos <= N / 2 ? pos : pos - N;
       const bool in = kb >= -K && kb <= K;
-      const Cx<T> x = Pl[(in ? kb + K : 0) * LD + rc0 + l];
+      const Cx<T> x = Pl[(in ? kb + K : 0) * LDC + lds_slot<PAD>(rc0 + l)];
       return in ? x : zero;
     };
     auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
@@ -256,7 +267,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     auto vf = [&](int l, int pos) { return vplane[pos * N + rc0 + l]; };
     auto sp = [&](int l, int pos, Cx<T> x) {
       const int kb = pos <= N / 2 ? pos : pos - N;
-      if (kb >= -K && kb <= K) Pl[(kb + K) * LD + rc0 + l] = x;
+      if (kb >= -K && kb <= K) Pl[(kb + K) * LDC + lds_slot<PAD>(rc0 + l)] = x;
     };
     if constexpr (P >= 2)
       fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true, PR, false>(Q, LD, CB, tw, tid,
@@ -271,14 +282,14 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   for (int r0 = 0; r0 < M; r0 += ML) {
     const int nl = M - r0 < ML ? M - r0 : ML;
     const int* rc = rowcol + (size_t)r0 * N;
-    Cx<T>* buf = Pl + r0 * LD;
-    auto lb = [&](int l, int pos) { return buf[l * LD + pos]; };
+    Cx<T>* buf = Pl + r0 * LDC;
+    auto lb = [&](int l, int pos) { return buf[l * LDC + lds_slot<PAD>(pos)]; };
     auto sw = [&](int l, int pos, Cx<T> x) {
       const int d = rc[l * N + pos];
       if (d >= 0) Wp[d] = x;
     };
-    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, kLineFast<T>, false, PR>(
-        buf, LD, nl, tw, tid, lb, sw);
+    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, kLineFast<T>, false, PR, PAD>(
+        buf, LDC, nl, tw, tid, lb, sw);
   }
 }
 
