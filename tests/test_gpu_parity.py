@@ -369,6 +369,37 @@ def test_plane_tiles_and_tuner_si64(hip, si64_system):
     assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
 
 
+def test_filter_si64_full_degree(hip, si64_system):
+    """The headline workload's filter at its full degree (Si64, 64^3, L = 1024,
+    kT = 0.01 Ha) on one orbital pair of its noise (seed 1) against the
+    oracle's filterChebyPolyHerm: FP64 filtered orbitals and density within
+    1e-10 relative (the plain form's bar), the FP32 form within 1e-4 relative
+    (orbitals) and 1e-6 L1 per electron (density)."""
+    from sdft_amd import systems
+    say("case: filter si64 full degree L=1024, 2 orbitals (oracle ~30 s)")
+    ctx = hip.Context(si64_system)
+    emin, emax = bounds(si64_system)
+    L = 1024
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.2, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si64_system, 2, seed=1)
+    ref, _ = O.filter_cheby(chi, coeffs, si64_system, scale, emean)
+    rho_ref = O.density_real(ref, si64_system, 1.0)
+    ne = rho_ref.sum()
+    out64, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP64)
+    assert rel(out64, ref) < 1e-10, rel(out64, ref)
+    rho64 = ctx.density(out64, 1.0)
+    assert rel(rho64, rho_ref) < 1e-10, rel(rho64, rho_ref)
+    out32, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP32)
+    assert rel(out32, ref) < 1e-4, rel(out32, ref)
+    rho32 = ctx.density(out32, 1.0)
+    say("  si64 L=1024: FP64 orbitals %.3g density %.3g; FP32 orbitals %.3g density L1/e %.3g" % (
+        rel(out64, ref), rel(rho64, rho_ref), rel(out32, ref),
+        np.abs(rho32 - rho_ref).sum() / ne))
+    assert np.abs(rho32 - rho_ref).sum() / ne < 1e-6
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-4)])
 def test_filter_fused_and_separate_expand(hip, si8_system, si8_ctx, prec, tol):
     """The filter with the contract kernel writing the next step's
