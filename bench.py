@@ -32,6 +32,8 @@ for p in (REPO, PKG):
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # vector (and matrix) FP64
 FP32_VALU_PEAK_TFLOPS = 157.3
+FP32_MFMA_PEAK_TFLOPS = 157.3   # dense MFMA fp32
+FP64_MFMA_PEAK_TFLOPS = 78.6    # dense MFMA fp64
 
 
 def log(msg):
@@ -233,6 +235,24 @@ def main():
                     "frac": None, "traffic": None, "kernel": dom_name,
                     "avg_launch_ms": round(avg_s * 1e3, 4)}
 
+    # KB products on the MFMA units: algorithmic flops per launch (projection
+    # Z = A X and force F = Y At, each 2 x nproj x 2 ncoef x orbitals) over
+    # the event-timed launch, which runs overlapped with the FFT kernels on
+    # its own stream (so these are lower bounds of the standalone rate)
+    kb_flops = 2.0 * sysm.nproj * 2 * sysm.ncoef * nst
+    kb_mfma = {"flops_per_launch": kb_flops,
+               "note": "event time of each launch while overlapped with the FFT kernels "
+                       "on the main stream; kb_project includes its split-K reduction"}
+    for tag, peak in (("fp32", FP32_MFMA_PEAK_TFLOPS), ("fp64", FP64_MFMA_PEAK_TFLOPS)):
+        kt = results[tag]["kernels"]
+        for name in ("kb_project", "kb_force"):
+            if name in kt and kt[name][1] > 0:
+                ms = kt[name][0] / kt[name][1]
+                tf = kb_flops / (ms * 1e-3) / 1e12
+                kb_mfma["%s_%s" % (tag, name)] = {"ms": round(ms, 4), "tflops": round(tf, 2),
+                                                  "peak_tflops": peak,
+                                                  "frac": round(tf / peak, 4)}
+
     cpu = None
     if not args.no_cpu_baseline:
         workers = min(16, os.cpu_count() or 1)
@@ -271,6 +291,7 @@ def main():
         "kernels_fp32_ms": {k: round(v[0] / v[1], 4) for k, v in r32["kernels"].items()},
         "kernels_fp64_ms": {k: round(v[0] / v[1], 4) for k, v in r64["kernels"].items()},
         "roofline": roofline,
+        "kb_mfma": kb_mfma,
         "cpu_baseline": cpu,
         "fragment_batch": batch,
         "ks_potential": ks,
