@@ -57,27 +57,33 @@ def survey_bytes_per_hpsi(sysm, wc, wv):
     return 2.0 * N * wc + N * wv / 2.0 + 80.0 * sysm.ncoef
 
 
-def cpu_baseline(sysm, chi, coeffs, scale, emean, nsteps, workers):
+def cpu_baseline(sysm, chi, coeffs, scale, emean, nsteps, workers, runs=3, warmup=3):
     """Time the CPU FP64 oracle (same algorithm: pair-packed 3D FFTs, v.psi,
-    G-space KB, kinetic, recurrence, accumulate) for `nsteps` Chebyshev
-    steps over the whole orbital block."""
+    G-space KB, kinetic, recurrence, accumulate) over the whole orbital
+    block: `warmup` untimed H.psi (plans, caches), then `runs` runs of
+    `nsteps` Chebyshev steps from chi; returns the median rate and the total
+    timed seconds."""
     from oracle import sdft_oracle as O
     O.set_fft_workers(workers)
-    c = chi.copy()
-    t_prev = chi.copy()
-    acc = coeffs[0] * chi
-    # one untimed step (plans, caches)
-    f = O.hpsi_raw(c, sysm)
-    t0 = time.perf_counter()
-    for k in range(1, nsteps + 1):
-        f = O.hpsi_raw(c, sysm)
-        O.norm_h_cheby(c, f, t_prev, k, scale, emean)
-        if k > 1:
-            t_prev = c
-        c = f
-        acc = acc + coeffs[k] * c
-    dt = time.perf_counter() - t0
-    return chi.shape[0] * nsteps / dt, dt
+    for _ in range(warmup):
+        O.hpsi_raw(chi, sysm)
+    rates, total = [], 0.0
+    for _ in range(runs):
+        c = chi.copy()
+        t_prev = chi.copy()
+        acc = coeffs[0] * chi
+        t0 = time.perf_counter()
+        for k in range(1, nsteps + 1):
+            f = O.hpsi_raw(c, sysm)
+            O.norm_h_cheby(c, f, t_prev, k, scale, emean)
+            if k > 1:
+                t_prev = c
+            c = f
+            acc = acc + coeffs[k] * c
+        dt = time.perf_counter() - t0
+        rates.append(chi.shape[0] * nsteps / dt)
+        total += dt
+    return float(np.median(rates)), total
 
 
 def main():
@@ -87,7 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="si64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="timed Chebyshev steps per CPU-baseline run (3 runs, median)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 343-fragment batch measurement")
     ap.add_argument("--batch-only", action="store_true",
@@ -258,8 +265,9 @@ def main():
         workers = min(16, os.cpu_count() or 1)
         rate, dt = cpu_baseline(sysm, chi, coeffs, scale, emean, args.cpu_steps, workers)
         cpu = {"value": round(rate, 2), "unit": "H.psi/s", "cores": workers, "kind": "port",
-               "sample": "%d Chebyshev steps x %d orbitals of the same Si64 workload, FP64 numpy/"
-                         "scipy.fft oracle (%.1f s)" % (args.cpu_steps, nst, dt)}
+               "sample": "median of 3 runs of %d Chebyshev steps x %d orbitals of the same Si64 "
+                         "workload after 3 untimed H.psi, FP64 numpy/scipy.fft oracle (%.1f s "
+                         "timed)" % (args.cpu_steps, nst, dt)}
         log("cpu baseline: %.1f H.psi/s on %d threads" % (rate, workers))
 
     batch = None if args.no_batch else batch_bench(args, mu)
