@@ -182,8 +182,19 @@ def main():
         blk.close()
         log("%s: profiled filter %.2f ms (unprofiled %.2f ms)" % (tag, 1e3 * dt_prof,
                                                                  1e3 * dt / args.steps))
+        # the separate moments pass (calcChebyMoments: L/2 H.psi steps per
+        # orbital, moments by the doubling identities), one warm call and one
+        # timed call; includes the host -> device copy of the block (~12 MB)
+        ctx.cheby_moments(chi, L, scale, emean, prec)
+        t3 = time.perf_counter()
+        mom_p = ctx.cheby_moments(chi, L, scale, emean, prec)
+        dt_mom = time.perf_counter() - t3
+        mom_err = float(np.max(np.abs(mom_p - mom)) / np.max(np.abs(mom)))
         results[tag] = dict(ms_per_step=1e3 * dt / args.steps,
-                            hpsi_per_s=hpsi_per_step * args.steps / dt, filtered=out, kernels=kt)
+                            hpsi_per_s=hpsi_per_step * args.steps / dt, filtered=out, kernels=kt,
+                            moments=dict(ms=round(1e3 * dt_mom, 3),
+                                         hpsi_per_s=round(nst * (L // 2) / dt_mom, 1),
+                                         max_rel_diff_vs_fp64=mom_err))
         log("%s: %.2f ms/filter, %.4g H.psi/s; kernels %s" % (
             tag, results[tag]["ms_per_step"], results[tag]["hpsi_per_s"],
             {k: (round(v[0], 3), v[1]) for k, v in kt.items()}))
@@ -300,6 +311,10 @@ def main():
         "kernels_fp64_ms": {k: round(v[0] / v[1], 4) for k, v in r64["kernels"].items()},
         "roofline": roofline,
         "kb_mfma": kb_mfma,
+        "moments_pass": {"hpsi_per_pass": nst * (L // 2),
+                         "fp32": r32["moments"], "fp64": r64["moments"],
+                         "note": "calcChebyMoments over the block (L/2 H.psi steps per orbital), "
+                                 "host arrays in and out"},
         "cpu_baseline": cpu,
         "fragment_batch": batch,
         "ks_potential": ks,
