@@ -110,6 +110,15 @@ def checkpoint_write(path, iscf, ndiis, vol, rho_up, bank_up, err_up, rho_dn=Non
     lsda = rho_dn is not None
     bu = [_f64(b, n) for b in bank_up]
     eu = [_f64(e, n) for e in err_up]
+    # the C writer reads exactly these many pointers from each array
+    nbank = checkpoint_nbank(len(eu), ndiis) if len(eu) <= ndiis else -1
+    if nbank < 0 or len(bu) != nbank:
+        raise ValueError("checkpoint: %d error entries and %d bank entries do not fit a bank "
+                         "of %d (need %d stored steps <= %d and nbank entries)"
+                         % (len(eu), len(bu), ndiis, len(eu), ndiis))
+    if lsda and (bank_dn is None or err_dn is None or len(bank_dn) != nbank
+                 or len(err_dn) != len(eu)):
+        raise ValueError("checkpoint: the down-spin banks must match the up-spin counts")
     keep = [rho_up, bu, eu]
     args = [_ptrs(bu), _ptrs(eu)]
     rd = None

@@ -108,10 +108,23 @@ def test_checkpoint_errors(pf, tmp_path):
     with pytest.raises(pf.PinyFileError) as e:
         pf.checkpoint_header(tmp_path / "bad")
     assert e.value.code == pf.EFORMAT
-    # the writer refuses a stored count above the bank size
-    with pytest.raises(pf.PinyFileError) as e:
+    # the writer refuses a stored count above the bank size, and bank lists
+    # that do not match the stored count (before any pointer is read in C)
+    with pytest.raises(ValueError):
         pf.checkpoint_write(p, 1, 1, 1.0, np.ones(n), [np.ones(n)] * 2, [np.ones(n)] * 2)
-    assert e.value.code == pf.EINVAL
+    with pytest.raises(ValueError):
+        pf.checkpoint_write(p, 1, 4, 1.0, np.ones(n), [np.ones(n)] * 2, [np.ones(n)] * 2)
+    # the C entry point itself rejects a stored count above the bank size
+    import ctypes
+    lib = pf.load()
+    arr = (ctypes.POINTER(ctypes.c_double) * 2)()
+    x = np.ones(n)
+    rc = lib.sdft_io_checkpoint_write(str(p).encode(), 1, 2, 1, n, 1.0, 0,
+                                      x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None,
+                                      ctypes.cast(arr, ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+                                      ctypes.cast(arr, ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+                                      None, None)
+    assert rc == pf.EINVAL
 
 
 def test_density_log_and_final(pf, tmp_path):
