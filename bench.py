@@ -100,8 +100,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 343-fragment batch measurement (JSON line of its own)")
     ap.add_argument("--batch-config", default=None,
-                    help="fixed KB split,overlap,plane_tile for the batch (e.g. 2,0,0) instead of the "
-                         "autotuner")
+                    help="fixed KB split,overlap,plane_tile[,fused_expand] for the batch (e.g. "
+                         "2,0,0,1) instead of the autotuner")
     ap.add_argument("--config", default=None,
                     help="plane_tile,kb_wg_target,fused_expand for the single-system filter "
                          "instead of the tuner's pick")
@@ -395,13 +395,15 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
             b.noise(seed, int(f0))
         bat = hip.Batch(blocks)
         if args.batch_config:
-            split, ov, pv = (int(x) for x in args.batch_config.split(","))
-            bat.config(split, ov, pv)
+            cfg = [int(x) for x in args.batch_config.split(",")]
+            split, ov, pv = cfg[:3]
+            fu = cfg[3] if len(cfg) > 3 else 1
+            bat.config(split, ov, pv, fu)
             cand = np.zeros(0)
         else:
-            (split, ov, pv), cand = bat.autotune(steps=6)
-        log("%s autotune: split=%d overlap=%d plane_tile=%d candidates(ms/step)=%s" % (
-            tag, split, ov, pv, np.round(cand, 3).tolist()))
+            (split, ov, pv, fu), cand = bat.autotune(steps=6)
+        log("%s autotune: split=%d overlap=%d plane_tile=%d fused_expand=%d "
+            "candidates(ms/step)=%s" % (tag, split, ov, pv, fu, np.round(cand, 3).tolist()))
         steps = args.steps if prec == hip.PREC_FP32 else 1
         for _ in range(args.warmup):
             bat.filter(coeffs, scale, emean)
@@ -413,7 +415,7 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
         bat.filter(coeffs, scale, emean, use_graph=False)
         dt_direct = time.perf_counter() - t2
         res[tag] = dict(s_per_filter=dt, hpsi_per_s=hpsi / dt, direct_s=dt_direct,
-                        split=split, overlap=ov, plane_tile=pv, cand=cand)
+                        split=split, overlap=ov, plane_tile=pv, fused_expand=fu, cand=cand)
         out[tag] = [blocks[i].get(0) for i in sample]
         log("%s batch: %.3f s/filter (graph), %.3f s (direct), %.4g H.psi/s" % (
             tag, dt, dt_direct, hpsi / dt))
@@ -450,10 +452,10 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
         "s_per_filter_fp64": round(r64["s_per_filter"], 4),
         "fp32_over_fp64": round(r32["hpsi_per_s"] / r64["hpsi_per_s"], 3),
         "autotune_fp32": {"kb_split": r32["split"], "kb_overlap": r32["overlap"],
-                          "plane_tile": r32["plane_tile"],
+                          "plane_tile": r32["plane_tile"], "fused_expand": r32["fused_expand"],
                           "candidates_ms_per_step": np.round(r32["cand"], 4).tolist()},
         "autotune_fp64": {"kb_split": r64["split"], "kb_overlap": r64["overlap"],
-                          "plane_tile": r64["plane_tile"]},
+                          "plane_tile": r64["plane_tile"], "fused_expand": r64["fused_expand"]},
         "fp32_error_sample": {"fragments": sample,
                               "energy_per_atom_ha_max": float(max(de)),
                               "density_l1_per_electron_max": float(max(dl1))},

@@ -234,17 +234,23 @@ int sdft_hip_batch_filter(sdft_batch *bat, int L, const double *coeffs, const do
 /* Autotuner: times `steps` Chebyshev steps (after one warm-up step) of each
  * candidate launch configuration -- the plane kernel's b-line tile (0: base
  * sub-chunk, 1: twice as wide, where it fits LDS), whether the KB products
- * overlap the FFT kernels on a second stream, and the split of the KB
- * projection's reduction (1, 2, 4 or 8 slices) -- on the batch's current data
- * and keeps the fastest.  config_out (nullable, 3 ints) receives split,
- * overlap, plane tile; cand_ms (nullable) the per-candidate step times in ms
- * (at most max_cand).  Returns the number of candidates timed (16).  Leaves
- * the blocks' T_k buffers modified (the accumulators are untouched). */
+ * overlap the FFT kernels on a second stream, the split of the KB
+ * projection's reduction (1, 2, 4 or 8 slices) and, where every fragment
+ * allows it, whether the contract kernel writes the next step's expansion
+ * (fused expand) or an expand kernel runs per step -- on the batch's current
+ * data and keeps the fastest.  config_out (nullable, 4 ints) receives split,
+ * overlap, plane tile, fused expand; cand_ms (nullable) the per-candidate
+ * step times in ms (at most max_cand; loop order tile, overlap, split, fused
+ * expand, the last fastest).  Returns the number of candidates timed (32, or
+ * 16 when the expand cannot be fused).  Leaves the blocks' T_k buffers
+ * modified (the accumulators are untouched). */
 int sdft_hip_batch_autotune(sdft_batch *bat, int steps, int *config_out, int max_cand,
                             double *cand_ms);
 /* Set the launch configuration by hand (split in {1,2,4,8}, overlap 0/1,
- * plane tile 0/1). */
-int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap, int plane_tile);
+ * plane tile 0/1, fused expand 0/1; 1 needs every fragment's sphere columns
+ * to be ka intervals, else SDFT_EINVAL). */
+int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap, int plane_tile,
+                          int fused_expand);
 
 /* Launch-configuration tuner for one context (no reference counterpart):
  * times `steps` Chebyshev filter steps of an nstate-orbital block (device

@@ -116,8 +116,10 @@ struct Batch {
   std::vector<int> nl_frags;
   int split = 1;       // target slices of the projection's reduction
   bool overlap = true;
-  // every fragment's columns are ka intervals: the contract writes the next
-  // step's expansion, and only the first step launches the expand kernel
+  // fused_ok: every fragment's columns are ka intervals, so the contract can
+  // write the next step's expansion; fused: it does (only the first step then
+  // launches the expand kernel), chosen by the autotuner or batch_config
+  bool fused_ok = true;
   bool fused = true;
   int plane_variant = 0;  // plane-kernel tile variant (falls back to 0 per group)
   int tot_proj = 0, tot_red = 0, tot_force = 0;
@@ -492,7 +494,7 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     f.b = &blocks[i]->b;
     f.c = f.b->ctx;
     f.npairs = f.b->nstate_pad / 2;
-    if (!f.c->g.col_intervals) B.fused = false;
+    if (!f.c->g.col_intervals) B.fused_ok = B.fused = false;
     if ((rc = dmalloc(&f.W, w_bytes(*f.c, f.npairs, prec)))) return fail(rc);
     const int P = f.c->nproj, S = f.b->nstate_pad;
     if (P > 0) {
@@ -575,16 +577,23 @@ int sdft_hip_batch_destroy(sdft_batch* h) {
   return 0;
 }
 
-int sdft_hip_batch_config(sdft_batch* h, int split, int overlap, int plane_tile) {
+int sdft_hip_batch_config(sdft_batch* h, int split, int overlap, int plane_tile,
+                          int fused_expand) {
   if (!h || (split != 1 && split != 2 && split != 4 && split != 8) ||
-      (overlap != 0 && overlap != 1) || (plane_tile != 0 && plane_tile != 1)) {
+      (overlap != 0 && overlap != 1) || (plane_tile != 0 && plane_tile != 1) ||
+      (fused_expand != 0 && fused_expand != 1)) {
     set_error("bad batch configuration");
     return SDFT_EINVAL;
   }
   Batch& B = h->B;
+  if (fused_expand && !B.fused_ok) {
+    set_error("fused expand needs every fragment's sphere columns to be ka intervals");
+    return SDFT_EINVAL;
+  }
   B.split = split;
   B.overlap = overlap != 0;
   B.plane_variant = plane_tile;
+  B.fused = fused_expand != 0;
   B.graph_valid = false;
   return build_kb_tables(B);
 }
@@ -622,17 +631,21 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
   SDFT_HIP_CHECK(hipEventCreate(&t0));
   SDFT_HIP_CHECK(hipEventCreate(&t1));
   const int splits[4] = {1, 2, 4, 8};
+  const int nfu = B.fused_ok ? 2 : 1;
   double best = 1e300;
   int best_split = B.split, best_ov = B.overlap ? 1 : 0, best_pv = B.plane_variant;
+  bool best_fu = B.fused;
   int ncand = 0;
   int rc = 0;
   if ((rc = copy_chi(B))) return rc;
   for (int pv = 0; pv < 2 && !rc; ++pv)
-  for (int ov = 0; ov < 2 && !rc; ++ov) {
-    for (int si = 0; si < 4 && !rc; ++si) {
+  for (int ov = 0; ov < 2 && !rc; ++ov)
+  for (int si = 0; si < 4 && !rc; ++si) {
+    for (int fu = 0; fu < nfu && !rc; ++fu) {
       B.split = splits[si];
       B.overlap = ov != 0;
       B.plane_variant = pv;
+      B.fused = fu != 0;
       if ((rc = build_kb_tables(B))) break;
       if ((rc = build_args(B, L, co.data(), sc.data(), em.data()))) break;
       // warm-up: the first step (which expands chi) and one more, so that the
@@ -660,6 +673,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
         best_split = splits[si];
         best_ov = ov;
         best_pv = pv;
+        best_fu = fu != 0;
       }
     }
   }
@@ -668,6 +682,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
   B.split = best_split;
   B.overlap = best_ov != 0;
   B.plane_variant = best_pv;
+  B.fused = best_fu;
   B.graph_valid = false;
   int rc2 = build_kb_tables(B);
   if (rc) return rc;
@@ -676,6 +691,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
     config_out[0] = best_split;
     config_out[1] = best_ov;
     config_out[2] = best_pv;
+    config_out[3] = best_fu ? 1 : 0;
   }
   return ncand;
 }

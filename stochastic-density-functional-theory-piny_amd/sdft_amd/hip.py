@@ -95,7 +95,7 @@ def load(path: str | None = None):
         "sdft_hip_batch_destroy": ([vp], C.c_int),
         "sdft_hip_batch_filter": ([vp, C.c_int, _dp, _dp, _dp, C.c_int], C.c_int),
         "sdft_hip_batch_autotune": ([vp, C.c_int, _ip, C.c_int, _dp], C.c_int),
-        "sdft_hip_batch_config": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "sdft_hip_batch_config": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
                                           C.c_double], C.c_int),
         "sdft_hip_spectral_bound": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
@@ -460,13 +460,14 @@ class Batch:
                                               1 if use_graph else 0))
 
     def autotune(self, steps=8):
-        """Returns ((split, overlap, plane_tile), per-candidate ms/step)."""
-        cfg = np.zeros(3, dtype=np.int32)
+        """Returns ((split, overlap, plane_tile, fused_expand), per-candidate
+        ms/step)."""
+        cfg = np.zeros(4, dtype=np.int32)
         ms = np.zeros(32)
         n = self.lib.sdft_hip_batch_autotune(self.h, steps, cfg.ctypes.data_as(_ip), 32, _ptr(ms))
         if n < 0:
             _check(n)
         return tuple(int(x) for x in cfg), ms[:n].copy()
 
-    def config(self, split, overlap, plane_tile=0):
-        _check(self.lib.sdft_hip_batch_config(self.h, split, overlap, plane_tile))
+    def config(self, split, overlap, plane_tile=0, fused_expand=1):
+        _check(self.lib.sdft_hip_batch_config(self.h, split, overlap, plane_tile, fused_expand))

@@ -123,22 +123,28 @@ def test_batch_fp32_matches_oracle(hip, frags):
 
 
 def test_batch_autotune_and_configs(hip, frags):
-    say("case: batch autotune + every KB split / overlap / plane tile configuration, L=9")
+    say("case: batch autotune + every KB split / overlap / plane tile / fused-expand "
+        "configuration, L=9")
     L = 9
     co, scale, emean = _coeffs(frags, L)
     ctxs, blocks, bat = _make_batch(hip, frags, hip.PREC_FP64)
-    (split, ov, pv), ms = bat.autotune(steps=4)
-    assert split in (1, 2, 4, 8) and ov in (0, 1) and pv in (0, 1)
-    assert len(ms) == 16 and np.all(ms > 0)
+    (split, ov, pv, fu), ms = bat.autotune(steps=4)
+    assert split in (1, 2, 4, 8) and ov in (0, 1) and pv in (0, 1) and fu in (0, 1)
+    assert len(ms) == 32 and np.all(ms > 0)
     refs = [O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])[0]
             for f, (sysm, chi, _, _) in enumerate(frags)]
     for sp in (1, 2, 4, 8):
         for o in (0, 1):
             for t in (0, 1):
-                bat.config(sp, o, t)
-                bat.filter(co, scale, emean, use_graph=True)
-                for f in range(len(frags)):
-                    assert rel(blocks[f].get(0), refs[f]) < 1e-10, (sp, o, t, f)
+                for e in (0, 1):
+                    bat.config(sp, o, t, e)
+                    bat.filter(co, scale, emean, use_graph=True)
+                    for f in range(len(frags)):
+                        assert rel(blocks[f].get(0), refs[f]) < 1e-10, (sp, o, t, e, f)
+    with pytest.raises(hip.SdftError):
+        bat.config(3, 0, 0, 0)
+    with pytest.raises(hip.SdftError):
+        bat.config(1, 0, 0, 2)
     bat.close()
 
 
