@@ -540,6 +540,21 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
   const int tid = threadIdx.x;
   const int c0 = chunk_col[chunk];
   const int nc = chunk_col[chunk + 1] - c0;
+  // the unpack's first coefficient indices (and, for the fused expand, each
+  // column's sphere half-height) are loaded before the transform, so that
+  // their latency overlaps it
+  const int j0 = chunk_coef[chunk], j1 = chunk_coef[chunk + 1];
+  int pp_first = 0, pm_first = 0;
+  if (j0 + tid < j1) {
+    pp_first = coef_plus[j0 + tid];
+    pm_first = coef_minus[j0 + tid];
+  }
+  Cx<T>* Wn = reinterpret_cast<Cx<T>*>(a.Wnext);
+  const bool emit =
+      E && Wn != nullptr && (mode == kModeFilter || mode == kModeMoments || mode == kModeNewton);
+  __shared__ int col_h[E ? CH : 1];
+  if (E && emit && tid < nc)
+    col_h[tid] = (a.ent_off[c0 + tid + 1] - a.ent_off[c0 + tid] - 1) >> 1;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
   const Cx<T>* Wq = W + (size_t)q * N * ncolp;
   for (int e = tid; e < nc * N; e += kThreads) {
@@ -551,12 +566,9 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
   fft_lines_map<T, N, -1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
 
   double dots[3] = {0.0, 0.0, 0.0};
-  Cx<T>* Wn = reinterpret_cast<Cx<T>*>(a.Wnext);
-  const bool emit =
-      E && Wn != nullptr && (mode == kModeFilter || mode == kModeMoments || mode == kModeNewton);
-  const int j0 = chunk_coef[chunk], j1 = chunk_coef[chunk + 1];
-  for (int j = j0 + tid; j < j1; j += kThreads) {
-    const int pp = coef_plus[j], pm = coef_minus[j];
+  // (pp, pm) of coefficient j; the increment loads the next trip's pair
+  for (int j = j0 + tid, pp = pp_first, pm = pm_first; j < j1;
+       j += kThreads, pp = j < j1 ? coef_plus[j] : 0, pm = j < j1 ? coef_minus[j] : 0) {
     Cx<T>* bp = &buf[((pp >> 8) - c0) * LD + (pp & 255)];
     Cx<T>* bm = &buf[((pm >> 8) - c0) * LD + (pm & 255)];
     const Cx<T> t = *bp;
@@ -695,11 +707,10 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
   if (emit) {
     // positions outside the sphere (|ka| > h of the column, h from its entry
     // count) become zero: disjoint from the positions written above
-    const int* __restrict__ ent_off = a.ent_off;
     const Cx<T> zero = cx(T(0), T(0));
     for (int e = tid; e < nc * N; e += kThreads) {
       const int dl = e / N, ia = e - dl * N;
-      const int h = (ent_off[c0 + dl + 1] - ent_off[c0 + dl] - 1) >> 1;
+      const int h = col_h[E ? dl : 0];
       const int ka = ia <= N / 2 ? ia : ia - N;
       if (ka > h || ka < -h) buf[dl * LD + ia] = zero;
     }
