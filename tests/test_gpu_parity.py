@@ -2,7 +2,7 @@
 
 Every call goes through the C ABI (libsdft_hip.so via ctypes).  Tolerances:
 FP64 form <= 1e-12 relative on H.psi and <= 1e-10 relative on the filtered
-orbitals / density (1e-10 relative in density); FP32
+orbitals / density (the plain form's bar: 1e-10 relative in density); FP32
 form checked against FP64 with the bound written in each test.
 """
 import sys
