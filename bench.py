@@ -113,7 +113,7 @@ def main():
         pmc_run(args.workload)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build's filter path runs on a single MI355X "
+        print("bench.py measures one GPU: the filter path runs on a single MI355X "
               "(orbital sharding across GPUs is a design note, not a launched mode); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)

@@ -7,7 +7,7 @@
  * keeps the reference's array conventions (1-based `cmalloc(n)-1` arrays,
  * state-major re/im coefficient arrays with G=0 at index ncoef) and its error
  * behaviour (print an @@@..._ERROR_@@@ banner, exit(1)).  A PINY build links
- * libsdft_host.so and fills SDFT_STODFT from its structs (INTEGRATION.md).
+ * libsdft_host.so and fills SDFT_STODFT from its structs.
  */
 #ifndef SDFT_STODFT_H
 #define SDFT_STODFT_H
