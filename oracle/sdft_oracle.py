@@ -17,6 +17,14 @@ Parity status of each function (how it is pinned; see tests/):
       Hermiticity of H
   nl_apply_gspace                           : Hermiticity, gradient of the NL
       energy by finite differences, projector table vs real-space quadrature
+  nl_apply_real                             : agreement with the G-space form
+      built from the analytic Bessel transform (two independent forms),
+      Hermiticity
+  gen_energy_bound                          : converges to the extreme
+      eigenvalues of the dense H (eigvalsh)
+  newton_filter / newton_coeffs / leja_order : scalar Newton polynomial on
+      exact eigenvectors for every mu column; agreement with the Chebyshev
+      filter of the same smooth function to the interpolation error
   norm_h_cheby / filter_cheby               : scalar recurrence on exact
       eigenvectors, dense-H spectral oracle (sqrt f(H) chi by eigh)
   cheby_coeffs / round_fft                  : direct cosine sums, fit-error
