@@ -470,4 +470,6 @@ class Batch:
         return tuple(int(x) for x in cfg), ms[:n].copy()
 
     def config(self, split, overlap, plane_tile=0, fused_expand=1):
+        """Launch configuration by hand; fused_expand=1 needs every fragment's
+        sphere columns to be ka intervals (true of spherical cutoffs)."""
         _check(self.lib.sdft_hip_batch_config(self.h, split, overlap, plane_tile, fused_expand))
