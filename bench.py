@@ -32,6 +32,7 @@ for p in (REPO, PKG):
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # vector (and matrix) FP64
 FP32_VALU_PEAK_TFLOPS = 157.3
+FP32_VALU_UNPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per clock (no v_pk_*)
 FP32_MFMA_PEAK_TFLOPS = 157.3   # dense MFMA fp32
 FP64_MFMA_PEAK_TFLOPS = 78.6    # dense MFMA fp64
 
@@ -247,7 +248,11 @@ def main():
                     "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
                     "achieved_tflops": round(tfl, 2),
                     "valu_peak_tflops": FP32_VALU_PEAK_TFLOPS,
-                    "valu_frac": round(tfl / FP32_VALU_PEAK_TFLOPS, 4)}
+                    "valu_frac": round(tfl / FP32_VALU_PEAK_TFLOPS, 4),
+                    # the build compiles without packed FP32, whose rate per
+                    # lane is that of FP64: the reachable VALU peak is half
+                    "valu_peak_unpacked_tflops": FP32_VALU_UNPACKED_TFLOPS,
+                    "valu_frac_unpacked": round(tfl / FP32_VALU_UNPACKED_TFLOPS, 4)}
     else:
         roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": None, "traffic": None, "kernel": dom_name,
