@@ -107,3 +107,29 @@ def test_filter_large_grids(hip, large):
     acc_r, tl_r = O.filter_cheby(chi, coeffs, s, scale, emean)
     assert rel(acc_g, acc_r) < 1e-10
     assert rel(tl_g, tl_r) < 1e-10
+
+
+def test_filter_many_steps_large_grids(hip, large):
+    """A longer filter at the Si512 / Si1728 sphere shapes (L = 64 at 128^3,
+    32 at 192^3; one orbital pair) against the oracle: FP64 filtered orbitals
+    and density within 1e-10 relative, FP32 orbitals within 1e-4, so the
+    recurrence's error growth over many steps stays bounded on the split
+    (192^3 FP64) and unsplit forms alike."""
+    from sdft_amd import stodft, systems
+    s, ctx = large
+    L = 64 if s.n[0] == 128 else 32
+    say("case: filter L=%d %s (oracle ~%d s)" % (L, s.name, 20 if L == 64 else 30))
+    emin, emax = stodft.spectral_bounds(s)
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.3, 100.0, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, 2, seed=57)
+    acc_r, _ = O.filter_cheby(chi, coeffs, s, scale, emean)
+    acc_g, _ = ctx.cheby_filter(chi, coeffs, scale, emean)
+    assert rel(acc_g, acc_r) < 1e-10, rel(acc_g, acc_r)
+    rho_g = ctx.density(acc_g, 1.0)
+    rho_r = O.density_real(acc_r, s, 1.0)
+    assert rel(rho_g, rho_r) < 1e-10, rel(rho_g, rho_r)
+    acc32, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP32)
+    say("  %s L=%d: FP64 orbitals %.3g density %.3g, FP32 orbitals %.3g" % (
+        s.name, L, rel(acc_g, acc_r), rel(rho_g, rho_r), rel(acc32, acc_r)))
+    assert rel(acc32, acc_r) < 1e-4, rel(acc32, acc_r)
