@@ -201,3 +201,26 @@ def test_batch_mixed_local_only_and_real_space_fragments(hip, frags):
             ref, _ = O.filter_cheby(chi, co[i], sysm, scale[i], emean[i])
         assert rel(blocks[i].get(0), ref) < 1e-10, (i, kinds[i])
     bat.close()
+
+
+def test_batch_long_filter_both_forms(hip, frags):
+    """L = 128 on the 8 fragments with the autotuned configuration, in FP64
+    (1e-10 relative to the oracle per fragment) and FP32 (1e-4): the graph
+    replays 63 step pairs, so a step-parity or counter slip would show."""
+    say("case: batch 8 fragments L=128, FP64 and FP32, autotuned")
+    L = 128
+    co, scale, emean = _coeffs(frags, L)
+    refs = [O.filter_cheby(chi, co[f], sysm, scale[f], emean[f])[0]
+            for f, (sysm, chi, _, _) in enumerate(frags)]
+    for prec, tol in ((hip.PREC_FP64, 1e-10), (hip.PREC_FP32, 1e-4)):
+        ctxs, blocks, bat = _make_batch(hip, frags, prec)
+        cfg, _ = bat.autotune(steps=4)
+        for b, (_, chi, _, _) in zip(blocks, frags):
+            b.set(chi)
+        bat.filter(co, scale, emean, use_graph=True)
+        worst = max(rel(blocks[f].get(0), refs[f]) for f in range(len(frags)))
+        say("  prec %d config %s: worst fragment rel %.3g" % (prec, cfg, worst))
+        assert worst < tol, (prec, cfg, worst)
+        bat.close()
+        for b in blocks:
+            b.close()
