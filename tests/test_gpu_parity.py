@@ -400,6 +400,32 @@ def test_filter_si64_full_degree(hip, si64_system):
     ctx.close()
 
 
+def test_moments_and_mu_si64_full_degree(hip, si64_system):
+    """The moments pass and the mu search at the headline degree (Si64,
+    L = 1024, two orbitals of seed 1): the L + 1 moments against the oracle's
+    calcChebyMoments within 1e-10 of their largest magnitude, and mu from the
+    device's bisection against the oracle's from the same moments within
+    1e-9 (electron count 2 x 2 orbitals scaled to the two-orbital sample)."""
+    from sdft_amd import systems
+    say("case: moments + mu si64 L=1024, 2 orbitals (oracle ~15 s)")
+    ctx = hip.Context(si64_system)
+    emin, emax = bounds(si64_system)
+    L = 1024
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(si64_system, 2, seed=1)
+    m_g = ctx.cheby_moments(chi, L, scale, emean)
+    m_r = O.cheby_moments(chi, si64_system, L, scale, emean)
+    err = np.max(np.abs(m_g - m_r)) / np.max(np.abs(m_r))
+    say("  si64 L=1024 moments max rel %.3g" % err)
+    assert err < 1e-10
+    beta, ne = 100.0, float(si64_system.nelec)
+    mu_r, n_r = O.chem_pot_cheby(m_r, 2, L, emin, emax, beta, ne, 0.3, 0.4)
+    mu_g, n_g = ctx.chem_pot(L, emin, emax, beta, m_r, 2, ne, 0.3, 0.4)
+    assert abs(n_g - ne) <= 1e-11 * ne * 1.0001
+    assert abs(mu_g - mu_r) < 1e-9, (mu_g, mu_r)
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-4)])
 def test_filter_fused_and_separate_expand(hip, si8_system, si8_ctx, prec, tol):
     """The filter with the contract kernel writing the next step's
