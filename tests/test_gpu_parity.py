@@ -343,6 +343,29 @@ def test_newton_filter_matches_oracle(hip, si8_ctx, si8_system, prec, tol):
     assert rel(pl_g, pl_r) < tol
 
 
+def test_newton_filter_si64_long(hip, si64_system):
+    """filterNewtonPolyHerm at 64^3 with L = 128 Leja-ordered nodes and three
+    chemical-potential columns (one orbital pair) against the oracle: FP64
+    within 1e-10 relative per column."""
+    from sdft_amd import systems
+    say("case: newton filter si64 L=128, 3 mu columns (oracle ~20 s)")
+    ctx = hip.Context(si64_system)
+    emin, emax = bounds(si64_system)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    L = 128
+    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
+    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 20.0)), z)
+                   for m in (0.1, 0.3, 0.5)], axis=1)
+    chi = systems.noise_orbitals(si64_system, 2, seed=13)
+    out_g, pl_g = ctx.newton_filter(chi, co, z, scale, emean)
+    out_r, pl_r = O.newton_filter(chi, co, z, si64_system, scale, emean)
+    errs = [rel(out_g[m], out_r[m]) for m in range(3)]
+    say("  si64 newton L=128: per-column rel %s, last P %.3g" % (
+        ["%.3g" % e for e in errs], rel(pl_g, pl_r)))
+    assert max(errs) < 1e-10, errs
+    ctx.close()
+
+
 @pytest.fixture(scope="module")
 def si64_system():
     from sdft_amd import systems
