@@ -58,6 +58,34 @@ def survey_bytes_per_hpsi(sysm, wc, wv):
     return 2.0 * N * wc + N * wv / 2.0 + 80.0 * sysm.ncoef
 
 
+def host_info():
+    """The host the CPU baseline runs on: CPUs this process may use (nproc),
+    the machine's CPU count, the lscpu model name, OMP_NUM_THREADS."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        nproc = os.cpu_count() or 1
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.split(":")[0].strip() == "Model name":
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"nproc": nproc, "cpu_count": os.cpu_count(), "model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(sysm, chi, coeffs, scale, emean, nsteps, workers, runs=3, warmup=3):
     """Time the CPU FP64 oracle (same algorithm: pair-packed 3D FFTs, v.psi,
     G-space KB, kinetic, recurrence, accumulate) over the whole orbital
@@ -278,13 +306,16 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        workers = min(16, os.cpu_count() or 1)
+        host = host_info()
+        workers = host["nproc"]
         rate, dt = cpu_baseline(sysm, chi, coeffs, scale, emean, args.cpu_steps, workers)
         cpu = {"value": round(rate, 2), "unit": "H.psi/s", "cores": workers, "kind": "port",
-               "sample": "median of 3 runs of %d Chebyshev steps x %d orbitals of the same Si64 "
-                         "workload after 3 untimed H.psi, FP64 numpy/scipy.fft oracle (%.1f s "
-                         "timed)" % (args.cpu_steps, nst, dt)}
-        log("cpu baseline: %.1f H.psi/s on %d threads" % (rate, workers))
+               "host": host,
+               "sample": "median of 3 runs of %d Chebyshev steps x %d orbitals of the same %s "
+                         "workload after 3 untimed H.psi, FP64 numpy/scipy.fft oracle with "
+                         "%d FFT threads (every CPU this process may run on) (%.1f s timed)"
+                         % (args.cpu_steps, nst, wl.name, workers, dt)}
+        log("cpu baseline: %.1f H.psi/s on %d threads (%s)" % (rate, workers, host["model"]))
 
     batch = None if args.no_batch else batch_bench(args, mu)
 

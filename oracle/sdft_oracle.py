@@ -357,6 +357,97 @@ def leja_order(nodes):
     return np.array(out)
 
 
+def newton_samp_points(L, energy_min, newton_scale, smin=-2.0, smax=2.0):
+    """genSampNewtonHermit, stodft/coeff.c:423-533: 32 L equally spaced
+    candidates on [Smin, Smax] (Smin = -2, Smax = 2, init.c:240-241,442-443),
+    the first sample point is Smin, then each next point is the candidate
+    that maximises sum_j log((x - z_j)^2) over the points already chosen (a
+    discrete Leja sequence; a candidate closer than 1e-10 to a chosen point
+    scores -1e30, ties go to the lowest index as the reference's per-thread
+    strict-greater scan and thread-order merge do).  newton_scale is
+    newtonInfo->scale = (Smax - Smin)/energyDiff (gen-stodft-wf.c:216-220);
+    returns (sampPoint, sampPointUnscale) with sampPointUnscale =
+    (sampPoint - Smin)/newton_scale + energyMin."""
+    ncand = 32 * L
+    delta = (smax - smin) / float(ncand - 1)
+    cand = smin + np.arange(ncand) * delta
+    samp = np.empty(L)
+    samp[0] = cand[0]
+
+    def score(z):
+        d = cand - z
+        d = d * d
+        out = np.full(ncand, -1.0e30)
+        ok = d >= 1.0e-20
+        out[ok] = np.log(d[ok])
+        return out
+
+    obj = score(samp[0])
+    for i in range(1, L):
+        k = int(np.argmax(obj))
+        if not obj[k] > -100000.0:  # the reference's scan starts from -1e5
+            raise ValueError("no admissible sample candidate left")
+        samp[i] = cand[k]
+        obj += score(samp[i])
+    unscale = (samp - smin) * (1.0 / newton_scale) + energy_min
+    return samp, unscale
+
+
+def newton_hermite_coeffs(samp, unscale, chem_pots, beta):
+    """genCoeffNewtonHermit, stodft/coeff.c:199-420, without energy windows
+    (energyWindowOn = 0) and with SQRTFERMI: for each chemical potential the
+    Newton divided differences of sqrt(f_FD) at the sample points, by the
+    reference's forward substitution (a_0 = g(x_0), a_1 = (g_1 - a_0)/(z_1 -
+    z_0), a_i = (g_i - a_0 - sum_{j=1}^{i-1} a_j prod_{m<j}(z_i - z_m)) /
+    prod_{m<i}(z_i - z_m), accumulated in the reference's order).  Returns
+    expanCoeff as (L, nmu): row iPoly holds expanCoeff[iPoly*numChemPot + imu]."""
+    z = [float(v) for v in samp]
+    L = len(z)
+    mus = np.atleast_1d(np.asarray(chem_pots, dtype=np.float64))
+    out = np.zeros((L, mus.size))
+    for imu, mu in enumerate(mus):
+        g = [math.sqrt(float(fermi(x, mu, beta))) for x in unscale]
+        a = [0.0] * L
+        a[0] = g[0]
+        if L > 1:
+            a[1] = (g[1] - a[0]) / (z[1] - z[0])
+        for i in range(2, L):
+            zi = z[i]
+            sm = g[i] - a[0]
+            prod = 1.0
+            for j in range(1, i):
+                prod *= zi - z[j - 1]
+                sm -= a[j] * prod
+            prod *= zi - z[i - 1]
+            a[i] = sm / prod
+        out[:, imu] = a
+    return out
+
+
+def newton_fit_error(coeffs, samp, emin, emax, newton_scale, chem_pots, beta, npoint=1000):
+    """calcFitErrorNewton, stodft/coeff.c:644-760 (energyWindowOn = 0): the
+    largest |sum_n a_n prod_{j<n}(s - z_j) - sqrt(f_FD(x))| over npoint
+    midpoints x_i = emin + (i + 1/2)(emax - emin)/npoint, s = (x - emean)
+    newton_scale, over all chemical potentials."""
+    co = np.asarray(coeffs, dtype=np.float64)
+    if co.ndim == 1:
+        co = co[:, None]
+    mus = np.atleast_1d(np.asarray(chem_pots, dtype=np.float64))
+    emean = 0.5 * (emin + emax)
+    x = emin + (np.arange(npoint) + 0.5) * (emax - emin) / npoint
+    s = (x - emean) * newton_scale
+    err = -1.0e30
+    for imu, mu in enumerate(mus):
+        val = np.full(npoint, co[0, imu])
+        prod = np.ones(npoint)
+        for n in range(1, co.shape[0]):
+            prod = prod * (s - samp[n - 1])
+            val = val + co[n, imu] * prod
+        ref = np.sqrt(fermi(x, mu, beta))
+        err = max(err, float(np.max(np.abs(val - ref))))
+    return err
+
+
 def round_fft(n_in: int) -> int:
     """roundFFT, coeff.c:1556-1600: nearest 2^n 3^m to n_in by the
     reference's search (2^nmax first, then 2^i 3^(m+1) with m = floor of the

@@ -740,13 +740,16 @@ static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, con
     if ((rc = dalloc(&didx, sizeof(int) * std::max<long long>(1, nent)))) break;
     if ((rc = dalloc(&dval, sizeof(double) * std::max<long long>(1, nent)))) break;
     if ((rc = dalloc(&field, pair_bytes * bq))) break;
-    SDFT_HIP_CHECK(hipMemcpyAsync(doff, off, sizeof(long long) * (nproj + 1),
-                                  hipMemcpyHostToDevice, c.stream));
+    SDFT_HIP_TRY(rc, hipMemcpyAsync(doff, off, sizeof(long long) * (nproj + 1),
+                                    hipMemcpyHostToDevice, c.stream));
+    if (rc) break;
     if (nent > 0) {
-      SDFT_HIP_CHECK(hipMemcpyAsync(didx, idx, sizeof(int) * nent, hipMemcpyHostToDevice,
-                                    c.stream));
-      SDFT_HIP_CHECK(hipMemcpyAsync(dval, val, sizeof(double) * nent, hipMemcpyHostToDevice,
-                                    c.stream));
+      SDFT_HIP_TRY(rc, hipMemcpyAsync(didx, idx, sizeof(int) * nent, hipMemcpyHostToDevice,
+                                      c.stream));
+      if (rc) break;
+      SDFT_HIP_TRY(rc, hipMemcpyAsync(dval, val, sizeof(double) * nent, hipMemcpyHostToDevice,
+                                      c.stream));
+      if (rc) break;
     }
     if ((rc = block_alloc(c, 2 * bq, 0, tmp))) break;
     if ((rc = ensure_W(c, w_bytes(c, bq, 0)))) break;
@@ -755,7 +758,8 @@ static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, con
     for (int q0 = 0; q0 < pairs && !rc; q0 += bq) {
       const int nq = std::min(bq, pairs - q0);
       const int p0 = 2 * q0, p1 = std::min(nproj, 2 * (q0 + nq));
-      SDFT_HIP_CHECK(hipMemsetAsync(field, 0, pair_bytes * nq, c.stream));
+      SDFT_HIP_TRY(rc, hipMemsetAsync(field, 0, pair_bytes * nq, c.stream));
+      if (rc) break;
       if ((rc = launch_scatter_proj(c, doff, didx, dval, p0, p1, field))) break;
       if ((rc = launch_grid_plane(c, c.W, nq, field))) break;
       StepParams sp{};
@@ -1091,16 +1095,18 @@ int sdft_hip_newton_filter(sdft_ctx* h, int nstate, int L, int nmu, const double
   do {
     if ((rc = dalloc(&accs, sizeof(double) * (size_t)stride * nmu))) break;
     if ((rc = dalloc(&cdev, sizeof(double) * (size_t)L * nmu))) break;
-    SDFT_HIP_CHECK(hipMemcpyAsync(cdev, coeffs, sizeof(double) * (size_t)L * nmu,
-                                  hipMemcpyHostToDevice, c.stream));
+    SDFT_HIP_TRY(rc, hipMemcpyAsync(cdev, coeffs, sizeof(double) * (size_t)L * nmu,
+                                    hipMemcpyHostToDevice, c.stream));
+    if (rc) break;
     if ((rc = block_put(b, b.X0, cre, cim))) break;
     if (L == 1) {
       for (int mu = 0; mu < nmu; ++mu) {
         hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0,
                            c.stream, b.X0, coeffs[mu], accs + (size_t)mu * stride,
                            (size_t)stride);
-        SDFT_HIP_CHECK(hipGetLastError());
+        SDFT_HIP_TRY(rc, hipGetLastError());
       }
+      if (rc) break;
     }
     // the contract kernel writes accumulator mu at acc + mu * stride: point
     // the block's accumulator at the nmu-accumulator buffer for the loop
@@ -1203,7 +1209,7 @@ int sdft_hip_cheby_filter(sdft_ctx* h, int nstate, int L, const double* coeffs, 
   if (!rc) rc = block_put(b, b.X0, cre, cim);
   if (!rc && moments) {
     const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
-    SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
+    SDFT_HIP_TRY(rc, hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
   }
   if (!rc) rc = block_filter_impl(b, L, coeffs, scale, emean, moments);
   if (!rc) rc = block_fetch(b, b.acc, ore, oim);
@@ -1228,7 +1234,7 @@ int sdft_hip_cheby_moments(sdft_ctx* h, int nstate, int L, double scale, double 
   if (!rc) rc = ensure_mom(b, nmom);
   if (!rc) {
     const size_t bytes = sizeof(double) * (size_t)b.nstate_pad * 2 * c.g.ncp;
-    SDFT_HIP_CHECK(hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
+    SDFT_HIP_TRY(rc, hipMemcpyAsync(b.chi, b.X0, bytes, hipMemcpyDeviceToDevice, c.stream));
     prof_reset(c);
     double* bufs[2] = {b.X0, b.X1};
     for (int k = 1; k < nmom && !rc; ++k) {
@@ -1277,6 +1283,7 @@ int sdft_hip_chem_pot(sdft_ctx* h, int L, double emin, double emax, double beta,
   return chem_pot_device(h->c, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ, tol,
                          mu_out, ne_out);
 }
+
 
 int sdft_hip_density(sdft_ctx* h, int nstate, const double* re, const double* im,
                      double occ_over_nsto, double* rho, int layout, int accumulate) {

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/sdft_hip.h"
@@ -61,6 +62,10 @@ struct Frag {
   Block* b = nullptr;
   Ctx* c = nullptr;
   int npairs = 0;
+  // projector count and padded count of the context when the batch was
+  // created: zpart, Y and Z are sized for them, and the KB launch lists
+  // include the fragment only when P > 0
+  int P = 0, ppad = 0;
   void* W = nullptr;
   double* zpart = nullptr;
   double* Y = nullptr;
@@ -225,6 +230,16 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
     int rc0 = dmalloc(&B.coeffs, need);
     if (rc0) return rc0;
     B.coeffs_cap = need;
+  }
+  for (int fi = 0; fi < nf; ++fi) {
+    const Frag& f = B.frags[fi];
+    if (f.c->nproj != f.P || f.c->ppad != f.ppad) {
+      set_error("fragment " + std::to_string(fi) + ": the context's projectors changed after "
+                "sdft_hip_batch_create (the batch's KB buffers are sized for " +
+                std::to_string(f.P) + " projectors, the context now has " +
+                std::to_string(f.c->nproj) + "); destroy and re-create the batch");
+      return SDFT_ESTATE;
+    }
   }
   SDFT_HIP_CHECK(hipMemcpyAsync(B.coeffs, coeffs_host, need, hipMemcpyHostToDevice, B.stream));
   for (Group& g : B.groups) {
@@ -497,6 +512,8 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     if (!f.c->g.col_intervals) B.fused_ok = B.fused = false;
     if ((rc = dmalloc(&f.W, w_bytes(*f.c, f.npairs, prec)))) return fail(rc);
     const int P = f.c->nproj, S = f.b->nstate_pad;
+    f.P = P;
+    f.ppad = f.c->ppad;
     if (P > 0) {
       B.nl_frags.push_back(i);
       if ((rc = dmalloc(&f.zpart, sizeof(double) * B.max_zpart_slices * P * S))) return fail(rc);

@@ -15,8 +15,11 @@
 //  * per-step moment partial reduction and the kinetic / nonlocal energy
 //    sums (stodft/calc-energy.c:133-191).
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
+#include "../../include/sdft_hip.h"
 #include "sdft_internal.h"
 
 namespace sdft {
@@ -68,9 +71,9 @@ __global__ void k_cheby_project(const double* __restrict__ g, const double* __re
   if (threadIdx.x == 0) c[k] = (k == 0 ? 1.0 : 2.0) * s / (double)M;
 }
 
-// h_j = m_0 + 2 sum_{k=1}^{L-1} m_k cos(2 pi jk/M)
+// h_j = (m_0 + 2 sum_{k=1}^{L-1} m_k cos(2 pi jk/M)) * inv_nsto
 __global__ void k_cheby_h(const double* __restrict__ m, int L, const double* __restrict__ costab,
-                          int M, double* __restrict__ h) {
+                          int M, double inv_nsto, double* __restrict__ h) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   double s = 0.0;
@@ -80,7 +83,7 @@ __global__ void k_cheby_h(const double* __restrict__ m, int L, const double* __r
     if (idx >= M) idx -= M;
     s += m[k] * costab[idx];
   }
-  h[j] = m[0] + 2.0 * s;
+  h[j] = (m[0] + 2.0 * s) * inv_nsto;
 }
 
 __device__ double num_elec_dev(const double* costab, const double* h, int M, double dE,
@@ -239,7 +242,7 @@ __global__ void k_normalize0(double* __restrict__ X, int ncp, int ncoef) {
 
 // ==========================================================================
 
-static int scratch(Ctx& c, size_t need) {
+static int ensure_red(Ctx& c, size_t need) {
   if (c.red_bytes >= need) return 0;
   if (c.red) SDFT_HIP_CHECK(hipFree(c.red));
   c.red = nullptr;
@@ -255,7 +258,7 @@ static int scratch(Ctx& c, size_t need) {
 int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,
                         int fun_flag, double* coeffs_host) {
   const int M = round_fft(32 * L);
-  int rc = scratch(c, sizeof(double) * ((size_t)2 * M + L));
+  int rc = ensure_red(c, sizeof(double) * ((size_t)2 * M + L));
   if (rc) return rc;
   double* costab = c.red;
   double* g = costab + M;
@@ -273,22 +276,26 @@ int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, doub
   return 0;
 }
 
-int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta, const double* moments,
-                    double nsto, double ne_true, double mu_init, double gap, double occ,
-                    double tol, double* mu_out, double* ne_out) {
+// The chemical-potential search on a device buffer `buf` of at least
+// chem_pot_scratch_doubles(L) doubles: cosine table and h_j (M each), the raw
+// moments (L), the result (3).  The moments are copied in unnormalised; the
+// 1/nsto factor is folded into h_j.
+static size_t chem_pot_scratch_doubles(int L) {
+  return (size_t)2 * round_fft(32 * L) + L + 4;
+}
+
+static int chem_pot_on(Ctx& c, double* buf, int L, double emin, double emax, double beta,
+                       const double* moments, double nsto, double ne_true, double mu_init,
+                       double gap, double occ, double tol, double* mu_out, double* ne_out) {
   const int M = round_fft(32 * L);
-  int rc = scratch(c, sizeof(double) * ((size_t)2 * M + L + 4));
-  if (rc) return rc;
-  double* costab = c.red;
+  double* costab = buf;
   double* h = costab + M;
   double* m = h + M;
   double* out = m + L;
-  std::vector<double> mn(L);
-  for (int k = 0; k < L; ++k) mn[k] = moments[k] / nsto;
-  SDFT_HIP_CHECK(hipMemcpyAsync(m, mn.data(), sizeof(double) * L, hipMemcpyHostToDevice, c.stream));
+  SDFT_HIP_CHECK(hipMemcpyAsync(m, moments, sizeof(double) * L, hipMemcpyHostToDevice, c.stream));
   const int nb = (M + 255) / 256;
   hipLaunchKernelGGL(k_costab, dim3(nb), dim3(256), 0, c.stream, costab, M);
-  hipLaunchKernelGGL(k_cheby_h, dim3(nb), dim3(256), 0, c.stream, m, L, costab, M, h);
+  hipLaunchKernelGGL(k_cheby_h, dim3(nb), dim3(256), 0, c.stream, m, L, costab, M, 1.0 / nsto, h);
   const double dE = (emax - emin) * 0.5, em = 0.5 * (emax + emin);
   hipLaunchKernelGGL(k_chem_pot, dim3(1), dim3(1024), 0, c.stream, costab, h, M, dE, em, beta, occ,
                      ne_true, mu_init, gap, tol, 400, out);
@@ -301,8 +308,17 @@ int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta, const 
   return 0;
 }
 
+int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta, const double* moments,
+                    double nsto, double ne_true, double mu_init, double gap, double occ,
+                    double tol, double* mu_out, double* ne_out) {
+  int rc = ensure_red(c, sizeof(double) * chem_pot_scratch_doubles(L));
+  if (rc) return rc;
+  return chem_pot_on(c, c.red, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ,
+                     tol, mu_out, ne_out);
+}
+
 int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3) {
-  int rc = scratch(c, sizeof(double) * (size_t)3 * nsteps);
+  int rc = ensure_red(c, sizeof(double) * (size_t)3 * nsteps);
   if (rc) return rc;
   hipLaunchKernelGGL(k_mom_reduce, dim3(nsteps), dim3(256), 0, c.stream, mom_part, nwg, c.red);
   SDFT_HIP_CHECK(hipGetLastError());
@@ -313,7 +329,7 @@ int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* 
 }
 
 int energy_ke_device(Ctx& c, const double* X, int nstate, double* out) {
-  int rc = scratch(c, sizeof(double) * (size_t)nstate);
+  int rc = ensure_red(c, sizeof(double) * (size_t)nstate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_eke, dim3(nstate), dim3(256), 0, c.stream, X, 2 * c.g.ncp, c.g.ncp,
                      c.g.ncoef, c.t.ak2, c.red);
@@ -336,7 +352,7 @@ int normalize0_device(Ctx& c, double* X) {
 
 int power_update_device(Ctx& c, double* X, const double* F, int mode, double emax,
                         double* energy_host) {
-  int rc = scratch(c, sizeof(double));
+  int rc = ensure_red(c, sizeof(double));
   if (rc) return rc;
   hipLaunchKernelGGL(k_power_update, dim3(1), dim3(1024), 0, c.stream, X, F, c.g.ncp, c.g.ncoef,
                      mode, emax, c.red);
@@ -348,7 +364,7 @@ int power_update_device(Ctx& c, double* X, const double* F, int mode, double ema
 }
 
 int energy_nl_device(Ctx& c, int ns, int nstate, double* out) {
-  int rc = scratch(c, sizeof(double));
+  int rc = ensure_red(c, sizeof(double));
   if (rc) return rc;
   hipLaunchKernelGGL(k_enl, dim3(1), dim3(1024), 0, c.stream, c.Z, c.E, c.nproj, ns, nstate, c.red);
   SDFT_HIP_CHECK(hipGetLastError());

@@ -37,6 +37,15 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (e_ != hipSuccess) return ::sdft::hip_fail(e_, #expr, __FILE__, __LINE__); \
   } while (0)
 
+// The same check without the early return: rc receives the status so that a
+// caller's shared cleanup still runs (use inside do { ... } while (0) bodies
+// as `SDFT_HIP_TRY(rc, expr); if (rc) break;`).
+#define SDFT_HIP_TRY(rc, expr)                                              \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) (rc) = ::sdft::hip_fail(e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
 struct Geometry {
   int n = 0;          // cubic grid extent
   long long N = 0;    // n^3

@@ -22,14 +22,32 @@ static void sdft_die(const char *where, int rc) {
     if (rc_ != 0) sdft_die(where, rc_); \
   } while (0)
 
-void calcCoefForceWrapSCF(SDFT_STODFT *st) {
+/* Internal calls go through these static helpers, never through an exported
+ * name, so the library's own call graph cannot be rebound by another
+ * definition of the same symbol in the process. */
+static void coef_force(SDFT_STODFT *st) {
   SDFT_CALL("calcCoefForceWrapSCF",
             sdft_hip_hpsi(st->ctx, st->numStateUpProc, &st->cre_up[1], &st->cim_up[1],
                           &st->fcre_up[1], &st->fcim_up[1], st->prec));
 }
 
+static void cheby_coeff(SDFT_STODFT *st, double mu, double *chebyCoeffs, int funFlag) {
+  SDFT_CALL("calcChebyCoeff",
+            sdft_hip_cheby_coeffs(st->ctx, st->polynormLength, st->energyMin, st->energyMax, mu,
+                                  st->beta, funFlag, chebyCoeffs));
+}
+
+static void cheby_moments(SDFT_STODFT *st) {
+  SDFT_CALL("calcChebyMoments",
+            sdft_hip_cheby_moments(st->ctx, st->numStateUpProc, st->polynormLength, st->scale,
+                                   st->energyMean, &st->cre_up[1], &st->cim_up[1],
+                                   st->chebyMomentsUp, st->prec));
+}
+
+SDFT_API void sdft_calcCoefForceWrapSCF(SDFT_STODFT *st) { coef_force(st); }
+
 /* normh.c:182-344 */
-void normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag) {
+SDFT_API void sdft_normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag) {
   const int nc = st->numCoeff;
   const int tot = st->numStateUpProc * nc;
   double s1 = -st->scale * 0.25;
@@ -39,7 +57,7 @@ void normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag) {
   double *fre = st->fcre_up, *fim = st->fcim_up;
   double *wre = st->wfUpRe1, *wim = st->wfUpIm1;
   int is, i;
-  calcCoefForceWrapSCF(st);
+  coef_force(st);
   if (iPoly == 1) {
     for (is = 0; is < st->numStateUpProc; is++) {
       const int off = is * nc;
@@ -75,7 +93,7 @@ void normHCheby(SDFT_STODFT *st, int iPoly, int cpyFlag) {
 }
 
 /* filter.c:318-706, the whole loop on the device */
-void filterChebyPolyHerm(SDFT_STODFT *st) {
+SDFT_API void sdft_filterChebyPolyHerm(SDFT_STODFT *st) {
   const int tot = st->numStateUpProc * st->numCoeff;
   SDFT_CALL("filterChebyPolyHerm",
             sdft_hip_cheby_filter(st->ctx, st->numStateUpProc, st->polynormLength, st->expanCoeff,
@@ -88,33 +106,27 @@ void filterChebyPolyHerm(SDFT_STODFT *st) {
   }
 }
 
-void calcChebyCoeff(SDFT_STODFT *st, double *chemPot, double *chebyCoeffs, int funFlag) {
-  SDFT_CALL("calcChebyCoeff",
-            sdft_hip_cheby_coeffs(st->ctx, st->polynormLength, st->energyMin, st->energyMax,
-                                  chemPot[0], st->beta, funFlag, chebyCoeffs));
+SDFT_API void sdft_calcChebyCoeff(SDFT_STODFT *st, double *chemPot, double *chebyCoeffs,
+                                  int funFlag) {
+  cheby_coeff(st, chemPot[0], chebyCoeffs, funFlag);
 }
 
-void calcChebyMoments(SDFT_STODFT *st) {
-  SDFT_CALL("calcChebyMoments",
-            sdft_hip_cheby_moments(st->ctx, st->numStateUpProc, st->polynormLength, st->scale,
-                                   st->energyMean, &st->cre_up[1], &st->cim_up[1],
-                                   st->chebyMomentsUp, st->prec));
-}
+SDFT_API void sdft_calcChebyMoments(SDFT_STODFT *st) { cheby_moments(st); }
 
 /* calc-chempot-chebyshev.c:918-972 (moments already normalised by N_sto) */
-double calcNumElecCheby(SDFT_STODFT *st, double chemPot, double *chebyCoeffs) {
+SDFT_API double sdft_calcNumElecCheby(SDFT_STODFT *st, double chemPot, double *chebyCoeffs) {
   double ne = 0.0;
   int k;
-  calcChebyCoeff(st, &chemPot, chebyCoeffs, 1);
+  cheby_coeff(st, chemPot, chebyCoeffs, 1);
   for (k = 0; k < st->polynormLength; k++) ne += chebyCoeffs[k] * st->chebyMomentsUp[k];
   return ne * st->occNumber;
 }
 
 /* calc-chempot-chebyshev.c:30-315 */
-void calcChemPotCheby(SDFT_STODFT *st) {
+SDFT_API void sdft_calcChemPotCheby(SDFT_STODFT *st) {
   double mu = 0.0, ne = 0.0;
   int k;
-  calcChebyMoments(st);
+  cheby_moments(st);
   for (k = 0; k <= st->polynormLength; k++) st->chebyMomentsUp[k] /= (double)st->numStateStoUp;
   SDFT_CALL("calcChemPotCheby",
             sdft_hip_chem_pot(st->ctx, st->polynormLength, st->energyMin, st->energyMax, st->beta,
@@ -124,7 +136,7 @@ void calcChemPotCheby(SDFT_STODFT *st) {
   printf("Correct Chemical Potential is %.16lg\n", mu);
 }
 
-void calcKSPot(SDFT_STODFT *st) {
+SDFT_API void sdft_calcKSPot(SDFT_STODFT *st) {
   double en[4];
   SDFT_CALL("calcKSPot", sdft_hip_ks_potential(st->ctx, st->rhoUp, SDFT_RHO_FFTW, &st->vextr[1],
                                                &st->vexti[1], st->volume, st->vxcType, en,
@@ -135,7 +147,7 @@ void calcKSPot(SDFT_STODFT *st) {
   st->muxc = en[3];
 }
 
-void calcRhoStoHybridCheby(SDFT_STODFT *st) {
+SDFT_API void sdft_calcRhoStoHybridCheby(SDFT_STODFT *st) {
   SDFT_CALL("calcRhoStoHybridCheby",
             sdft_hip_density(st->ctx, st->numStateUpProc, &st->stoWfUpRe[0][1],
                              &st->stoWfUpIm[0][1], st->occNumber / (double)st->numStateStoUp,
@@ -145,7 +157,7 @@ void calcRhoStoHybridCheby(SDFT_STODFT *st) {
 /* filter.c:33-312: one device call for the whole Newton recurrence; the
  * nmu filtered blocks come back contiguous and are scattered to
  * stoWfUpRe/Im[imu] */
-void filterNewtonPolyHerm(SDFT_STODFT *st) {
+SDFT_API void sdft_filterNewtonPolyHerm(SDFT_STODFT *st) {
   const int tot = st->numStateUpProc * st->numCoeff;
   const int nmu = st->numChemPot > 0 ? st->numChemPot : 1;
   double *ore = (double *)malloc(sizeof(double) * (size_t)tot * nmu);
@@ -182,6 +194,6 @@ static void spectral(SDFT_STODFT *st, int which, double *re, double *im, const c
   st->energyDiff = st->energyMax - st->energyMin;
 }
 
-void genEnergyMax(SDFT_STODFT *st) { spectral(st, 0, st->testWfMaxRe, st->testWfMaxIm, "genEnergyMax"); }
+SDFT_API void sdft_genEnergyMax(SDFT_STODFT *st) { spectral(st, 0, st->testWfMaxRe, st->testWfMaxIm, "genEnergyMax"); }
 
-void genEnergyMin(SDFT_STODFT *st) { spectral(st, 1, st->testWfMinRe, st->testWfMinIm, "genEnergyMin"); }
+SDFT_API void sdft_genEnergyMin(SDFT_STODFT *st) { spectral(st, 1, st->testWfMinRe, st->testWfMinIm, "genEnergyMin"); }

@@ -62,8 +62,12 @@ def build(verbose: bool = False) -> str:
         if os.path.isdir(HOST) else []
     if host_src:
         htmp = host_lib_path() + ".tmp"
+        # exports limited to the sdft_* entry points (version script), the
+        # library's own calls bound inside it (-Bsymbolic-functions)
         _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-Wall", f"-I{INCLUDE}", "-o", htmp]
-             + host_src + [f"-L{LIB_DIR}", "-lsdft_hip", "-Wl,-rpath,$ORIGIN"])
+             + host_src + [f"-L{LIB_DIR}", "-lsdft_hip", "-Wl,-rpath,$ORIGIN",
+                           "-Wl,-Bsymbolic-functions",
+                           "-Wl,--version-script=" + os.path.join(HOST, "libsdft_host.map")])
         os.replace(htmp, host_lib_path())
     if verbose:
         print("built", out)

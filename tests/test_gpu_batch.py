@@ -165,6 +165,39 @@ def test_batch_rejects_bad_input(hip, frags):
     bat.close()
 
 
+def test_batch_refuses_projector_change_after_create(hip, frags):
+    """The batch sizes its KB buffers from each context's projector count at
+    sdft_hip_batch_create; a context whose projectors change afterwards (more,
+    or from none to some) makes the next batch filter return an error instead
+    of launching with the old sizes, and re-creating the batch works."""
+    say("case: batch projector change after create")
+    sysm, chi, emin, emax = frags[0]
+    L = 4
+    co, sc, em = _coeffs([frags[0]], L)
+    c = hip.Context(sysm, with_nl=False)
+    b = hip.Block(c, 2, hip.PREC_FP64)
+    b.set(chi[:2])
+    bat = hip.Batch([b])
+    bat.filter(co, sc, em)
+    c.set_projectors(sysm.proj, sysm.proj_energy)  # 0 -> nproj projectors
+    with pytest.raises(hip.SdftError, match="projectors changed"):
+        bat.filter(co, sc, em)
+    bat.close()
+    bat = hip.Batch([b])
+    bat.filter(co, sc, em)
+    acc = b.get(0)
+    ref, _ = O.filter_cheby(chi[:2], co[0], sysm, sc[0], em[0])
+    assert rel(acc, ref) < 1e-10
+    # more projectors than at create time
+    c.set_projectors(np.concatenate([sysm.proj, sysm.proj]),
+                     np.concatenate([sysm.proj_energy, sysm.proj_energy]))
+    with pytest.raises(hip.SdftError, match="projectors changed"):
+        bat.filter(co, sc, em)
+    bat.close()
+    b.close()
+    c.close()
+
+
 def test_batch_mixed_local_only_and_real_space_fragments(hip, frags):
     """A batch whose fragments mix the three KB forms: G-space table, no
     projectors (local potential only: the contract takes no NL force) and

@@ -323,46 +323,58 @@ def test_spectral_bound_matches_oracle(hip, si8_ctx, si8_system, which):
         assert it_g == it_r and abs(e_g - e_r) < 1e-9 * abs(e_r)
 
 
+def piny_newton_set(L, emin, emax, mus, beta):
+    """The sample points and coefficients PINY hands the Newton filter
+    (genSampNewtonHermit / genCoeffNewtonHermit, stodft/coeff.c:199-533, as
+    the oracle restates them) and the Newton map's scale and centre
+    (gen-stodft-wf.c:216-220: (Smax - Smin)/energyDiff, energyMean)."""
+    ns = 4.0 / (emax - emin)
+    samp, unscale = O.newton_samp_points(L, emin, ns)
+    co = O.newton_hermite_coeffs(samp, unscale, mus, beta)
+    return co, samp, ns, 0.5 * (emin + emax)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-4)])
 def test_newton_filter_matches_oracle(hip, si8_ctx, si8_system, prec, tol):
-    """sdft_hip_newton_filter (filterNewtonPolyHerm) against the oracle:
-    three chemical-potential columns, Leja-ordered Chebyshev nodes, L = 20."""
+    """sdft_hip_newton_filter (filterNewtonPolyHerm) against the oracle with
+    PINY's own sample points and divided differences for three chemical
+    potentials, L = 64."""
     from sdft_amd import systems
-    say("case: newton filter prec=%d" % prec)
+    say("case: newton filter prec=%d (PINY sample points, L=64)" % prec)
     emin, emax = bounds(si8_system)
-    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
-    L = 20
-    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
-    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 4.0)), z)
-                   for m in (0.1, 0.3, 0.5)], axis=1)
+    co, z, ns, emean = piny_newton_set(64, emin, emax, [0.1, 0.3, 0.5], 4.0)
     chi = systems.noise_orbitals(si8_system, 5, seed=12)
-    out_g, pl_g = si8_ctx.newton_filter(chi, co, z, scale, emean, prec=prec)
-    out_r, pl_r = O.newton_filter(chi, co, z, si8_system, scale, emean)
+    out_g, pl_g = si8_ctx.newton_filter(chi, co, z[:-1], ns, emean, prec=prec)
+    out_r, pl_r = O.newton_filter(chi, co, z[:-1], si8_system, ns, emean)
     for m in range(3):
         assert rel(out_g[m], out_r[m]) < tol, (m, rel(out_g[m], out_r[m]))
     assert rel(pl_g, pl_r) < tol
 
 
-def test_newton_filter_si64_long(hip, si64_system):
-    """filterNewtonPolyHerm at 64^3 with L = 128 Leja-ordered nodes and three
-    chemical-potential columns (one orbital pair) against the oracle: FP64
-    within 1e-10 relative per column."""
+def test_newton_filter_si64_piny_points_l1024(hip, si64_system):
+    """filterNewtonPolyHerm at 64^3 with the sample points and coefficients
+    PINY generates (genSampNewtonHermit / genCoeffNewtonHermit) at L = 1024,
+    kT = 0.01 Ha, two chemical-potential columns, one orbital pair, against
+    the oracle: FP64 within 1e-10 relative per column, FP32 within 1e-4; the
+    expansion's own fit error (calcFitErrorNewton) is below 1e-5 there."""
     from sdft_amd import systems
-    say("case: newton filter si64 L=128, 3 mu columns (oracle ~20 s)")
+    say("case: newton filter si64 L=1024 PINY points, 2 mu columns (oracle ~20 s)")
     ctx = hip.Context(si64_system)
     emin, emax = bounds(si64_system)
-    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
-    L = 128
-    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
-    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 20.0)), z)
-                   for m in (0.1, 0.3, 0.5)], axis=1)
+    mus, beta = [0.25, 0.35], 100.0
+    L = 1024
+    co, z, ns, emean = piny_newton_set(L, emin, emax, mus, beta)
+    fit = O.newton_fit_error(co, z, emin, emax, ns, mus, beta)
+    say("  fit error %.3e" % fit)
+    assert fit < 1e-5
     chi = systems.noise_orbitals(si64_system, 2, seed=13)
-    out_g, pl_g = ctx.newton_filter(chi, co, z, scale, emean)
-    out_r, pl_r = O.newton_filter(chi, co, z, si64_system, scale, emean)
-    errs = [rel(out_g[m], out_r[m]) for m in range(3)]
-    say("  si64 newton L=128: per-column rel %s, last P %.3g" % (
-        ["%.3g" % e for e in errs], rel(pl_g, pl_r)))
-    assert max(errs) < 1e-10, errs
+    out_r, pl_r = O.newton_filter(chi, co, z[:-1], si64_system, ns, emean)
+    for prec, tol in ((hip.PREC_FP64, 1e-10), (hip.PREC_FP32, 1e-4)):
+        out_g, pl_g = ctx.newton_filter(chi.copy(), co, z[:-1], ns, emean, prec=prec)
+        errs = [rel(out_g[m], out_r[m]) for m in range(len(mus))]
+        say("  si64 newton L=1024 prec=%d: per-column rel %s, last P %.3g" % (
+            prec, ["%.3g" % e for e in errs], rel(pl_g, pl_r)))
+        assert max(errs) < tol, (prec, errs)
     ctx.close()
 
 
@@ -392,21 +404,31 @@ def test_plane_tiles_and_tuner_si64(hip, si64_system):
     assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
 
 
-def test_filter_si64_full_degree(hip, si64_system):
-    """The headline workload's filter at its full degree (Si64, 64^3, L = 1024,
-    kT = 0.01 Ha) on one orbital pair of its noise (seed 1) against the
-    oracle's filterChebyPolyHerm: FP64 filtered orbitals and density within
-    1e-10 relative (the plain form's bar), the FP32 form within 1e-4 relative
-    (orbitals) and 1e-6 L1 per electron (density)."""
+@pytest.fixture(scope="module")
+def si64_full_ref(si64_system):
+    """The oracle's filter of one orbital pair (seed 1) of the headline
+    workload at its full degree: L = 1024, kT = 0.01 Ha, mu = 0.2."""
     from sdft_amd import systems
-    say("case: filter si64 full degree L=1024, 2 orbitals (oracle ~30 s)")
-    ctx = hip.Context(si64_system)
+    say("case: oracle filter si64 full degree L=1024, 2 orbitals (~30 s)")
     emin, emax = bounds(si64_system)
     L = 1024
     coeffs = O.cheby_coeffs(L, emin, emax, 0.2, 100.0, 2)
     scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
     chi = systems.noise_orbitals(si64_system, 2, seed=1)
     ref, _ = O.filter_cheby(chi, coeffs, si64_system, scale, emean)
+    return dict(chi=chi, coeffs=coeffs, scale=scale, emean=emean, ref=ref)
+
+
+def test_filter_si64_full_degree(hip, si64_system, si64_full_ref):
+    """The headline workload's filter at its full degree (Si64, 64^3, L = 1024,
+    kT = 0.01 Ha) on one orbital pair of its noise (seed 1) against the
+    oracle's filterChebyPolyHerm: FP64 filtered orbitals and density within
+    1e-10 relative (the plain form's bar), the FP32 form within 1e-4 relative
+    (orbitals) and 1e-6 L1 per electron (density)."""
+    say("case: filter si64 full degree L=1024, 2 orbitals")
+    ctx = hip.Context(si64_system)
+    r = si64_full_ref
+    chi, coeffs, scale, emean, ref = r["chi"], r["coeffs"], r["scale"], r["emean"], r["ref"]
     rho_ref = O.density_real(ref, si64_system, 1.0)
     ne = rho_ref.sum()
     out64, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP64)
@@ -421,6 +443,70 @@ def test_filter_si64_full_degree(hip, si64_system):
         np.abs(rho32 - rho_ref).sum() / ne))
     assert np.abs(rho32 - rho_ref).sum() / ne < 1e-6
     ctx.close()
+
+
+def energy_and_forces_device(ctx, sysm, C, occ, nsto):
+    """Total energy and NL + local Hellmann-Feynman forces of filtered
+    orbitals, on the device: E_kin + E_NL (calcEnergyChemPot,
+    stodft/calc-energy.c:133-191) + E_H + E_ext + E_xc of their density
+    (cpGetVksStodft's energies, assembled as calcTotEnergy :329-333 without
+    the ion-ion term), forces = NL (calc-nuclei-force.c:302-489) + local
+    (:254).  Replaces the context's v_KS."""
+    from sdft_amd import systems
+    dens = systems.density_sphere(sysm)
+    vext = systems.vloc_density(sysm, dens)
+    eke, enl = ctx.energies(C, occ, nsto)
+    rho = ctx.density(C, occ / nsto)
+    ctx.ks_init(dens)
+    _, en = ctx.ks_potential(rho, vext, sysm.volume, xc=0)
+    e = eke + enl + en["eh"] + en["eext"] + en["exc"]
+    vloc = systems.vloc_form_factor(sysm, dens)
+    f = ctx.nl_forces(C, occ, nsto) + ctx.local_forces(sysm.pos, np.zeros(sysm.natm, int), vloc,
+                                                       dens.gvec)
+    return e, f
+
+
+def energy_and_forces_oracle(sysm, C, occ, nsto):
+    from sdft_amd import systems
+    dens = systems.density_sphere(sysm)
+    vext = systems.vloc_density(sysm, dens)
+    eke = O.energy_ke(C, sysm.ak2, occ, nsto)
+    enl = O.energy_nl(C, sysm.proj, sysm.proj_energy, occ, nsto)
+    rho = O.density_real(C, sysm, occ / nsto)
+    _, out = O.ks_potential(rho, vext, dens, sysm.volume, 0)
+    e = eke + enl + out["eh"] + out["eext"] + out["exc"]
+    vloc = systems.vloc_form_factor(sysm, dens)
+    f = O.nl_forces(C, sysm, occ, nsto) + O.local_forces(out["Q"], dens, sysm.pos, vloc,
+                                                         np.zeros(sysm.natm, int))
+    return e, f
+
+
+def test_fp32_acceptance_energy_and_forces_si64(hip, si64_system, si64_full_ref):
+    """The precision form's acceptance bar on the headline workload at its
+    full degree (Si64, L = 1024, one orbital pair of seed 1): the total
+    energy (kinetic + nonlocal + Hartree + external + LDA exchange-correlation
+    of the filtered orbitals and their density) within 1e-4 Ha/atom of the
+    FP64 oracle's, and the nonlocal + local Hellmann-Feynman forces within
+    1e-3 Ha/bohr; the FP64 form meets the same quantities to 1e-9."""
+    say("case: FP32 acceptance energy/forces si64 L=1024")
+    r = si64_full_ref
+    s = si64_system
+    e_r, f_r = energy_and_forces_oracle(s, r["ref"], 2.0, 2.0)
+    out = {}
+    for prec in (hip.PREC_FP64, hip.PREC_FP32):
+        ctx = hip.Context(s)
+        C, _ = ctx.cheby_filter(r["chi"], r["coeffs"], r["scale"], r["emean"], prec=prec)
+        out[prec] = energy_and_forces_device(ctx, s, C, 2.0, 2.0)
+        ctx.close()
+    (e64, f64), (e32, f32) = out[hip.PREC_FP64], out[hip.PREC_FP32]
+    de32 = abs(e32 - e_r) / s.natm
+    df32 = float(np.max(np.abs(f32 - f_r)))
+    say("  E oracle %.10f, FP64 dE %.3g, FP32 dE/atom %.3g Ha; |F| max %.3g, FP32 dF %.3g Ha/bohr"
+        % (e_r, abs(e64 - e_r), de32, np.abs(f_r).max(), df32))
+    assert abs(e64 - e_r) < 1e-9 * abs(e_r)
+    assert np.max(np.abs(f64 - f_r)) < 1e-9 * max(1.0, np.abs(f_r).max())
+    assert de32 < 1e-4, de32
+    assert df32 < 1e-3, df32
 
 
 def test_moments_and_mu_si64_full_degree(hip, si64_system):

@@ -47,10 +47,10 @@ def shim():
     for f in ("filterChebyPolyHerm", "calcCoefForceWrapSCF", "calcChebyMoments", "calcChemPotCheby",
               "calcRhoStoHybridCheby", "filterNewtonPolyHerm", "genEnergyMax", "genEnergyMin",
               "calcKSPot"):
-        getattr(lib, f).argtypes = [C.POINTER(SdftStodft)]
-        getattr(lib, f).restype = None
-    lib.normHCheby.argtypes = [C.POINTER(SdftStodft), C.c_int, C.c_int]
-    lib.normHCheby.restype = None
+        getattr(lib, "sdft_" + f).argtypes = [C.POINTER(SdftStodft)]
+        getattr(lib, "sdft_" + f).restype = None
+    lib.sdft_normHCheby.argtypes = [C.POINTER(SdftStodft), C.c_int, C.c_int]
+    lib.sdft_normHCheby.restype = None
     return lib
 
 
@@ -84,7 +84,7 @@ def test_shim_filter_normh_mu_density(shim, si8_system):
                     chemPot=chem.ctypes.data_as(_dp), rhoUp=rho.ctypes.data_as(_dp))
     scale, emean = st.scale, st.energyMean
     acc_r, tl_r = O.filter_cheby(chi, coeffs, s, scale, emean)
-    shim.filterChebyPolyHerm(C.byref(st))
+    shim.sdft_filterChebyPolyHerm(C.byref(st))
     acc = (bufs["ore"] + 1j * bufs["oim"]).reshape(B, nc)
     tl = (bufs["cre"] + 1j * bufs["cim"]).reshape(B, nc)
     assert np.max(np.abs(acc - acc_r)) < 1e-10 * np.max(np.abs(acc_r))
@@ -96,20 +96,20 @@ def test_shim_filter_normh_mu_density(shim, si8_system):
     bufs["wim"][:] = chi.imag.ravel()
     acc2 = coeffs[0] * chi
     for k in range(1, L):
-        shim.normHCheby(C.byref(st), k, 1)
+        shim.sdft_normHCheby(C.byref(st), k, 1)
         acc2 = acc2 + coeffs[k] * (bufs["cre"] + 1j * bufs["cim"]).reshape(B, nc)
     assert np.max(np.abs(acc2 - acc_r)) < 1e-10 * np.max(np.abs(acc_r))
     # chemical potential from the moments of chi
     bufs["cre"][:] = chi.real.ravel()
     bufs["cim"][:] = chi.imag.ravel()
-    shim.calcChemPotCheby(C.byref(st))
+    shim.sdft_calcChemPotCheby(C.byref(st))
     m_r = O.cheby_moments(chi, s, L, scale, emean)
     mu_r, _ = O.chem_pot_cheby(m_r, B, L, emin, emax, 100.0, 16.0, 0.3, 0.4)
     assert abs(chem[0] - mu_r) < 1e-9
     # density of the filtered orbitals
     bufs["ore"][:] = acc_r.real.ravel()
     bufs["oim"][:] = acc_r.imag.ravel()
-    shim.calcRhoStoHybridCheby(C.byref(st))
+    shim.sdft_calcRhoStoHybridCheby(C.byref(st))
     rho_r = O.density_real(acc_r, s, 2.0 / B)
     assert np.max(np.abs(rho - rho_r)) < 1e-12 * np.max(np.abs(rho_r))
     ctx.close()
@@ -123,13 +123,14 @@ def test_shim_newton_filter_and_spectral_range(shim, si8_system):
     print("case: host shim newton + spectral range", flush=True)
     s = si8_system
     ctx = hip.Context(s)
-    B, L, nc, nmu = 3, 12, s.ncoef, 2
+    B, L, nc, nmu = 3, 24, s.ncoef, 2
     chi = systems.noise_orbitals(s, B, 8)
     emin, emax = stodft.spectral_bounds(s)
-    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
-    z = O.leja_order(np.cos(np.pi * (np.arange(L) + 0.5) / L))
-    co = np.stack([O.newton_coeffs(lambda y, m=m: np.sqrt(O.fermi(y / scale + emean, m, 4.0)), z)
-                   for m in (0.2, 0.4)], axis=1)
+    # PINY's Newton map (gen-stodft-wf.c:216-220) and its own sample points
+    # and coefficients (genSampNewtonHermit / genCoeffNewtonHermit)
+    scale, emean = 4.0 / (emax - emin), 0.5 * (emin + emax)
+    z, unscale = O.newton_samp_points(L, emin, scale)
+    co = O.newton_hermite_coeffs(z, unscale, [0.2, 0.4], 4.0)
     cre = np.ascontiguousarray(chi.real.ravel())
     cim = np.ascontiguousarray(chi.imag.ravel())
     fre, fim = np.zeros(B * nc), np.zeros(B * nc)
@@ -149,7 +150,7 @@ def test_shim_newton_filter_and_spectral_range(shim, si8_system):
     st.expanCoeff = coef.ctypes.data_as(_dp)
     st.numChemPot, st.newtonScale = nmu, scale
     st.sampPoint = zz.ctypes.data_as(_dp)
-    shim.filterNewtonPolyHerm(C.byref(st))
+    shim.sdft_filterNewtonPolyHerm(C.byref(st))
     out_r, pl_r = O.newton_filter(chi, co, z, s, scale, emean)
     for m in range(nmu):
         got = outs[m].reshape(B, nc) + 1j * outs[nmu + m].reshape(B, nc)
@@ -162,7 +163,7 @@ def test_shim_newton_filter_and_spectral_range(shim, si8_system):
     vim = np.ascontiguousarray(v0.imag.copy())
     st.testWfMaxRe, st.testWfMaxIm = vre.ctypes.data_as(_dp), vim.ctypes.data_as(_dp)
     st.energyMin = emin
-    shim.genEnergyMax(C.byref(st))
+    shim.sdft_genEnergyMax(C.byref(st))
     e_r, v_r, _ = O.gen_energy_bound(v0, s, 0, conv=1e-5)
     assert abs(st.energyMax - (e_r + 0.1)) < 1e-9 * abs(e_r)
     assert np.max(np.abs(vre + 1j * vim - v_r)) < 1e-8
@@ -189,7 +190,7 @@ def test_shim_ks_potential(shim, si8_system):
     st = SdftStodft(ctx=ctx.h, prec=0, rhoUp=rho.ctypes.data_as(_dp), vextr=one_based(vr),
                     vexti=one_based(vi), volume=s.volume, vxcType=O.XC_PW_LDA,
                     vKsUp=vks.ctypes.data_as(_dp))
-    shim.calcKSPot(C.byref(st))
+    shim.sdft_calcKSPot(C.byref(st))
     assert np.max(np.abs(vks - ref)) < 1e-10 * np.max(np.abs(ref))
     for k, v in (("eh", st.ehart), ("eext", st.eext), ("exc", st.exc), ("muxc", st.muxc)):
         assert abs(v - en[k]) <= 1e-10 * abs(en[k]), k

@@ -477,6 +477,92 @@ def test_newton_filter_matches_chebyshev_filter(tiny_system):
     assert np.max(np.abs(out[0] - acc)) < 1e-7 * np.max(np.abs(acc))
 
 
+# ---- Newton-Hermite sample points and coefficients (stodft/coeff.c) --------
+
+def test_newton_samp_points_are_discrete_leja_by_brute_force():
+    """genSampNewtonHermit restated: on the 32 L equally spaced candidates of
+    [-2, 2] the first point is Smin and every next point maximises the
+    product of distances to the points already chosen; checked here with
+    exact rational arithmetic over all candidates (no logarithms), so an
+    off-by-one candidate grid, a wrong start or a wrong objective fails."""
+    from fractions import Fraction
+    L = 7
+    emin, emax = -0.7, 5.3
+    ns = 4.0 / (emax - emin)
+    samp, unscale = O.newton_samp_points(L, emin, ns)
+    ncand = 32 * L
+    cand = [Fraction(-2) + Fraction(4 * i, ncand - 1) for i in range(ncand)]
+    chosen = [0]
+    for _ in range(1, L):
+        best, arg = None, None
+        for i, x in enumerate(cand):
+            if i in chosen:
+                continue
+            prod = Fraction(1)
+            for j in chosen:
+                prod *= (x - cand[j]) ** 2
+            if best is None or prod > best:
+                best, arg = prod, i
+        chosen.append(arg)
+    assert np.allclose(samp, [float(cand[i]) for i in chosen], rtol=0, atol=1e-14)
+    assert samp[0] == -2.0 and abs(samp[1] - 2.0) < 1e-14
+    # sampPointUnscale maps [Smin, Smax] onto [energyMin, energyMax]
+    assert np.allclose(unscale, (samp + 2.0) / ns + emin, rtol=0, atol=1e-13)
+    assert abs(unscale[0] - emin) < 1e-14 and abs(unscale[1] - emax) < 1e-12
+
+
+def test_newton_hermite_coeffs_interpolate_and_match_divided_differences():
+    """genCoeffNewtonHermit restated: the Newton form with its coefficients
+    reproduces sqrt(f_FD) at every sample point (the defining interpolation
+    conditions), and equals the classical divided-difference table
+    (O.newton_coeffs, an independent recursion) on the same nodes; for
+    two chemical potentials at once (numChemPot columns)."""
+    L = 40
+    emin, emax = -0.5, 4.5
+    ns = 4.0 / (emax - emin)
+    samp, unscale = O.newton_samp_points(L, emin, ns)
+    mus, beta = [0.8, 1.6], 6.0
+    a = O.newton_hermite_coeffs(samp, unscale, mus, beta)
+    assert a.shape == (L, 2)
+    for k, mu in enumerate(mus):
+        g = np.sqrt(O.fermi(unscale, mu, beta))
+        for i in range(L):
+            p = a[0, k] + sum(a[n, k] * np.prod(samp[i] - samp[:n]) for n in range(1, L))
+            assert abs(p - g[i]) < 1e-10, (k, i, p, g[i])
+        dd = O.newton_coeffs(lambda s, mu=mu: np.sqrt(O.fermi((s + 2.0) / ns + emin, mu, beta)),
+                             samp)
+        assert np.allclose(a[:, k], dd, rtol=1e-8, atol=1e-10)
+
+
+def test_newton_fit_error_matches_barycentric_interpolant_and_converges():
+    """calcFitErrorNewton restated: the Newton form it evaluates equals the
+    barycentric Lagrange interpolant on the same nodes (an independent
+    formula for the same polynomial) at the 1000 test points, and the fit
+    error of sqrt(f_FD) falls with the degree (beta = 100, the Si64 range)
+    below 1e-5 at L = 1024, as testChebyCoeff's criterion expects."""
+    emin, emax = -0.6, 6.8
+    ns = 4.0 / (emax - emin)
+    mu, beta = 0.3, 100.0
+    L = 48
+    samp, unscale = O.newton_samp_points(L, emin, ns)
+    a = O.newton_hermite_coeffs(samp, unscale, [mu], 6.0)
+    x = emin + (np.arange(1000) + 0.5) * (emax - emin) / 1000
+    s = (x - 0.5 * (emin + emax)) * ns
+    w = np.array([1.0 / np.prod([samp[j] - samp[k] for k in range(L) if k != j])
+                  for j in range(L)])
+    g = np.sqrt(O.fermi(unscale, mu, 6.0))
+    bary = np.array([np.sum(w * g / (si - samp)) / np.sum(w / (si - samp)) for si in s])
+    err = O.newton_fit_error(a, samp, emin, emax, ns, [mu], 6.0)
+    ref_err = float(np.max(np.abs(bary - np.sqrt(O.fermi(x, mu, 6.0)))))
+    assert abs(err - ref_err) < 1e-9 * max(1.0, ref_err) + 1e-12, (err, ref_err)
+    errs = []
+    for L in (256, 1024):
+        samp, unscale = O.newton_samp_points(L, emin, ns)
+        a = O.newton_hermite_coeffs(samp, unscale, [mu], beta)
+        errs.append(O.newton_fit_error(a, samp, emin, emax, ns, [mu], beta))
+    assert errs[1] < errs[0] and errs[1] < 1e-5, errs
+
+
 # ---- Kohn-Sham potential from the density -----------------------------------
 
 CA_EC = {1.0: -0.0600, 2.0: -0.0448, 5.0: -0.0281, 10.0: -0.0186}  # Ceperley-Alder (Ha)
@@ -494,6 +580,45 @@ def test_lda_correlation_matches_ceperley_alder(kind):
     ex = -0.75 * np.cbrt(3.0 * rho / math.pi)
     ec = e / rho - ex
     assert np.all(np.abs(ec - np.array([CA_EC[r] for r in rs])) < 1.0e-3), ec
+
+
+def _eps_c(rs, kind):
+    rho = 3.0 / (4.0 * math.pi * np.asarray(rs, dtype=np.float64) ** 3)
+    _, e, _ = O.xc_lda(rho, kind)
+    return e / rho + 0.75 * np.cbrt(3.0 * rho / math.pi)
+
+
+def test_lda_correlation_limits_pin_the_constants():
+    """Limits that the PZ and PW92 parametrisations are built to reproduce,
+    read off the oracle's output, each tight enough that a one-digit change of
+    the constant it fixes fails:
+    * high density, eps_c = A ln rs + B + O(rs ln rs): the slope in ln rs is
+      the Gell-Mann-Brueckner coefficient (1 - ln 2)/pi^2 = 0.0310907 (PW92
+      to 1e-6; PZ uses 0.0311, within 2e-5) and the constant term is PZ's
+      -0.048 / PW92's -0.046644 (to 1e-6: pins PZ's B and PW92's beta1 =
+      exp(-c1/2A)/2A through c1);
+    * PZ: its two branches meet at rs = 1 in value (|jump| < 4e-5, the
+      published constants leave 3.2e-5) and slope (< 3e-5): pins C and D;
+    * PW92 low density, rs eps_c -> -d0 + d1 rs^(-1/2) with the published
+      d0 = 0.4335, d1 = 1.4408 of its interpolation (alpha1 / beta4 and
+      beta3 / beta4 to four digits): within 5e-5 at rs = 1e6."""
+    r1, r2 = 1.0e-7, 1.0e-6
+    gmb = (1.0 - math.log(2.0)) / math.pi ** 2
+    for kind, tol_a, b_ref in ((O.XC_PZ_LDA, 2e-5, -0.048), (O.XC_PW_LDA, 1e-6, -0.046644)):
+        e1, e2 = _eps_c([r1, r2], kind)
+        A = (e2 - e1) / math.log(r2 / r1)
+        assert abs(A - gmb) < tol_a, (kind, A)
+        B = e1 - A * math.log(r1)
+        assert abs(B - b_ref) < 1e-6, (kind, B)
+    h = 1.0e-7
+    lo, hi = _eps_c([1.0 - h, 1.0 + h], O.XC_PZ_LDA)
+    assert abs(hi - lo) < 4e-5, hi - lo
+    d_lo = (_eps_c([1.0 - h], O.XC_PZ_LDA) - _eps_c([1.0 - 2 * h], O.XC_PZ_LDA))[0] / h
+    d_hi = (_eps_c([1.0 + 2 * h], O.XC_PZ_LDA) - _eps_c([1.0 + h], O.XC_PZ_LDA))[0] / h
+    assert abs(d_hi - d_lo) < 3e-5, (d_lo, d_hi)
+    rs = 1.0e6
+    lim = float(_eps_c([rs], O.XC_PW_LDA)[0] * rs)
+    assert abs(lim - (-0.4335 + 1.4408 / math.sqrt(rs))) < 5e-5, lim
 
 
 @pytest.mark.parametrize("kind", [O.XC_PZ_LDA, O.XC_PW_LDA, O.XC_PADE_LDA])
