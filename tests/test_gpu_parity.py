@@ -355,8 +355,11 @@ def test_newton_filter_si64_piny_points_l1024(hip, si64_system):
     """filterNewtonPolyHerm at 64^3 with the sample points and coefficients
     PINY generates (genSampNewtonHermit / genCoeffNewtonHermit) at L = 1024,
     kT = 0.01 Ha, two chemical-potential columns, one orbital pair, against
-    the oracle: FP64 within 1e-10 relative per column, FP32 within 1e-4; the
-    expansion's own fit error (calcFitErrorNewton) is below 1e-5 there."""
+    the oracle: FP64 within 1e-10 relative per column, FP32 within 1e-4.  The
+    expansion's own fit error (calcFitErrorNewton) at this degree is 5.1e-5
+    (bounded here by 1e-4); PINY would raise L until it falls below
+    poly_fit_error (default 1e-9), which changes the points, not the
+    recurrence under test."""
     from sdft_amd import systems
     say("case: newton filter si64 L=1024 PINY points, 2 mu columns (oracle ~20 s)")
     ctx = hip.Context(si64_system)
@@ -366,7 +369,7 @@ def test_newton_filter_si64_piny_points_l1024(hip, si64_system):
     co, z, ns, emean = piny_newton_set(L, emin, emax, mus, beta)
     fit = O.newton_fit_error(co, z, emin, emax, ns, mus, beta)
     say("  fit error %.3e" % fit)
-    assert fit < 1e-5
+    assert fit < 1e-4
     chi = systems.noise_orbitals(si64_system, 2, seed=13)
     out_r, pl_r = O.newton_filter(chi, co, z[:-1], si64_system, ns, emean)
     for prec, tol in ((hip.PREC_FP64, 1e-10), (hip.PREC_FP32, 1e-4)):
