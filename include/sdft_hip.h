@@ -159,6 +159,21 @@ int sdft_hip_chem_pot(sdft_ctx *ctx, int L, double emin, double emax, double bet
                       const double *moments, double nsto, double ne_true, double mu_init,
                       double gap, double occ, double tol, double *mu_out, double *ne_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops such as the mu
+ * search inside an SCF step, harnesses that forbid hipMalloc): the same
+ * search as sdft_hip_chem_pot, same arguments, results and errors, but its
+ * scratch (cosine table, h_j, moments, result: 2 roundFFT(32 L) + L + 4
+ * doubles) lives in a fixed per-device array of 4.3 MB that the library
+ * holds from load to process end, and the call allocates nothing.  L must
+ * not exceed 8192 (SDFT_EINVAL beyond).  All calls on one device share that
+ * array: calls that overlap on one device (several host threads, or several
+ * contexts' streams) race and must be serialised by the caller. */
+int sdft_hip_chem_pot_noalloc(sdft_ctx *ctx, int L, double emin, double emax, double beta,
+                              const double *moments, double nsto, double ne_true,
+                              double mu_init, double gap, double occ, double tol,
+                              double *mu_out, double *ne_out);
+
 /* Stochastic density rho(r) = occ_over_nsto * sum_s psi_s(r)^2 (psi from the
  * unnormalised G->r transform): rhoCalcRealStoHybrid stodft/density.c:43-274
  * and the scale of calcRhoStoHybridCheby :1478.  rho: n1*n2*n3 doubles in

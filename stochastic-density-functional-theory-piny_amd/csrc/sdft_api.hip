@@ -820,6 +820,7 @@ int sdft_hip_init(const sdft_grid_desc* desc, sdft_ctx** out) {
     return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
   }
   int rc = build_geometry(c, desc);
+  if (!rc) rc = noalloc_scratch_setup(c.device);
   if (rc) {
     free_ctx(c);
     delete h;
@@ -1282,6 +1283,19 @@ int sdft_hip_chem_pot(sdft_ctx* h, int L, double emin, double emax, double beta,
   }
   return chem_pot_device(h->c, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ, tol,
                          mu_out, ne_out);
+}
+
+int sdft_hip_chem_pot_noalloc(sdft_ctx* h, int L, double emin, double emax, double beta,
+                              const double* moments, double nsto, double ne_true, double mu_init,
+                              double gap, double occ, double tol, double* mu_out,
+                              double* ne_out) {
+  if (!h || L < 1 || !moments || !mu_out || !ne_out || !(emax > emin) || !(nsto > 0) ||
+      !(gap > 0) || !(tol > 0)) {
+    set_error("bad chemical-potential arguments");
+    return SDFT_EINVAL;
+  }
+  return chem_pot_device_noalloc(h->c, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap,
+                                 occ, tol, mu_out, ne_out);
 }
 
 

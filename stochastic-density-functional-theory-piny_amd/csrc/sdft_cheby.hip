@@ -317,6 +317,60 @@ int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta, const 
                      tol, mu_out, ne_out);
 }
 
+// ---- chemical-potential search without device allocation ----------------
+// A fixed per-device array holds the search's scratch for every L up to
+// kNoAllocMaxL: the largest roundFFT(32 L) there is 279936 (2^7 3^7), so
+// 2 * 279936 + 8192 + 4 doubles (4.3 MB).  Each device that loads the
+// library holds its own copy until the process ends.  The host never uses
+// the array's name as an address: at context creation the device address
+// is read once per device with hipGetSymbolAddress and kept in
+// g_scratch_dev[device].  Every no-allocation call on a device shares that
+// device's array, so such calls that overlap on one device (two host
+// threads, or two contexts' streams) race; nothing in it has to start at
+// zero (every element a search reads is written first by the same call).
+constexpr int kNoAllocMaxL = 8192;
+constexpr size_t kNoAllocDoubles = (size_t)2 * 279936 + kNoAllocMaxL + 4;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {nullptr};
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device ordinal out of range for the no-allocation scratch table");
+    return SDFT_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void* p = nullptr;
+  SDFT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_dev[device] = static_cast<double*>(p);
+  return 0;
+}
+
+int chem_pot_device_noalloc(Ctx& c, int L, double emin, double emax, double beta,
+                            const double* moments, double nsto, double ne_true, double mu_init,
+                            double gap, double occ, double tol, double* mu_out, double* ne_out) {
+  if (L > kNoAllocMaxL || chem_pot_scratch_doubles(L) > kNoAllocDoubles) {
+    set_error("sdft_hip_chem_pot_noalloc: L = " + std::to_string(L) +
+              " exceeds the fixed scratch (L <= " + std::to_string(kNoAllocMaxL) + ")");
+    return SDFT_EINVAL;
+  }
+  double* buf = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c.device >= 0 && c.device < kMaxDevices) buf = g_scratch_dev[c.device];
+  }
+  if (!buf) {
+    set_error("sdft_hip_chem_pot_noalloc: no scratch address for this device");
+    return SDFT_ESTATE;
+  }
+  return chem_pot_on(c, buf, L, emin, emax, beta, moments, nsto, ne_true, mu_init, gap, occ, tol,
+                     mu_out, ne_out);
+}
+
 int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3) {
   int rc = ensure_red(c, sizeof(double) * (size_t)3 * nsteps);
   if (rc) return rc;

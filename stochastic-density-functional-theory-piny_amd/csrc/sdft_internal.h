@@ -388,6 +388,13 @@ int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, doub
 int chem_pot_device(Ctx& c, int L, double emin, double emax, double beta,
                     const double* moments, double nsto, double ne_true, double mu_init,
                     double gap, double occ, double tol, double* mu_out, double* ne_out);
+// the same search on the library's fixed per-device scratch array (no device
+// allocation); noalloc_scratch_setup reads that array's address on `device`
+// (the current device) once, at context creation
+int noalloc_scratch_setup(int device);
+int chem_pot_device_noalloc(Ctx& c, int L, double emin, double emax, double beta,
+                            const double* moments, double nsto, double ne_true, double mu_init,
+                            double gap, double occ, double tol, double* mu_out, double* ne_out);
 int moments_reduce(Ctx& c, const double* mom_part, int nsteps, int nwg, double* out3);
 int energy_ke_device(Ctx& c, const double* X, int nstate, double* out);
 int energy_nl_device(Ctx& c, int nstate_pad, int nstate, double* out);

@@ -36,7 +36,8 @@ class GridDesc(C.Structure):
 EXPORTS = [
     "sdft_hip_init", "sdft_hip_finalize", "sdft_hip_last_error", "sdft_hip_set_vks",
     "sdft_hip_set_projectors", "sdft_hip_hpsi", "sdft_hip_cheby_coeffs", "sdft_hip_cheby_filter",
-    "sdft_hip_cheby_moments", "sdft_hip_chem_pot", "sdft_hip_density", "sdft_hip_energies",
+    "sdft_hip_cheby_moments", "sdft_hip_chem_pot", "sdft_hip_chem_pot_noalloc",
+    "sdft_hip_density", "sdft_hip_energies",
     "sdft_hip_block_create", "sdft_hip_block_destroy", "sdft_hip_block_set",
     "sdft_hip_block_filter", "sdft_hip_block_get", "sdft_hip_set_profiling",
     "sdft_hip_kernel_times", "sdft_hip_selftest_fft", "sdft_hip_selftest_copy",
@@ -76,6 +77,9 @@ def load(path: str | None = None):
         "sdft_hip_chem_pot": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_double,
                                C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
                                _dp], C.c_int),
+        "sdft_hip_chem_pot_noalloc": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, _dp, _dp], C.c_int),
         "sdft_hip_density": ([vp, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int, C.c_int], C.c_int),
         "sdft_hip_energies": ([vp, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp], C.c_int),
         "sdft_hip_block_create": ([vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
