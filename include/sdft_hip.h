@@ -83,15 +83,28 @@ int sdft_hip_set_projectors(sdft_ctx *ctx, int nproj, const double *proj_re,
  * a projector); energy[p] is its KB energy (vpsNormList, doubled for the m > 0
  * Re/Im rows as at :318-404); vol_elem = V/N (volElem).  The reference adds
  *   V psi(r) = sum_p energy_p beta_p(r) vol_elem sum_r' beta_p(r') psi(r')
- * to v_KS psi on the grid before the forward FFT.  Restricted to the
- * plane-wave set that operator is exactly a KB operator with table
- * Q_p(G) = N^(-1/2) sum_r beta_p(r) e^{iG.r} and energies energy_p vol_elem:
- * the library computes that table on the device (projectors scattered onto
- * dense grids, its own forward transform) and applies it with the same MFMA
- * products as sdft_hip_set_projectors, whose table it replaces. */
+ * to v_KS psi on the grid before the forward FFT.  The library applies
+ * that operator in this per-atom form on the real-space grid (see
+ * sdft_hip_set_nl_form); restricted to the plane-wave set it is exactly a KB
+ * operator with table Q_p(G) = N^(-1/2) sum_r beta_p(r) e^{iG.r} and energies
+ * energy_p vol_elem, which the library builds on the device when the force
+ * path or nl_form 1 needs it.  Replaces any G-space table. */
 int sdft_hip_set_projectors_real(sdft_ctx *ctx, int nproj, const long long *offsets,
                                  const int *grid_index, const double *values,
                                  const double *energy, double vol_elem);
+
+/* How real-space projectors are applied (no reference counterpart: a choice
+ * of this library).  form 0 (default): in their per-atom form, as the
+ * reference does -- each H application transforms the pair grids to real
+ * space, forms Z_p = sum_r beta_p(r) psi(r) over each projector's grid points
+ * (one [channels x points] . [points x pairs] product per atom), adds
+ * sum_p energy_p vol_elem beta_p(r) Z_p to v_KS psi at every grid point and
+ * transforms back; the cost grows with the projectors' grid points, not with
+ * P x ncoef.  form 1: through the equivalent G-space table (built on the
+ * device when first needed), applied by the MFMA products of
+ * sdft_hip_set_projectors.  sdft_hip_nl_forces always uses the table (it is
+ * built on its first call); fragment batches need form 1. */
+int sdft_hip_set_nl_form(sdft_ctx *ctx, int form);
 
 /* Raw coefficient force f = -dE/dc* of H applied to nstate orbitals
  * (-4 Hc for G != 0, -2 Hc at G = 0): calcCoefForceWrapSCF,

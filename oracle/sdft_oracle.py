@@ -34,6 +34,9 @@ Parity status of each function (how it is pinned; see tests/):
   density_real                              : sum(rho)/N = occ <chi|chi>/Nsto
       identity and the brute-force real-space sum
   energy_ke / energy_nl / nl_forces         : closed forms and FD derivatives
+  energy_nl_real                            : equals occ/Nsto sum_s <psi_s|V_NL|psi_s>
+      with V_NL psi from nl_apply_real through hpsi_raw (Gamma dot of the
+      raw-force difference)
   local_forces                              : central differences of E_ext
       (ks_potential's external energy) w.r.t. the atom positions
   xc_lda / ks_potential                     : published correlation energies
@@ -510,7 +513,7 @@ def cheby_eval(coeffs, y):
 # Moments and chemical potential (stodft/calc-chempot-chebyshev.c)
 # ---------------------------------------------------------------------------
 
-def cheby_moments(chi, sysm, L, scale, emean, with_nl=True):
+def cheby_moments(chi, sysm, L, scale, emean, with_nl=True, nl_real=None):
     """calcChebyMoments, calc-chempot-chebyshev.c:596-913 (summed over states,
     not yet normalised): mu_0 = <chi|chi>, mu_1 = <chi|T_1>, then for
     k = 2..L/2 one normHCheby per step and
@@ -522,13 +525,13 @@ def cheby_moments(chi, sysm, L, scale, emean, with_nl=True):
     mom = np.zeros(L + 1)
     mom[0] = gamma_dot(chi, chi).sum()
     c = chi.copy()
-    f = hpsi_raw(c, sysm, with_nl)
+    f = hpsi_raw(c, sysm, with_nl, nl_real)
     norm_h_cheby(c, f, chi, 1, scale, emean)
     mom[1] = gamma_dot(chi, f).sum()
     t_prev = chi.copy()
     c = f
     for k in range(2, nmom):
-        f = hpsi_raw(c, sysm, with_nl)
+        f = hpsi_raw(c, sysm, with_nl, nl_real)
         norm_h_cheby(c, f, t_prev, k, scale, emean)
         mom[k] = gamma_dot(chi, f).sum()
         mom[2 * k - 1] = 2.0 * gamma_dot(f, c).sum() - mom[1]
@@ -612,6 +615,28 @@ def energy_ke(C, ak2, occ, nsto):
 def energy_nl(C, proj, energy, occ, nsto):
     """calc-energy.c:160-191: E_NL = occ sum_s <psi_s|V_NL|psi_s> / N_sto."""
     return occ * nl_energy_per_state(C, proj, energy).sum() / nsto
+
+
+def energy_nl_real(C, sysm, rs, occ, nsto):
+    """Real-space KB energy, nlppKBRealEnergy (energy/cp/cp_energy_eext_nonloc_real.c:
+    137-372): per orbital the grid values psi(r) (the unnormalised G->r
+    transform), per projector channel dot = volElem sum_r beta_p(r) psi(r)
+    (:255-265, :284-286) and E += dot^2 vpsNorm_p / V (:270, :302; the caller's
+    energy_p carries the factor 2 of the m > 0 rows), scaled by occ / N_sto
+    (stodft/calc-energy.c:160-191).  Each packed pair's real and imaginary
+    parts are its two orbitals."""
+    nstate = C.shape[0]
+    shape = tuple(sysm.n)
+    zero = np.zeros(C.shape[1], dtype=np.complex128)
+    e = 0.0
+    for s in range(0, nstate, 2):
+        c2 = C[s + 1] if s + 1 < nstate else zero
+        z = to_r(pack_pair(C[s], c2, sysm.map, sysm.map_conj, shape)).ravel()
+        for p in range(len(rs.energy)):
+            sl = slice(rs.offsets[p], rs.offsets[p + 1])
+            d = rs.vol_elem * (rs.values[sl] @ z[rs.index[sl]])
+            e += (d.real ** 2 + d.imag ** 2) * rs.energy[p] / sysm.volume
+    return occ * e / nsto
 
 
 def nl_forces(C, sysm, occ, nsto):

@@ -268,6 +268,7 @@ static int build_geometry(Ctx& c, const sdft_grid_desc* d) {
 }
 
 static void free_ctx(Ctx& c) {
+  rs_free(c);
   if (c.dens) {
     free_ctx(*c.dens);
     delete c.dens;
@@ -466,8 +467,12 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
   int rc;
   if ((rc = ensure_W(c, w_bytes(c, npairs, b.prec)))) return rc;
   const void* fnl = nullptr;
+  // real-space KB in its per-atom form: applied on the real-space grid
+  // between the inverse and forward plane stages, no G-space products
+  const bool rs = rs_active(c);
+  const bool table = c.nproj > 0 && !rs;
   // the FP32 form's KB products read T in float
-  const bool x32 = b.prec == 1 && c.nproj > 0;
+  const bool x32 = b.prec == 1 && table;
   if (x32 && !(flags & kHaveX32)) {
     const size_t n = (size_t)b.nstate_pad * 2 * c.g.ncp;
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, cur,
@@ -476,8 +481,8 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
   }
   // The KB chain runs on its own stream, overlapping the FFT kernels; with
   // profiling on, each kernel's events are recorded on the stream it runs on.
-  const bool overlap = c.nproj > 0;
-  if (c.nproj > 0) {
+  const bool overlap = table;
+  if (table) {
     hipStream_t main = c.stream;
     if (overlap) {
       SDFT_HIP_CHECK(hipEventRecord(c.ev_cur, c.stream));
@@ -505,7 +510,24 @@ static int apply_step(Block& b, int mode, const double* cur, double* out, const 
     if ((rc = launch_expand(c, b.prec, cur, npairs, c.W))) return rc;
   }
   void* Wres = c.W;
-  {
+  if (rs) {
+    {
+      ProfScope ps(c, "plane_bc_to_r");
+      if ((rc = rs_to_grid(c, b.prec, c.W, npairs))) return rc;
+    }
+    {
+      ProfScope ps(c, "rs_project");
+      if ((rc = rs_project(c, b.prec, npairs))) return rc;
+    }
+    {
+      ProfScope ps(c, "rs_apply_vpsi");
+      if ((rc = rs_apply(c, b.prec, npairs))) return rc;
+    }
+    {
+      ProfScope ps(c, "plane_r_to_bc");
+      if ((rc = rs_from_grid(c, b.prec, c.W, npairs))) return rc;
+    }
+  } else {
     ProfScope ps(c, "plane_bc_vpsi");
     if ((rc = launch_plane(c, b.prec, c.W, npairs, &Wres))) return rc;
   }
@@ -697,31 +719,15 @@ static int set_projectors_impl(Ctx& c, int nproj, const double* pre, const doubl
 // grids (re = p even, im = p odd) in batches, transformed with the noise
 // path's pruned forward transform (c(G) = (1/N) sum_r f(r) e^{-iG.r} for both
 // real fields) and conjugated: Q = sqrt(N) conj(c).
-static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, const int* idx,
-                                    const double* val, const double* energy, double vol_elem) {
+static int real_table_impl(Ctx& c) {
+  int nproj;
+  const long long* off;
+  const int* idx;
+  const double *val, *energy;
+  double vol_elem;
+  rs_host_lists(c, &nproj, &off, &idx, &val, &energy, &vol_elem);
   if (nproj == 0) return set_projectors_impl(c, 0, nullptr, nullptr, nullptr);
   const long long N = c.g.N;
-  if (off[0] != 0) {
-    set_error("real-space projector offsets must start at 0");
-    return SDFT_EINVAL;
-  }
-  for (int p = 0; p < nproj; ++p) {
-    if (off[p + 1] < off[p]) {
-      set_error("real-space projector offsets must be non-decreasing");
-      return SDFT_EINVAL;
-    }
-    std::vector<int> ix(idx + off[p], idx + off[p + 1]);
-    for (int v : ix)
-      if (v < 0 || v >= N) {
-        set_error("real-space projector grid index out of range");
-        return SDFT_EINVAL;
-      }
-    std::sort(ix.begin(), ix.end());
-    if (std::adjacent_find(ix.begin(), ix.end()) != ix.end()) {
-      set_error("a real-space projector lists a grid point twice");
-      return SDFT_EINVAL;
-    }
-  }
   const long long nent = off[nproj];
   int rc = 0;
   long long* doff = nullptr;
@@ -787,6 +793,25 @@ static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, con
   std::vector<double> e(nproj);
   for (int p = 0; p < nproj; ++p) e[p] = energy[p] * vol_elem;
   return set_projectors_impl(c, nproj, qre.data(), qim.data(), e.data());
+}
+
+// The G-space table of the current real-space projectors, if not built yet
+// (the force path and nl_form 1 use it).
+static int ensure_real_table(Ctx& c) {
+  if (!c.rs || c.nproj > 0) return 0;
+  return real_table_impl(c);
+}
+
+// Real-space projectors: validated and kept in their per-atom form; any
+// G-space table is dropped (rebuilt from these lists on demand).
+static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, const int* idx,
+                                    const double* val, const double* energy, double vol_elem) {
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream_nl));
+  int rc = set_projectors_impl(c, 0, nullptr, nullptr, nullptr);
+  if (!rc) rc = rs_set(c, nproj, off, idx, val, energy, vol_elem);
+  if (!rc && c.nl_form == 1) rc = ensure_real_table(c);
+  return rc;
 }
 
 }  // namespace sdft
@@ -867,7 +892,19 @@ int sdft_hip_set_projectors(sdft_ctx* h, int nproj, const double* pre, const dou
     set_error("bad projector arguments");
     return SDFT_EINVAL;
   }
+  rs_free(h->c);  // a G-space table replaces any real-space projectors
   return set_projectors_impl(h->c, nproj, pre, pim, pe);
+}
+
+int sdft_hip_set_nl_form(sdft_ctx* h, int form) {
+  if (!h || (form != 0 && form != 1)) {
+    set_error("bad nonlocal form (0 per-atom real space, 1 G-space table)");
+    return SDFT_EINVAL;
+  }
+  Ctx& c = h->c;
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  c.nl_form = form;
+  return form == 1 ? ensure_real_table(c) : 0;
 }
 
 int sdft_hip_set_projectors_real(sdft_ctx* h, int nproj, const long long* offsets,
@@ -890,6 +927,9 @@ int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* 
   }
   Ctx& c = h->c;
   for (int a = 0; a < natm * 3; ++a) forces[a] = 0.0;
+  // real-space projectors: the forces use the G-space table they restrict to
+  int rc0 = ensure_real_table(c);
+  if (rc0) return rc0;
   if (c.nproj == 0) return 0;
   const int P = c.nproj;
   for (int p = 0; p < P; ++p)
@@ -1347,7 +1387,15 @@ int sdft_hip_energies(sdft_ctx* h, int nstate, const double* re, const double* i
   if (!rc) rc = block_put(b, b.X0, re, im);
   double ke = 0.0, nl = 0.0;
   if (!rc) rc = energy_ke_device(c, b.X0, nstate, &ke);
-  if (!rc && c.nproj > 0) {
+  if (!rc && rs_active(c)) {
+    // per-atom form: Z_p from the real-space grids, E = sum E_p dV |Z_p|^2 / N
+    const int npairs = b.nstate_pad / 2;
+    rc = ensure_W(c, w_bytes(c, npairs, 0));
+    if (!rc) rc = launch_expand(c, 0, b.X0, npairs, c.W);
+    if (!rc) rc = rs_to_grid(c, 0, c.W, npairs);
+    if (!rc) rc = rs_project(c, 0, npairs);
+    if (!rc) rc = rs_energy(c, npairs, nstate, &nl);
+  } else if (!rc && c.nproj > 0) {
     rc = launch_nl_project(c, 0, b.X0, b.nstate_pad);
     if (!rc) rc = energy_nl_device(c, b.nstate_pad, nstate, &nl);
   }

@@ -233,6 +233,11 @@ int build_args(Batch& B, int L, const double* coeffs_host, const double* scale,
   }
   for (int fi = 0; fi < nf; ++fi) {
     const Frag& f = B.frags[fi];
+    if (rs_active(*f.c)) {
+      set_error("fragment " + std::to_string(fi) + ": the context switched to the per-atom "
+                "real-space KB form after sdft_hip_batch_create");
+      return SDFT_ESTATE;
+    }
     if (f.c->nproj != f.P || f.c->ppad != f.ppad) {
       set_error("fragment " + std::to_string(fi) + ": the context's projectors changed after "
                 "sdft_hip_batch_create (the batch's KB buffers are sized for " +
@@ -470,6 +475,11 @@ int sdft_hip_batch_create(int nfrag, sdft_block* const* blocks, sdft_batch** out
     if (!b.ctx->have_vks) {
       set_error("v_KS not set on a fragment context (call sdft_hip_set_vks first)");
       return SDFT_ESTATE;
+    }
+    if (rs_active(*b.ctx)) {
+      set_error("fragment context with real-space projectors in their per-atom form: the batch "
+                "applies the G-space table (sdft_hip_set_nl_form(ctx, 1) first)");
+      return SDFT_EINVAL;
     }
     for (int j = 0; j < i; ++j)
       if (blocks[j] == blocks[i]) {

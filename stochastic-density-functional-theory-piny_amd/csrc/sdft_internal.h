@@ -87,6 +87,9 @@ struct DeviceTables {
 // system-wide cache writeback.
 constexpr unsigned kSyncEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
 
+// Per-atom real-space KB data (sdft_hip_set_projectors_real), see sdft_rs.hip.
+struct RsKB;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -148,6 +151,11 @@ struct Ctx {
   double* ks_q = nullptr;       // rho(G) of the last call (density sphere, internal order)
   bool ks_have_q = false;
   size_t ks_part_n = 0;
+  // real-space KB projectors in their per-atom form (null: none set);
+  // nl_form 0 applies them per atom on the real-space grid, 1 through the
+  // G-space table they restrict to (built on first use)
+  RsKB* rs = nullptr;
+  int nl_form = 0;
   // profiling
   bool profiling = false;
   std::vector<std::string> prof_names;
@@ -204,6 +212,7 @@ struct ExpandArgs {
 struct PlaneArgs {
   void* W;
   void* Wout;  // split-plane form only: the output buffer (W stays the input)
+  void* field; // real-space KB path: the pair grids [pair][ra][rb][rc]
   int ncolp;
   const int* rowcol;
   int M;
@@ -381,6 +390,26 @@ int launch_kb_batch(int kind, int prec, const void* args, const int* prefix, int
                     int total, hipStream_t s);
 int nl_force_sums(Ctx& c, const double* X, int ns, int nstate, const double* gint, double* Ad,
                   double* Zd, double* part, double* S);
+
+// ---- real-space KB, per-atom form (sdft_rs.hip) --------------------------
+// rs_set: validate the CSR projector lists and build the device data (the
+// groups of projectors sharing one point list, the point -> projector
+// inverse lists); keeps a host copy for rs_table_args.  rs_free releases it.
+int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double* val,
+           const double* energy, double vol_elem);
+void rs_free(Ctx& c);
+bool rs_active(const Ctx& c);
+// the host CSR (for building the equivalent G-space table)
+void rs_host_lists(const Ctx& c, int* nproj, const long long** off, const int** idx,
+                   const double** val, const double** energy, double* vol_elem);
+// the four stages of the per-atom form on npairs pairs (W in: the a-expansion;
+// W out: what the plane kernel would have produced), and the NL energy sum
+// sum_p E_p dV |Z_p|^2 / N over the pairs' orbitals
+int rs_to_grid(Ctx& c, int prec, void* W, int npairs);
+int rs_project(Ctx& c, int prec, int npairs);
+int rs_apply(Ctx& c, int prec, int npairs);
+int rs_from_grid(Ctx& c, int prec, void* W, int npairs);
+int rs_energy(Ctx& c, int npairs, int nstate, double* out);
 
 // ---- Chebyshev scalars (sdft_cheby.hip) ---------------------------------
 int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,

@@ -23,23 +23,10 @@
 // first and the last kernel instead of n^2, and no n^3 grid touches HBM.
 #include "fft_lds.h"
 #include "sdft_internal.h"
+#include "sdft_plane.h"
 
 namespace sdft {
 
-// ---- per-size configuration ----------------------------------------------
-constexpr int largest_divisor_le(int n, int cap) {
-  for (int d = cap; d >= 1; --d)
-    if (n % d == 0) return d;
-  return 1;
-}
-
-template <int N>
-struct Cfg {
-  static constexpr int ML = fft_max_lines(N, kThreads);  // lines per FFT call
-  static constexpr int CH = ML < 32 ? ML : 32;           // columns per chunk
-  static constexpr int CB = largest_divisor_le(N, ML < 16 ? ML : 16);
-  static constexpr int LD = N + 1;                       // padded LDS line stride
-};
 
 int chunk_cols_for(int n) {
   switch (n) {
@@ -55,27 +42,6 @@ int chunk_cols_for(int n) {
   }
 }
 
-
-// Thread mapping of the passes over lines that sit next to each other in
-// LDS (c rows of a plane, a columns of a chunk): with consecutive threads on
-// consecutive butterflies of one line, a first pass stores with a stride of R
-// elements, which for 16-byte (double) elements puts a line's butterflies in
-// one LDS bank; consecutive threads on consecutive lines (stride n + 1)
-// avoid that.  For the plane's c rows in float the line-major order keeps
-// the global gathers of the rows denser and measured faster (kLineFast);
-// the a columns of the expand / contract chunks are transformed in LDS only,
-// and take the line-fast order in both precisions (kLineFastA).
-template <typename T>
-constexpr bool kLineFast = sizeof(T) == 8;
-template <typename T>
-constexpr bool kLineFastA = true;
-
-template <typename T, int N>
-size_t plane_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * Cfg<N>::LD + (size_t)Cfg<N>::CB * Cfg<N>::LD + N);
-}
-
-__device__ inline int wrapi(int k, int n) { return k < 0 ? k + n : k; }
 
 // ---- block reduction of 3 doubles ----------------------------------------
 __device__ inline void block_sum3(double* v, double* sh, int tid) {
@@ -171,127 +137,6 @@ __global__ __launch_bounds__(kThreads) void k_expand_batch(const ExpandArgs* __r
   expand_body<T, N>(a, local % a.gx, local / a.gx);
 }
 
-// ---- K2: one a-plane: c and b transforms around v.psi --------------------
-// rowcol[row * N + pos] is the sphere column whose (kb, kc) is (row - K, pos)
-// (pos = kc mod N), or -1: the c-lines' first pass reads W through it and
-// their last pass writes W through it, so the plane is never zero-filled or
-// scattered.  The b-lines read and write only the 2K+1 rows |kb| <= K of the
-// c-transformed plane; their inverse last pass, the v_KS multiply and the
-// forward first pass run fused in registers.
-// Tile variants of the b-line sub-chunk CB (columns rc transformed together):
-// V = 0 the base width (one radix butterfly per thread and pass at n = 64),
-// V = 1 twice it (half the sub-chunks and barriers, two butterflies per
-// thread); the tuner picks per context or batch.  PR = true (for 4K < n, as
-// on every density grid of the reference, n >= 4K + 1) prunes the first pass
-// of the two inverse transforms (c and b: inputs with |k| >= n/4 are zero)
-// and the last pass of the two forward ones (outputs with |k| >= n/4 are
-// not kept); the c-forward runs in reversed radix order so that its last
-// pass is the radix-4/8 one.
-template <typename T, int N, int V = 0>
-struct PlaneCfg {
-  static constexpr int ML = Cfg<N>::ML;
-  static constexpr int CB0 = largest_divisor_le(N, ML < (sizeof(T) == 4 ? 32 : 16)
-                                                       ? ML
-                                                       : (sizeof(T) == 4 ? 32 : 16));
-  static constexpr int CB = V == 0 ? CB0 : largest_divisor_le(N, 2 * CB0 < N ? 2 * CB0 : N);
-};
-
-// The c rows of the plane buffer in float at n = 64 use the padded slot
-// layout (lds_slot<true>, row stride 72): their first passes run line-major
-// (for coalesced W gathers) and store with a stride of 8 elements.
-template <typename T, int N>
-struct PlaneRows {
-  static constexpr bool PAD = sizeof(T) == 4 && N == 64;
-  static constexpr int LDC = PAD ? N + N / 8 : Cfg<N>::LD;
-};
-
-template <typename T, int N, int V = 0>
-size_t plane2_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * PlaneRows<T, N>::LDC +
-                          (size_t)PlaneCfg<T, N, V>::CB * Cfg<N>::LD + N);
-}
-
-template <typename T, int N, int V, bool PR>
-__device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N, V>::CB;
-  constexpr bool PAD = PlaneRows<T, N>::PAD;
-  constexpr int LDC = PlaneRows<T, N>::LDC;
-  constexpr int P = n_passes(N);
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
-  const int ncolp = a.ncolp, M = a.M;
-  const int* __restrict__ rowcol = a.rowcol;
-  const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
-  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
-  Cx<T>* Pl = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows, stride LDC
-  Cx<T>* Q = Pl + M * LDC;
-  Cx<T>* tw = Q + CB * LD;
-  const int tid = threadIdx.x;
-  const int K = (M - 1) / 2;
-  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
-  __syncthreads();
-  Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
-  const Cx<T> zero = cx(T(0), T(0));
-
-  // inverse along c: rows of the plane, W -> LDS
-  for (int r0 = 0; r0 < M; r0 += ML) {
-    const int nl = M - r0 < ML ? M - r0 : ML;
-    const int* rc = rowcol + (size_t)r0 * N;
-    // loads are unconditional (a column outside the sphere reads entry 0)
-    // and masked by a multiply afterwards: one vector load per element, all
-    // of a butterfly's index and data loads in flight together, no
-    // divergent branch (a select would be turned back into a guarded load)
-    auto lw = [&](int l, int pos) {
-      const int d = rc[l * N + pos];
-      const Cx<T> x = Wp[d >= 0 ? d : 0];
-      const T m = d >= 0 ? T(1) : T(0);
-      return cx(x.x * m, x.y * m);
-    };
-    Cx<T>* buf = Pl + r0 * LDC;
-    auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LDC + lds_slot<PAD>(pos)] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, kLineFast<T>, PR, false, PAD>(
-        buf, LDC, nl, tw, tid, lw, sb);
-  }
-
-  // along b, CB columns rc at a time
-  const T* vplane = v + (size_t)ra * N * N;
-  for (int rc0 = 0; rc0 < N; rc0 += CB) {
-    auto lp = [&](int l, int pos) {
-      const int kb = pos <= N / 2 ? pos : pos - N;
-      const bool in = kb >= -K && kb <= K;
-      const Cx<T> x = Pl[(in ? kb + K : 0) * LDC + lds_slot<PAD>(rc0 + l)];
-      return in ? x : zero;
-    };
-    auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
-    auto sq = [&](int l, int pos, Cx<T> x) { Q[l * LD + pos] = x; };
-    auto vf = [&](int l, int pos) { return vplane[pos * N + rc0 + l]; };
-    auto sp = [&](int l, int pos, Cx<T> x) {
-      const int kb = pos <= N / 2 ? pos : pos - N;
-      if (kb >= -K && kb <= K) Pl[(kb + K) * LDC + lds_slot<PAD>(rc0 + l)] = x;
-    };
-    if constexpr (P >= 2)
-      fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true, PR, false>(Q, LD, CB, tw, tid,
-                                                                              lp, sq);
-    fft_fused_vmul<T, N, CB, kThreads, true>(CB, tw, tid, lq, vf, sq);
-    if constexpr (P >= 2)
-      fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true, false, PR>(Q, LD, CB, tw, tid,
-                                                                             lq, sp);
-  }
-
-  // forward along c: LDS -> W
-  for (int r0 = 0; r0 < M; r0 += ML) {
-    const int nl = M - r0 < ML ? M - r0 : ML;
-    const int* rc = rowcol + (size_t)r0 * N;
-    Cx<T>* buf = Pl + r0 * LDC;
-    auto lb = [&](int l, int pos) { return buf[l * LDC + lds_slot<PAD>(pos)]; };
-    auto sw = [&](int l, int pos, Cx<T> x) {
-      const int d = rc[l * N + pos];
-      if (d >= 0) Wp[d] = x;
-    };
-    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, kLineFast<T>, false, PR, PAD>(
-        buf, LDC, nl, tw, tid, lb, sw);
-  }
-}
 
 template <typename T, int N, int V, bool PR>
 __global__ __launch_bounds__(kThreads) void k_plane(PlaneArgs a) {
@@ -308,107 +153,6 @@ __global__ __launch_bounds__(kThreads) void k_plane_batch(const PlaneArgs* __res
   plane_body<T, N, V, PR>(a, local % a.gx, local / a.gx);
 }
 
-// ---- K2, split form: for planes whose M x (n+1) buffer exceeds LDS --------
-// The plane is processed in two column halves (rc in [h n/2, (h+1) n/2)).
-// Per half the inverse c-transform of the M rows is recomputed from W (ML
-// rows at a time through Q), only the half's columns are kept in LDS
-// (M x (n/2+1)), their b-lines run as in plane_body, and the forward
-// c-transform of the rows with the other half's entries zero is stored
-// (h = 0) or added (h = 1) to Wout: the forward transform is linear in its
-// input, and W (the input of both halves) is left intact.
-template <typename T, int N>
-struct SplitCfg {
-  static constexpr int H = N / 2, LDH = H + 1;
-  static constexpr int QL = Cfg<N>::ML > PlaneCfg<T, N>::CB ? Cfg<N>::ML : PlaneCfg<T, N>::CB;
-};
-
-template <typename T, int N>
-size_t plane_split_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N>::LDH + (size_t)SplitCfg<T, N>::QL *
-                          Cfg<N>::LD + N);
-}
-
-template <typename T, int N>
-__device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
-  constexpr int H = SplitCfg<T, N>::H, LDH = SplitCfg<T, N>::LDH, QL = SplitCfg<T, N>::QL;
-  constexpr int P = n_passes(N);
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
-  Cx<T>* __restrict__ Wo = reinterpret_cast<Cx<T>*>(a.Wout);
-  const int ncolp = a.ncolp, M = a.M;
-  const int* __restrict__ rowcol = a.rowcol;
-  const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
-  const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
-  Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows x the half's columns
-  Cx<T>* Q = Ph + M * LDH;                         // QL lines of length N
-  Cx<T>* tw = Q + QL * LD;
-  const int tid = threadIdx.x;
-  const int K = (M - 1) / 2;
-  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
-  __syncthreads();
-  const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
-  Cx<T>* Wq = Wo + ((size_t)q * N + ra) * ncolp;
-  const T* vplane = v + (size_t)ra * N * N;
-  const Cx<T> zero = cx(T(0), T(0));
-  for (int h = 0; h < 2; ++h) {
-    const int c_lo = h * H;
-    // inverse along c, keeping this half's columns
-    for (int r0 = 0; r0 < M; r0 += ML) {
-      const int nl = M - r0 < ML ? M - r0 : ML;
-      const int* rc = rowcol + (size_t)r0 * N;
-      auto lw = [&](int l, int pos) {
-        const int d = rc[l * N + pos];
-        const Cx<T> x = Wp[d >= 0 ? d : 0];
-        const T m = d >= 0 ? T(1) : T(0);
-        return cx(x.x * m, x.y * m);
-      };
-      auto sh = [&](int l, int pos, Cx<T> x) {
-        const int c = pos - c_lo;
-        if (c >= 0 && c < H) Ph[(r0 + l) * LDH + c] = x;
-      };
-      fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lw, sh);
-    }
-    // along b, CB columns at a time
-    for (int rc0 = 0; rc0 < H; rc0 += CB) {
-      const int nb = H - rc0 < CB ? H - rc0 : CB;
-      auto lp = [&](int l, int pos) {
-        const int kb = pos <= N / 2 ? pos : pos - N;
-        return (kb >= -K && kb <= K) ? Ph[(kb + K) * LDH + rc0 + l] : zero;
-      };
-      auto lq = [&](int l, int pos) { return Q[l * LD + pos]; };
-      auto sq = [&](int l, int pos, Cx<T> x) { Q[l * LD + pos] = x; };
-      auto vf = [&](int l, int pos) { return vplane[pos * N + c_lo + rc0 + l]; };
-      auto sp = [&](int l, int pos, Cx<T> x) {
-        const int kb = pos <= N / 2 ? pos : pos - N;
-        if (kb >= -K && kb <= K) Ph[(kb + K) * LDH + rc0 + l] = x;
-      };
-      if constexpr (P >= 2)
-        fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true>(Q, LD, nb, tw, tid, lp, sq);
-      fft_fused_vmul<T, N, CB, kThreads, true>(nb, tw, tid, lq, vf, sq);
-      if constexpr (P >= 2)
-        fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lq, sp);
-    }
-    // forward along c of the rows with the other half zero: store / add
-    for (int r0 = 0; r0 < M; r0 += ML) {
-      const int nl = M - r0 < ML ? M - r0 : ML;
-      const int* rc = rowcol + (size_t)r0 * N;
-      auto lb = [&](int l, int pos) {
-        const int c = pos - c_lo;
-        return (c >= 0 && c < H) ? Ph[(r0 + l) * LDH + c] : zero;
-      };
-      auto sw = [&](int l, int pos, Cx<T> x) {
-        const int d = rc[l * N + pos];
-        if (d < 0) return;
-        if (h == 0)
-          Wq[d] = x;
-        else
-          Wq[d] = Wq[d] + x;
-      };
-      fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lb, sw);
-    }
-  }
-}
 
 template <typename T, int N>
 __global__ __launch_bounds__(kThreads) void k_plane_split(PlaneArgs a) {
@@ -911,23 +655,6 @@ __global__ __launch_bounds__(kThreads) void k_fft_test(const double* __restrict_
 // launchers
 // ==========================================================================
 
-constexpr int SDFT_EINVAL_CODE = -1;
-
-#define SDFT_SIZE_SWITCH(n, CALL)          \
-  switch (n) {                             \
-    case 24: { constexpr int NN = 24; CALL; } break;   \
-    case 32: { constexpr int NN = 32; CALL; } break;   \
-    case 36: { constexpr int NN = 36; CALL; } break;   \
-    case 48: { constexpr int NN = 48; CALL; } break;   \
-    case 64: { constexpr int NN = 64; CALL; } break;   \
-    case 96: { constexpr int NN = 96; CALL; } break;   \
-    case 128: { constexpr int NN = 128; CALL; } break; \
-    case 192: { constexpr int NN = 192; CALL; } break; \
-    default: set_error("unsupported grid size"); return SDFT_EINVAL_CODE; \
-  }
-
-template <typename T>
-constexpr int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
 
 ExpandArgs make_expand_args(const Ctx& c, int prec, const double* X, void* W) {
   ExpandArgs a;
@@ -949,6 +676,7 @@ PlaneArgs make_plane_args(const Ctx& c, int prec, void* W) {
   PlaneArgs a;
   a.W = W;
   a.Wout = nullptr;
+  a.field = nullptr;
   a.ncolp = c.g.ncolp;
   a.rowcol = c.t.rowcol;
   a.M = c.g.M;

@@ -1,0 +1,527 @@
+// sdft_rs.hip -- the real-space Kleinman-Bylander term in its per-atom form:
+// nlppKBRealEnergyFilter (energy/cp/cp_energy_eext_nonloc_real_filter.c:
+// 110-450), which controlPotentialEnergyFilter (:36-105) adds to v.psi on the
+// real-space grid before the forward FFT.
+//
+// Per H application on npairs Gamma-packed pairs z = psi1 + i psi2:
+//   rs_to_grid   (plane stage, MODE 1): W -> z(r) on the pair grids
+//                F[pair][ra][rb][rc] (inverse c and b transforms);
+//   rs_project   k_rs_project: Z_p(pair) = sum_k beta_p(r_k) z(r_k) over the
+//                projector's grid points -- the gather of :246-258 and the
+//                per-channel dots of :318-404 -- as one [nch x K_a] .
+//                [K_a x pairs] product per atom (the channels of an atom
+//                share its point list, gridNlppMap);
+//   rs_apply     k_rs_apply: z(r) <- v(r) z(r) + sum_p w_p(r) Z_p(pair) with
+//                w_p = E_p vol_elem beta_p, at every grid point through the
+//                point -> projector inverse lists -- the daxpy into
+//                forceTemp and the scatter-add of :404-441 plus z v (:78, :97);
+//   rs_from_grid (plane stage, MODE 2): forward b and c transforms back to W.
+// beta is real, so the complex z carries both orbitals of a pair through the
+// arithmetic the reference applies to the real and imaginary parts in turn.
+// Cost per H application: O(sum_a K_a) for the gather and scatter, against
+// O(P ncoef) for the equivalent G-space table.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sdft_hip.h"
+#include "sdft_plane.h"
+
+namespace sdft {
+
+struct RsKB {
+  int nproj = 0;
+  int ngroup = 0;
+  long long nent = 0;      // entries over all projectors
+  long long nent_g = 0;    // entries over the groups (one per shared point)
+  // groups: consecutive projectors (at most 4) with one point list
+  long long* goff = nullptr;  // [ngroup + 1]
+  int* gidx = nullptr;        // [nent_g] FFTW linear grid index
+  double* gval64 = nullptr;   // [nent_g][4] beta of the group's channels (0 past nch)
+  float* gval32 = nullptr;
+  int* gp0 = nullptr;         // [ngroup] first projector
+  int* gnch = nullptr;        // [ngroup] channels
+  // inverse lists: grid point -> (projector, E_p vol_elem beta_p(r))
+  long long* ptoff = nullptr; // [N + 1]
+  int* ptp = nullptr;         // [nent]
+  double* ptw64 = nullptr;    // [nent]
+  float* ptw32 = nullptr;
+  double* escale = nullptr;   // [nproj] E_p vol_elem / N (energies)
+  // per-call buffers
+  void* F = nullptr;          // pair grids, complex in the form's type
+  size_t F_bytes = 0;
+  void* Z = nullptr;          // [nproj][npairs] complex in the form's type
+  size_t Z_bytes = 0;
+  double* epart = nullptr;    // energy partials
+  size_t epart_bytes = 0;
+  // the caller's lists (host), for the equivalent G-space table on demand
+  std::vector<long long> off;
+  std::vector<int> idx;
+  std::vector<double> val, energy;
+  double vol_elem = 0.0;
+};
+
+constexpr int SDFT_ENOMEM_CODE = -3;
+
+template <typename P>
+static int rs_alloc(P** p, size_t bytes) {
+  *p = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(p), bytes ? bytes : 16) != hipSuccess) {
+    *p = nullptr;
+    set_error("device allocation failed (real-space projectors, " + std::to_string(bytes) +
+              " bytes)");
+    return SDFT_ENOMEM_CODE;
+  }
+  return 0;
+}
+
+template <typename V>
+static int rs_upload(V** dst, const std::vector<V>& src, hipStream_t s) {
+  int rc = rs_alloc(dst, sizeof(V) * src.size());
+  if (rc) return rc;
+  if (!src.empty())
+    SDFT_HIP_CHECK(hipMemcpyAsync(*dst, src.data(), sizeof(V) * src.size(),
+                                  hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+static int rs_grow(void** p, size_t* have, size_t need) {
+  if (*have >= need) return 0;
+  if (*p) SDFT_HIP_CHECK(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  int rc = rs_alloc(p, need);
+  if (rc) return rc;
+  *have = need;
+  return 0;
+}
+
+static void rs_release(RsKB* r) {
+  void* ptrs[] = {r->goff,  r->gidx,  r->gval64, r->gval32, r->gp0, r->gnch, r->ptoff,
+                  r->ptp,   r->ptw64, r->ptw32,  r->escale, r->F,   r->Z,    r->epart};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete r;
+}
+
+void rs_free(Ctx& c) {
+  if (c.rs) {
+    (void)hipStreamSynchronize(c.stream);
+    rs_release(c.rs);
+  }
+  c.rs = nullptr;
+}
+
+bool rs_active(const Ctx& c) { return c.rs != nullptr && c.nl_form == 0; }
+
+void rs_host_lists(const Ctx& c, int* nproj, const long long** off, const int** idx,
+                   const double** val, const double** energy, double* vol_elem) {
+  const RsKB* r = c.rs;
+  *nproj = r ? r->nproj : 0;
+  *off = r ? r->off.data() : nullptr;
+  *idx = r ? r->idx.data() : nullptr;
+  *val = r ? r->val.data() : nullptr;
+  *energy = r ? r->energy.data() : nullptr;
+  *vol_elem = r ? r->vol_elem : 0.0;
+}
+
+int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double* val,
+           const double* energy, double vol_elem) {
+  rs_free(c);
+  if (nproj == 0) return 0;
+  const long long N = c.g.N;
+  if (off[0] != 0) {
+    set_error("real-space projector offsets must start at 0");
+    return SDFT_EINVAL;
+  }
+  for (int p = 0; p < nproj; ++p) {
+    if (off[p + 1] < off[p]) {
+      set_error("real-space projector offsets must be non-decreasing");
+      return SDFT_EINVAL;
+    }
+    std::vector<int> ix(idx + off[p], idx + off[p + 1]);
+    for (int v : ix)
+      if (v < 0 || v >= N) {
+        set_error("real-space projector grid index out of range");
+        return SDFT_EINVAL;
+      }
+    std::sort(ix.begin(), ix.end());
+    if (std::adjacent_find(ix.begin(), ix.end()) != ix.end()) {
+      set_error("a real-space projector lists a grid point twice");
+      return SDFT_EINVAL;
+    }
+    if (!std::isfinite(energy[p])) {
+      set_error("real-space projector energy is not finite");
+      return SDFT_EINVAL;
+    }
+  }
+  const long long nent = off[nproj];
+  for (long long e = 0; e < nent; ++e)
+    if (!std::isfinite(val[e])) {
+      set_error("real-space projector value is not finite");
+      return SDFT_EINVAL;
+    }
+  if (nent >= (1LL << 31)) {
+    set_error("real-space projectors: more than 2^31 entries");
+    return SDFT_EINVAL;
+  }
+  RsKB* r = new RsKB();
+  r->nproj = nproj;
+  r->nent = nent;
+  r->vol_elem = vol_elem;
+  r->off.assign(off, off + nproj + 1);
+  r->idx.assign(idx, idx + nent);
+  r->val.assign(val, val + nent);
+  r->energy.assign(energy, energy + nproj);
+  // groups of consecutive projectors with identical point lists (an atom's
+  // channels), at most 4 per group
+  std::vector<long long> goff{0};
+  std::vector<int> gidx, gp0, gnch;
+  std::vector<double> gval;
+  for (int p = 0; p < nproj;) {
+    const long long k0 = off[p], kn = off[p + 1] - off[p];
+    int nch = 1;
+    while (nch < 4 && p + nch < nproj && off[p + nch + 1] - off[p + nch] == kn &&
+           std::equal(idx + k0, idx + k0 + kn, idx + off[p + nch]))
+      ++nch;
+    gp0.push_back(p);
+    gnch.push_back(nch);
+    for (long long k = 0; k < kn; ++k) {
+      gidx.push_back(idx[k0 + k]);
+      for (int ch = 0; ch < 4; ++ch) gval.push_back(ch < nch ? val[off[p + ch] + k] : 0.0);
+    }
+    goff.push_back(goff.back() + kn);
+    p += nch;
+  }
+  r->ngroup = (int)gp0.size();
+  r->nent_g = goff.back();
+  // inverse lists by grid point, projectors ascending within a point
+  std::vector<long long> ptoff((size_t)N + 1, 0);
+  for (long long e = 0; e < nent; ++e) ++ptoff[(size_t)idx[e] + 1];
+  for (long long i = 0; i < N; ++i) ptoff[(size_t)i + 1] += ptoff[(size_t)i];
+  std::vector<long long> fill(ptoff.begin(), ptoff.end() - 1);
+  std::vector<int> ptp((size_t)nent);
+  std::vector<double> ptw((size_t)nent);
+  for (int p = 0; p < nproj; ++p) {
+    const double s = energy[p] * vol_elem;
+    for (long long e = off[p]; e < off[p + 1]; ++e) {
+      const long long d = fill[(size_t)idx[e]]++;
+      ptp[(size_t)d] = p;
+      ptw[(size_t)d] = s * val[e];
+    }
+  }
+  std::vector<float> gval32(gval.begin(), gval.end());
+  std::vector<float> ptw32(ptw.begin(), ptw.end());
+  std::vector<double> esc(nproj);
+  for (int p = 0; p < nproj; ++p) esc[p] = energy[p] * vol_elem / (double)N;
+  int rc = 0;
+  if (!rc) rc = rs_upload(&r->goff, goff, c.stream);
+  if (!rc) rc = rs_upload(&r->gidx, gidx, c.stream);
+  if (!rc) rc = rs_upload(&r->gval64, gval, c.stream);
+  if (!rc) rc = rs_upload(&r->gval32, gval32, c.stream);
+  if (!rc) rc = rs_upload(&r->gp0, gp0, c.stream);
+  if (!rc) rc = rs_upload(&r->gnch, gnch, c.stream);
+  if (!rc) rc = rs_upload(&r->ptoff, ptoff, c.stream);
+  if (!rc) rc = rs_upload(&r->ptp, ptp, c.stream);
+  if (!rc) rc = rs_upload(&r->ptw64, ptw, c.stream);
+  if (!rc) rc = rs_upload(&r->ptw32, ptw32, c.stream);
+  if (!rc) rc = rs_upload(&r->escale, esc, c.stream);
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) {
+    set_error("real-space projector upload failed");
+    rc = SDFT_EHIP;
+  }
+  if (rc) {
+    rs_release(r);
+    return rc;
+  }
+  c.rs = r;
+  return 0;
+}
+
+// ---- kernels ----------------------------------------------------------------
+
+template <typename T, int N, bool PR, int MODE>
+__global__ __launch_bounds__(kThreads) void k_plane_field(PlaneArgs a) {
+  plane_body<T, N, 0, PR, MODE>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_plane_split_field(PlaneArgs a) {
+  plane_split_body<T, N, MODE>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int kRsThreads = 256;
+
+// Z[p][q] = sum_k beta_p[k] z_q(r_k) for the channels p of group g = blockIdx.x
+// and the pairs q0 <= q < q0 + QC (q0 = blockIdx.y QC).  Each thread keeps
+// QC x 4 complex partial sums over its share of the group's points; the wave
+// sums, then the workgroup's four waves, give Z.
+template <typename T, int QC>
+__global__ __launch_bounds__(kRsThreads) void k_rs_project(
+    const long long* __restrict__ goff, const int* __restrict__ gidx, const T* __restrict__ gval,
+    const int* __restrict__ gp0, const int* __restrict__ gnch, const Cx<T>* __restrict__ F,
+    long long Ngrid, int npairs, Cx<T>* __restrict__ Z) {
+  __shared__ T red[kRsThreads / 64][QC * 8];
+  const int g = blockIdx.x;
+  const int q0 = blockIdx.y * QC;
+  const int nq = npairs - q0 < QC ? npairs - q0 : QC;
+  const long long e0 = goff[g], e1 = goff[g + 1];
+  T ar[QC][4], ai[QC][4];
+#pragma unroll
+  for (int qq = 0; qq < QC; ++qq)
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      ar[qq][ch] = T(0);
+      ai[qq][ch] = T(0);
+    }
+  const Cx<T>* Fq = F + (size_t)q0 * Ngrid;
+  for (long long e = e0 + threadIdx.x; e < e1; e += kRsThreads) {
+    const size_t r = (size_t)gidx[e];
+    T b[4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) b[ch] = gval[e * 4 + ch];
+#pragma unroll
+    for (int qq = 0; qq < QC; ++qq) {
+      if (qq < nq) {
+        const Cx<T> z = Fq[(size_t)qq * Ngrid + r];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+          ar[qq][ch] += b[ch] * z.x;
+          ai[qq][ch] += b[ch] * z.y;
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int qq = 0; qq < QC; ++qq)
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      const T sr = wave_sum(ar[qq][ch]);
+      const T si = wave_sum(ai[qq][ch]);
+      if (lane == 0) {
+        red[wave][(qq * 4 + ch) * 2 + 0] = sr;
+        red[wave][(qq * 4 + ch) * 2 + 1] = si;
+      }
+    }
+  __syncthreads();
+  const int nch = gnch[g], p0 = gp0[g];
+  for (int i = threadIdx.x; i < QC * 4; i += kRsThreads) {
+    const int qq = i >> 2, ch = i & 3;
+    if (qq < nq && ch < nch) {
+      T sr = T(0), si = T(0);
+#pragma unroll
+      for (int w = 0; w < kRsThreads / 64; ++w) {
+        sr += red[w][i * 2 + 0];
+        si += red[w][i * 2 + 1];
+      }
+      Z[(size_t)(p0 + ch) * npairs + q0 + qq] = cx(sr, si);
+    }
+  }
+}
+
+// z_q(r) <- v(r) z_q(r) + sum_{e in point r} w_e Z[p_e][q] for the pairs
+// q0 <= q < q0 + QC, one grid point per thread.
+template <typename T, int QC>
+__global__ __launch_bounds__(kRsThreads) void k_rs_apply(
+    const long long* __restrict__ ptoff, const int* __restrict__ ptp, const T* __restrict__ ptw,
+    const T* __restrict__ v, const Cx<T>* __restrict__ Z, long long Ngrid, int npairs,
+    Cx<T>* __restrict__ F) {
+  const long long r = (long long)blockIdx.x * kRsThreads + threadIdx.x;
+  if (r >= Ngrid) return;
+  const int q0 = blockIdx.y * QC;
+  const int nq = npairs - q0 < QC ? npairs - q0 : QC;
+  const T vr = v[r];
+  Cx<T> acc[QC];
+  Cx<T>* Fq = F + (size_t)q0 * Ngrid + r;
+#pragma unroll
+  for (int qq = 0; qq < QC; ++qq) {
+    if (qq < nq) {
+      const Cx<T> z = Fq[(size_t)qq * Ngrid];
+      acc[qq] = cx(vr * z.x, vr * z.y);
+    }
+  }
+  const long long e1 = ptoff[r + 1];
+  for (long long e = ptoff[r]; e < e1; ++e) {
+    const T w = ptw[e];
+    const Cx<T>* zp = Z + (size_t)ptp[e] * npairs + q0;
+#pragma unroll
+    for (int qq = 0; qq < QC; ++qq) {
+      if (qq < nq) {
+        const Cx<T> zz = zp[qq];
+        acc[qq].x += w * zz.x;
+        acc[qq].y += w * zz.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int qq = 0; qq < QC; ++qq)
+    if (qq < nq) Fq[(size_t)qq * Ngrid] = acc[qq];
+}
+
+// part[b] = sum over (p, q) of escale[p] (Re Z^2 + Im Z^2), the imaginary
+// part only where orbital 2q + 1 exists
+__global__ void k_rs_energy(const Cx<double>* __restrict__ Z, const double* __restrict__ esc,
+                            int nproj, int npairs, int nstate, double* __restrict__ part) {
+  __shared__ double sh[kRsThreads / 64];
+  double acc = 0.0;
+  const long long tot = (long long)nproj * npairs;
+  for (long long i = (long long)blockIdx.x * kRsThreads + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * kRsThreads) {
+    const int p = (int)(i / npairs), q = (int)(i - (long long)p * npairs);
+    const Cx<double> z = Z[i];
+    acc += esc[p] * (z.x * z.x + (2 * q + 1 < nstate ? z.y * z.y : 0.0));
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kRsThreads / 64; ++w) s += sh[w];
+    part[blockIdx.x] = s;
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+
+template <typename T, int N, int MODE>
+static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
+  const Geometry& g = c.g;
+  PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
+  a.field = F;
+  const dim3 grid(N, npairs);
+  const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
+  if (lds <= 160 * 1024) {
+    if (plane_prunable(g.K, N)) {
+      SDFT_HIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&k_plane_field<T, N, true, MODE>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_plane_field<T, N, true, MODE>), grid, dim3(kThreads), lds, c.stream,
+                         a);
+    } else {
+      SDFT_HIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&k_plane_field<T, N, false, MODE>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_plane_field<T, N, false, MODE>), grid, dim3(kThreads), lds, c.stream,
+                         a);
+    }
+    SDFT_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
+  if (lds2 > 160 * 1024) {
+    set_error("plane buffer exceeds LDS for this grid in this precision");
+    return SDFT_EINVAL_CODE;
+  }
+  a.Wout = W;  // MODE 2 writes the result in place of its (unread) input
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_split_field<T, N, MODE>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+  hipLaunchKernelGGL((k_plane_split_field<T, N, MODE>), grid, dim3(kThreads), lds2, c.stream, a);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <int MODE>
+static int plane_field(Ctx& c, int prec, void* W, int npairs) {
+  RsKB& r = *c.rs;
+  const size_t need = (prec == 0 ? sizeof(Cx<double>) : sizeof(Cx<float>)) * (size_t)npairs *
+                      (size_t)c.g.N;
+  if (MODE == 1) {
+    int rc = rs_grow(&r.F, &r.F_bytes, need);
+    if (rc) return rc;
+  } else if (r.F_bytes < need) {
+    set_error("internal error: real-space grid not produced before its forward transform");
+    return SDFT_EINVAL_CODE;
+  }
+  if (prec == 0) {
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_field_impl<double, NN, MODE>(c, W, r.F, npairs)));
+  } else {
+    SDFT_SIZE_SWITCH(c.g.n, return (plane_field_impl<float, NN, MODE>(c, W, r.F, npairs)));
+  }
+  return 0;
+}
+
+int rs_to_grid(Ctx& c, int prec, void* W, int npairs) { return plane_field<1>(c, prec, W, npairs); }
+
+int rs_from_grid(Ctx& c, int prec, void* W, int npairs) {
+  return plane_field<2>(c, prec, W, npairs);
+}
+
+// pairs per workgroup of the product / apply kernels: register budget of the
+// partial sums (QC x 4 complex per thread in the product)
+template <typename T>
+constexpr int rs_qc() { return sizeof(T) == 8 ? 4 : 8; }
+
+template <typename T>
+static int project_impl(Ctx& c, int npairs) {
+  RsKB& r = *c.rs;
+  constexpr int QC = rs_qc<T>();
+  int rc = rs_grow(&r.Z, &r.Z_bytes, sizeof(Cx<T>) * (size_t)r.nproj * npairs);
+  if (rc) return rc;
+  const dim3 grid(r.ngroup, (npairs + QC - 1) / QC);
+  const T* gval = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.gval64)
+                                 : reinterpret_cast<const T*>(r.gval32);
+  hipLaunchKernelGGL((k_rs_project<T, QC>), grid, dim3(kRsThreads), 0, c.stream, r.goff, r.gidx,
+                     gval, r.gp0, r.gnch, reinterpret_cast<const Cx<T>*>(r.F), (long long)c.g.N,
+                     npairs, reinterpret_cast<Cx<T>*>(r.Z));
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int apply_impl(Ctx& c, int npairs) {
+  RsKB& r = *c.rs;
+  constexpr int QC = rs_qc<T>();
+  const long long N = c.g.N;
+  const dim3 grid((unsigned)((N + kRsThreads - 1) / kRsThreads), (npairs + QC - 1) / QC);
+  const T* ptw = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.ptw64)
+                                : reinterpret_cast<const T*>(r.ptw32);
+  const T* v = sizeof(T) == 8 ? reinterpret_cast<const T*>(c.vks64)
+                              : reinterpret_cast<const T*>(c.vks32);
+  hipLaunchKernelGGL((k_rs_apply<T, QC>), grid, dim3(kRsThreads), 0, c.stream, r.ptoff, r.ptp,
+                     ptw, v, reinterpret_cast<const Cx<T>*>(r.Z), N, npairs,
+                     reinterpret_cast<Cx<T>*>(r.F));
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int rs_project(Ctx& c, int prec, int npairs) {
+  return prec == 0 ? project_impl<double>(c, npairs) : project_impl<float>(c, npairs);
+}
+
+int rs_apply(Ctx& c, int prec, int npairs) {
+  return prec == 0 ? apply_impl<double>(c, npairs) : apply_impl<float>(c, npairs);
+}
+
+// after rs_project in FP64 on the same pairs
+int rs_energy(Ctx& c, int npairs, int nstate, double* out) {
+  RsKB& r = *c.rs;
+  const long long tot = (long long)r.nproj * npairs;
+  const int nb = (int)std::max<long long>(1, std::min<long long>(256, (tot + kRsThreads - 1) /
+                                                                          kRsThreads));
+  int rc = rs_grow(reinterpret_cast<void**>(&r.epart), &r.epart_bytes, sizeof(double) * nb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rs_energy, dim3(nb), dim3(kRsThreads), 0, c.stream,
+                     reinterpret_cast<const Cx<double>*>(r.Z), r.escale, r.nproj, npairs, nstate,
+                     r.epart);
+  SDFT_HIP_CHECK(hipGetLastError());
+  std::vector<double> h(nb);
+  SDFT_HIP_CHECK(hipMemcpyAsync(h.data(), r.epart, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += h[i];
+  *out = s;
+  return 0;
+}
+
+}  // namespace sdft

@@ -19,7 +19,7 @@ REPO = os.path.dirname(PKG_DIR)
 INCLUDE = os.path.join(REPO, "include")
 
 HIP_SOURCES = ["sdft_kernels.hip", "sdft_gemm.hip", "sdft_cheby.hip", "sdft_api.hip",
-               "sdft_batch.hip", "sdft_ks.hip"]
+               "sdft_batch.hip", "sdft_ks.hip", "sdft_rs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -44,6 +44,7 @@ EXPORTS = [
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
+    "sdft_hip_set_nl_form",
     "sdft_hip_spectral_bound", "sdft_hip_newton_filter", "sdft_hip_tune", "sdft_hip_set_config",
     "sdft_hip_ks_init", "sdft_hip_ks_potential", "sdft_hip_local_forces",
 ]
@@ -102,6 +103,7 @@ def load(path: str | None = None):
         "sdft_hip_batch_config": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
                                           C.c_double], C.c_int),
+        "sdft_hip_set_nl_form": ([vp, C.c_int], C.c_int),
         "sdft_hip_spectral_bound": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                                      _ip, C.c_int], C.c_int),
         "sdft_hip_newton_filter": ([vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
@@ -203,6 +205,10 @@ class Context:
         _check(self.lib.sdft_hip_set_projectors_real(
             self.h, len(e), off.ctypes.data_as(C.POINTER(C.c_longlong)),
             idx.ctypes.data_as(_ip), _ptr(val), _ptr(e), float(rs.vol_elem)))
+
+    def set_nl_form(self, form):
+        """Real-space projectors: 0 per-atom form on the grid, 1 G-space table."""
+        _check(self.lib.sdft_hip_set_nl_form(self.h, int(form)))
 
     def clear_projectors(self):
         _check(self.lib.sdft_hip_set_projectors(self.h, 0, None, None, None))

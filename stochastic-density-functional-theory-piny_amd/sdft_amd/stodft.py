@@ -16,13 +16,41 @@ import numpy as np
 from . import hip as _hip
 
 
-def spectral_bounds(sysm) -> tuple[float, float]:
+def _real_space_nl_bounds(rs, ngrid) -> tuple[float, float]:
+    """Bounds of the real-space KB operator V = vol_elem B E B^T (B: grid x
+    projectors, the sampled beta_p): its eigenvalues are those of
+    vol_elem E^(1/2) B^T B E^(1/2) (|E| and signs for negative energies), and
+    restricting V to the plane-wave set keeps them inside; Gershgorin's
+    row sums of that projector-space matrix bound them."""
+    import scipy.sparse as sp
+    nproj = len(rs.energy)
+    rows = np.repeat(np.arange(nproj), np.diff(rs.offsets))
+    B = sp.csr_matrix((rs.values[:rs.offsets[-1]], (rows, rs.index[:rs.offsets[-1]])),
+                      shape=(nproj, ngrid))
+    gram = (B @ B.T).tocsr()
+    s = np.sqrt(np.abs(rs.energy) * rs.vol_elem)
+    m = sp.diags(s) @ gram @ sp.diags(s)
+    absrow = np.asarray(abs(m).sum(axis=1)).ravel()
+    diag = m.diagonal()
+    off = absrow - np.abs(diag)
+    sign = np.sign(rs.energy)
+    hi = max(0.0, float(np.max(np.where(sign > 0, diag + off, off))))
+    lo = min(0.0, float(np.min(np.where(sign < 0, -(diag + off), -off)))) if np.any(sign < 0) \
+        else 0.0
+    return lo, hi
+
+
+def spectral_bounds(sysm, rs=None) -> tuple[float, float]:
     """Rigorous bounds of the spectrum of H = T + v + V_NL on the coefficient
     set: T in [0, max|G|^2/2], v in [min v, max v], and V_NL = B E B^T lies
     in [lambda_min, lambda_max] of |E|^(1/2) Gram |E|^(1/2) (signed by E),
-    Gram_pq = Q_p(0)Q_q(0) + 2 Re sum_half conj(Q_p) Q_q."""
+    Gram_pq = Q_p(0)Q_q(0) + 2 Re sum_half conj(Q_p) Q_q; with real-space
+    projectors rs (systems.RealSpaceProjectors) instead, the Gershgorin
+    bounds of their grid-space operator."""
     lo_nl = hi_nl = 0.0
-    if sysm.nproj > 0:
+    if rs is not None:
+        lo_nl, hi_nl = _real_space_nl_bounds(rs, sysm.ngrid)
+    elif sysm.nproj > 0:
         q = sysm.proj
         gram = np.outer(q[:, -1].real, q[:, -1].real) + 2.0 * np.real(np.conj(q[:, :-1]) @ q[:, :-1].T)
         e = sysm.proj_energy

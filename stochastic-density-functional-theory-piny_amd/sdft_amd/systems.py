@@ -196,6 +196,20 @@ def _structure_factor_grid(n: tuple, L: float, pos: np.ndarray) -> np.ndarray:
     return np.einsum("ia,ja,ka->ijk", e[0], e[1], e[2], optimize=True).ravel()
 
 
+def _structure_factor_sep(k: np.ndarray, L: float, pos: np.ndarray) -> np.ndarray:
+    """S(G) at the integer triplets k from per-axis phase tables over the
+    triplets' bounding box (one contraction over atoms, then a lookup); equal
+    to _structure_factor up to rounding."""
+    ki = np.asarray(k).astype(np.int64)
+    lo, hi = ki.min(axis=0), ki.max(axis=0)
+    frac = pos / L
+    e = [np.exp(-2j * math.pi * np.outer(np.arange(lo[d], hi[d] + 1), frac[:, d]))
+         for d in range(3)]
+    cube = np.einsum("ia,ja,ka->ijk", e[0], e[1], e[2], optimize=True)
+    j = ki - lo
+    return cube[j[:, 0], j[:, 1], j[:, 2]]
+
+
 def ks_potential(L: float, n: tuple, pos: np.ndarray, z_val: float, rc_loc: float,
                  sigma_rho: float, separable: bool = False) -> np.ndarray:
     """v_KS(r) on the FFT grid, FFTW order, real.
@@ -397,12 +411,14 @@ def vloc_form_factor(sysm: System, dens: DensitySphere) -> np.ndarray:
     return out
 
 
-def vloc_density(sysm: System, dens: DensitySphere) -> np.ndarray:
+def vloc_density(sysm: System, dens: DensitySphere, separable: bool = False) -> np.ndarray:
     """v_ext(G) on the density sphere: the erf-screened Coulomb local
     potential of the atoms, -4 pi Z / (V G^2) exp(-G^2 rc^2 / 4) S(G), zero
-    at G = 0 (the same local potential ks_potential puts into v_KS)."""
+    at G = 0 (the same local potential ks_potential puts into v_KS);
+    separable = True takes S(G) from per-axis tables (large cells)."""
     V = sysm.volume
-    S = _structure_factor(dens.k.astype(np.float64), sysm.L, sysm.pos)
+    S = (_structure_factor_sep(dens.k, sysm.L, sysm.pos) if separable
+         else _structure_factor(dens.k.astype(np.float64), sysm.L, sysm.pos))
     g2 = dens.ak2
     out = np.zeros(dens.ncoef, dtype=np.complex128)
     nz = g2 > 0
@@ -453,6 +469,33 @@ WORKLOADS = {
     "si512": Workload("si512-128^3-B512x128-L2048", 10.26, 4, 128, 512, 128, 2048, 0.01, 2),
     "si1728": Workload("si1728-192^3-B1024x128-L3000", 10.26, 6, 192, 1024, 128, 3000, 0.005, 4),
 }
+
+
+def displaced_positions(pos: np.ndarray, sigma: float, seed: int) -> np.ndarray:
+    """Atoms displaced by independent N(0, sigma^2) offsets per coordinate
+    (numpy PCG64 stream of `seed`), as the Si1728 force workload specifies."""
+    rng = np.random.default_rng(seed)
+    return np.asarray(pos, dtype=np.float64) + rng.normal(0.0, sigma, size=np.shape(pos))
+
+
+# The large workloads apply their KB projectors in real space (the reference's
+# nlpp_real_on path): s + 3 p Gaussian channels per atom sampled within this
+# cutoff (8217 grid points per atom at the 0.3206-bohr spacing of both cells).
+WORKLOAD_RS_CUTOFF = 4.0
+
+
+def workload_system(key: str) -> tuple:
+    """(System, RealSpaceProjectors) of the Si512 / Si1728 workloads: the
+    4x4x4 / 6x6x6 diamond supercell (a = 41.04 / 61.56 bohr), 12 Ry, 128^3 /
+    192^3 grid, Si1728's atoms displaced by N(0, 0.05 bohr) from seed 4; the
+    potential from the per-axis structure-factor tables, and the projectors in
+    real space (the System carries no G-space table)."""
+    wl = WORKLOADS[key]
+    L, p = diamond_supercell(wl.a0, wl.rep)
+    if key == "si1728":
+        p = displaced_positions(p, 0.05, 4)
+    sysm = build_system(wl.name, wl.a0, wl.rep, wl.n, pos=p, with_nl=False, separable_sf=True)
+    return sysm, real_space_projectors(sysm, WORKLOAD_RS_CUTOFF)
 
 
 # ---------------------------------------------------------------------------

@@ -201,7 +201,8 @@ def test_batch_refuses_projector_change_after_create(hip, frags):
 def test_batch_mixed_local_only_and_real_space_fragments(hip, frags):
     """A batch whose fragments mix the three KB forms: G-space table, no
     projectors (local potential only: the contract takes no NL force) and
-    real-space projectors (device-built table); each against the oracle."""
+    real-space projectors (through their device-built table, nl_form 1: the
+    batch refuses the per-atom form); each against the oracle."""
     from sdft_amd import systems
     say("case: batch with local-only and real-space-KB fragments, L=9")
     L = 9
@@ -217,6 +218,11 @@ def test_batch_mixed_local_only_and_real_space_fragments(hip, frags):
             c = hip.Context(sysm, with_nl=False)
             rs[i] = systems.real_space_projectors(sysm, 0.45 * sysm.L)
             c.set_projectors_real(rs[i])
+            b0 = hip.Block(c, chi.shape[0], hip.PREC_FP64)
+            with pytest.raises(hip.SdftError):
+                hip.Batch([b0])
+            b0.close()
+            c.set_nl_form(1)
         else:
             c = hip.Context(sysm)
         b = hip.Block(c, chi.shape[0], hip.PREC_FP64)

@@ -133,3 +133,32 @@ def test_filter_many_steps_large_grids(hip, large):
     say("  %s L=%d: FP64 orbitals %.3g density %.3g, FP32 orbitals %.3g" % (
         s.name, L, rel(acc_g, acc_r), rel(rho_g, rho_r), rel(acc32, acc_r)))
     assert rel(acc32, acc_r) < 1e-4, rel(acc32, acc_r)
+
+
+def test_real_space_kb_large_grids(hip, large):
+    """The per-atom real-space KB at 128^3 and 192^3 (at 192^3 in FP64 the
+    plane stage runs its split form on both sides of the real-space grid):
+    the projectors of 8 atoms with a 2-bohr cutoff (about 27k grid points per
+    atom at 192^3), H.psi against the oracle's literal gather / dot /
+    scatter (FP64 1e-12, FP32 2e-5 relative) and the NL energy against
+    nlppKBRealEnergy restated (1e-12)."""
+    import dataclasses
+
+    from sdft_amd import systems
+    s, _ = large
+    say("case: per-atom real-space KB %s (8 atoms)" % s.name)
+    rs = systems.real_space_projectors(s, 2.0)
+    rs = dataclasses.replace(rs, offsets=rs.offsets[:33], energy=rs.energy[:32],
+                             atom=rs.atom[:32])
+    ctx = hip.Context(s, with_nl=False)
+    ctx.set_projectors_real(rs)
+    C = systems.noise_orbitals(s, 3, seed=61)
+    ref = O.hpsi_raw(C, s, nl_real=rs)
+    e64 = rel(ctx.hpsi_raw(C), ref)
+    e32 = rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref)
+    _, enl = ctx.energies(C, 2.0, 3.0)
+    enl_ref = O.energy_nl_real(C, s, rs, 2.0, 3.0)
+    say("  %s: hpsi fp64 %.3g fp32 %.3g, enl %.12g / %.12g" % (s.name, e64, e32, enl, enl_ref))
+    assert e64 < 1e-12 and e32 < 2e-5, (e64, e32)
+    assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref)
+    ctx.close()

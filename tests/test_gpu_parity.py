@@ -232,13 +232,15 @@ def test_device_noise_matches_host_stream(hip, name, n, g2, nst, seed, first):
 
 @pytest.mark.parametrize("nproj", [32, 31])
 def test_hpsi_real_space_kb(hip, si8_system, nproj):
-    """sdft_hip_set_projectors_real (device-built table from the grid-sampled
-    projectors) against the oracle's literal real-space KB
-    (nlppKBRealEnergyFilter restated): FP64 1e-12, FP32 2e-5 relative."""
+    """sdft_hip_set_projectors_real against the oracle's literal real-space
+    KB (nlppKBRealEnergyFilter restated), in both forms: the per-atom form on
+    the real-space grid (nl_form 0, the default) and the device-built G-space
+    table (nl_form 1).  H.psi FP64 1e-12, FP32 2e-5 relative; a short filter
+    1e-10; the NL energy against nlppKBRealEnergy restated, 1e-12.  nproj = 31
+    leaves the last atom with three channels (a partial projector group)."""
     import dataclasses
 
     from sdft_amd import systems
-    say("case: hpsi real-space KB si8 nproj=%d" % nproj)
     rs = systems.real_space_projectors(si8_system, 4.0)
     rs = dataclasses.replace(rs, offsets=rs.offsets[:nproj + 1], energy=rs.energy[:nproj],
                              atom=rs.atom[:nproj])
@@ -246,16 +248,25 @@ def test_hpsi_real_space_kb(hip, si8_system, nproj):
     ctx.set_projectors_real(rs)
     C = systems.noise_orbitals(si8_system, 5, seed=21)
     ref = O.hpsi_raw(C, si8_system, nl_real=rs)
-    assert rel(ctx.hpsi_raw(C), ref) < 1e-12
-    assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
-    # a short filter through the same table
     emin, emax = bounds(si8_system)
     L = 12
     coeffs = O.cheby_coeffs(L, emin - 1.0, emax + 1.0, 0.2, 100.0, 2)
     scale, emean = 2.0 / (emax - emin + 2.0), 0.5 * (emin + emax)
-    acc_g, _ = ctx.cheby_filter(C, coeffs, scale, emean)
     acc, _ = O.filter_cheby(C, coeffs, si8_system, scale, emean, nl_real=rs)
-    assert rel(acc_g, acc) < 1e-10
+    enl_ref = O.energy_nl_real(C, si8_system, rs, 2.0, 5.0)
+    for form in (0, 1):
+        say("case: hpsi real-space KB si8 nproj=%d nl_form=%d" % (nproj, form))
+        ctx.set_nl_form(form)
+        e64 = rel(ctx.hpsi_raw(C), ref)
+        e32 = rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref)
+        acc_g, _ = ctx.cheby_filter(C, coeffs, scale, emean)
+        ef = rel(acc_g, acc)
+        _, enl = ctx.energies(C, 2.0, 5.0)
+        say("  hpsi fp64 %.3g fp32 %.3g filter %.3g enl %.12g / %.12g" % (e64, e32, ef, enl,
+                                                                         enl_ref))
+        assert e64 < 1e-12 and e32 < 2e-5 and ef < 1e-10, (form, e64, e32, ef)
+        assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref), (form, enl, enl_ref)
+    ctx.close()
 
 
 def test_density_generic_layout_and_accumulate(hip, si8_ctx, si8_system):

@@ -420,6 +420,52 @@ def test_real_space_kb_is_hermitian(si8_system):
     assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1.0)
 
 
+def test_real_space_kb_energy_is_expectation_of_operator(si8_system):
+    """energy_nl_real (nlppKBRealEnergy restated) equals occ/Nsto sum_s
+    <psi_s|V_NL|psi_s>, with V_NL psi taken from the real-space operator
+    (nl_apply_real through hpsi_raw: raw force with minus without the KB
+    term).  Odd orbital count, so the zero-padded pair is covered."""
+    from sdft_amd import systems
+    rs = systems.real_space_projectors(si8_system, 3.5)
+    C = systems.noise_orbitals(si8_system, 3, seed=17)
+    f_nl = O.hpsi_raw(C, si8_system, nl_real=rs) - O.hpsi_raw(C, si8_system, with_nl=False)
+    h_nl = -0.25 * f_nl
+    h_nl[:, -1] *= 2.0
+    expect = 2.0 * sum(gdot(C[i], h_nl[i]) for i in range(3)) / 5.0
+    e = O.energy_nl_real(C, si8_system, rs, 2.0, 5.0)
+    assert expect > 0
+    assert abs(e - expect) < 1e-11 * expect, (e, expect)
+
+
+def test_real_space_spectral_bounds_contain_dense_spectrum(tiny_system):
+    """stodft.spectral_bounds with real-space projectors (Gershgorin bound of
+    the grid-space KB operator) contains the eigenvalues of the dense H built
+    from the oracle's real-space form, and is not loose by more than the
+    operator's own size."""
+    from sdft_amd import stodft, systems
+    s = tiny_system
+    rs = systems.real_space_projectors(s, 3.5)
+    nc = s.ncoef
+    dim = O.gamma_real_basis_dim(nc)
+    C = O.from_real_vec(np.eye(dim), nc)
+    f = O.hpsi_raw(C, s, nl_real=rs)
+    h = -0.25 * f
+    h[:, -1] *= 2.0
+    H = O.to_real_vec(h).T
+    assert np.allclose(H, H.T, atol=1e-10)
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+    lo, hi = stodft.spectral_bounds(s, rs)
+    assert lo <= ev[0] + 1e-12 and ev[-1] <= hi + 1e-12, (lo, ev[0], ev[-1], hi)
+    lo0, hi0 = stodft.spectral_bounds(dataclasses_replace_noproj(s))
+    assert hi > hi0 and hi - ev[-1] < 4.0 * (hi - hi0) + 1.0
+
+
+def dataclasses_replace_noproj(s):
+    import dataclasses
+    return dataclasses.replace(s, proj=np.zeros((0, s.ncoef), dtype=np.complex128),
+                               proj_energy=np.zeros(0))
+
+
 def test_spectral_bounds_match_dense_eigenvalues(tiny_system):
     """gen_energy_bound (genEnergyMax / genEnergyMin restated) converges to
     the extreme eigenvalues of the dense H built from the oracle."""
