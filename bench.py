@@ -126,6 +126,10 @@ def main():
                     help="timed Chebyshev steps per CPU-baseline run (3 runs, median)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 343-fragment batch measurement")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the Si512 / Si1728 block measurements")
+    ap.add_argument("--workloads-only", action="store_true",
+                    help="only the Si512 / Si1728 block measurements (JSON line of its own)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 343-fragment batch measurement (JSON line of its own)")
     ap.add_argument("--batch-config", default=None,
@@ -151,6 +155,9 @@ def main():
 
     if args.batch_only:
         print(json.dumps(batch_bench(args, 0.2)), flush=True)
+        return
+    if args.workloads_only:
+        print(json.dumps({k: workload_bench(k) for k in ("si512", "si1728")}), flush=True)
         return
     wl = systems.WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -318,6 +325,9 @@ def main():
         log("cpu baseline: %.1f H.psi/s on %d threads (%s)" % (rate, workers, host["model"]))
 
     batch = None if args.no_batch else batch_bench(args, mu)
+    workloads = None
+    if not args.no_workloads:
+        workloads = {k: workload_bench(k) for k in ("si512", "si1728")}
 
     r32, r64 = results["fp32"], results["fp64"]
     survey_frac32 = survey_bytes_per_hpsi(sysm, 8, 4) * r32["hpsi_per_s"] / (HBM_PEAK_GBS * 1e9)
@@ -354,8 +364,92 @@ def main():
         "cpu_baseline": cpu,
         "fragment_batch": batch,
         "ks_potential": ks,
+        "workloads": workloads,
     }
     print(json.dumps(line), flush=True)
+
+
+def workload_bench(key, nsteps=24, block=128):
+    """One 128-orbital block of the Si512 / Si1728 workload (systems.
+    workload_system: real-space KB projectors in the library's per-atom
+    form) through `nsteps` Chebyshev steps of its degree-L expansion: the
+    per-step cost does not depend on L, so the rate of a short pass is the
+    rate of the full one.  FP32 and FP64 forms; per-kernel event times of one
+    profiled pass; the NL kernels (real-space product and v.psi + scatter)
+    against the FFT kernels (expand, the two plane stages, contract); and the
+    HBM rate of each kernel from its algorithmic bytes:
+      plane_bc_to_r: W read + the pair grids written (n ncol + n^3) w_c per pair
+      rs_project:    the grids read at the projectors' points (each point once
+                     per atom that covers it) + the beta values and indices
+      rs_apply_vpsi: the grids read and written + v + the inverse lists
+      plane_r_to_bc: the grids read + W written."""
+    from sdft_amd import hip, stodft, systems
+    wl = systems.WORKLOADS[key]
+    t0 = time.perf_counter()
+    sysm, rs = systems.workload_system(key)
+    ctx = hip.Context(sysm, with_nl=False)
+    ctx.set_projectors_real(rs)
+    emin, emax = stodft.spectral_bounds(sysm, rs)
+    log("%s setup: ncoef %d, P %d, %d KB grid entries (%.1fs)" % (
+        key, sysm.ncoef, len(rs.energy), rs.offsets[-1], time.perf_counter() - t0))
+    L = wl.degree
+    beta = 1.0 / wl.kt
+    coeffs = ctx.cheby_coeffs(L, emin, emax, 0.3, beta, 2)[:nsteps + 1]
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    N = sysm.ngrid
+    npairs = block // 2
+    ncol = ctx_ncol(sysm)
+    nent = int(rs.offsets[-1])
+    # distinct (atom point list, channel group) entries: 4 channels share a list
+    nent_g = nent // 4
+    out = {"workload": wl.name, "grid": wl.n, "ncoef": sysm.ncoef, "natm": sysm.natm,
+           "nproj": int(len(rs.energy)), "kb_grid_entries": nent, "block": block,
+           "degree": L, "timed_steps": nsteps, "nl_form": "per-atom real space"}
+    for prec, tag in ((hip.PREC_FP32, "fp32"), (hip.PREC_FP64, "fp64")):
+        wc, wv = (8, 4) if prec == hip.PREC_FP32 else (16, 8)
+        blk = hip.Block(ctx, block, prec)
+        blk.noise(wl.seed, 0)
+        blk.filter(coeffs, scale, emean)  # warm
+        t1 = time.perf_counter()
+        blk.filter(coeffs, scale, emean)
+        dt = time.perf_counter() - t1
+        ctx.set_profiling(True)
+        blk.filter(coeffs, scale, emean)
+        ctx.set_profiling(False)
+        kt = ctx.kernel_times()
+        blk.close()
+        per = {k: v[0] / v[1] for k, v in kt.items() if v[1] > 0}
+        steps = nsteps  # Chebyshev steps applied (L' - 1 with L' = nsteps + 1)
+        nl_ms = sum(kt[k][0] for k in ("rs_project", "rs_apply_vpsi") if k in kt) / steps
+        fft_ms = sum(kt[k][0] for k in ("expand_a", "plane_bc_to_r", "plane_r_to_bc",
+                                        "contract_a") if k in kt) / steps
+        grid_b = float(N) * wc * npairs
+        wbytes = float(wl.n) * ncol * wc * npairs
+        qc = 8 if prec == hip.PREC_FP32 else 4  # pairs per workgroup (product, apply)
+        nch = -(-npairs // qc)
+        wr = wc // 2  # one real value
+        model = {"plane_bc_to_r": wbytes + grid_b,
+                 "rs_project": float(nent_g) * wc * npairs + float(nent_g) * (4 + 4 * wr) * nch,
+                 "rs_apply_vpsi": 2.0 * grid_b + (float(N) * (wv + 8) + float(nent) * (4 + wr))
+                 * nch,
+                 "plane_r_to_bc": grid_b + wbytes}
+        kern = {}
+        for k, ms in per.items():
+            d = {"ms": round(ms, 4)}
+            if k in model:
+                gbs = model[k] / (ms * 1e-3) / 1e9
+                d.update({"algorithmic_bytes": model[k], "gbs": round(gbs, 1),
+                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)})
+            kern[k] = d
+        out[tag] = {"hpsi_per_s": round(block * nsteps / dt, 1),
+                    "ms_per_step": round(1e3 * dt / nsteps, 4),
+                    "nl_ms_per_step": round(nl_ms, 4), "fft_ms_per_step": round(fft_ms, 4),
+                    "kernels": kern}
+        log("%s %s: %.4g H.psi/s, %.3f ms/step (NL %.3f, FFT %.3f ms/step)" % (
+            key, tag, block * nsteps / dt, 1e3 * dt / nsteps, nl_ms, fft_ms))
+    out["fp32_over_fp64"] = round(out["fp32"]["hpsi_per_s"] / out["fp64"]["hpsi_per_s"], 3)
+    ctx.close()
+    return out
 
 
 def ks_bench(ctx, sysm, rho, with_cpu, reps=20):

@@ -540,15 +540,17 @@ __global__ __launch_bounds__(kThreads) void k_density(const Cx<T>* __restrict__ 
 }
 
 // Split form of k_density for planes whose M x (n+1) buffer exceeds LDS: per
-// pair and column half, the inverse c-transform of the M rows (through Q,
-// recomputed per half) keeps the half's columns, and its b-lines accumulate
-// |psi1|^2 + |psi2|^2 straight from the last pass's outputs.
-template <typename T, int N, bool RE>
+// pair and column part (S parts of n / S columns), the inverse c-transform
+// of the M rows (through Q, recomputed per part) keeps the part's columns,
+// and its b-lines accumulate |psi1|^2 + |psi2|^2 straight from the last
+// pass's outputs.
+template <typename T, int N, bool RE, int S = 2>
 __global__ __launch_bounds__(kThreads) void k_density_split(
     const Cx<T>* __restrict__ W, int ncolp, const int* __restrict__ rowcol, int M,
     const Cx<T>* __restrict__ twg, double* __restrict__ rho_part, int npairs, int ngroups) {
   constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
-  constexpr int H = SplitCfg<T, N>::H, LDH = SplitCfg<T, N>::LDH, QL = SplitCfg<T, N>::QL;
+  constexpr int H = SplitCfg<T, N, S>::H, LDH = SplitCfg<T, N, S>::LDH;
+  constexpr int QL = SplitCfg<T, N, S>::QL;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(kThreads) void k_density_split(
   for (int q = g; q < npairs; q += ngroups) {
     const bool first = (q == g);
     const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < S; ++h) {
       const int c_lo = h * H;
       for (int r0 = 0; r0 < M; r0 += ML) {
         const int nl = M - r0 < ML ? M - r0 : ML;
@@ -914,18 +916,28 @@ static int density_impl(Ctx& c, const void* W, int npairs, double* rho_part, int
   const Geometry& g = c.g;
   const size_t lds = plane_lds_bytes<T, N>(g.M);
   if (lds > 160 * 1024) {
-    const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
-    if (lds2 > 160 * 1024) {
+    const size_t lds2 = plane_split_lds_bytes<T, N, 2>(g.M);
+    const size_t lds4 = plane_split_lds_bytes<T, N, 4>(g.M);
+    if (lds2 <= 160 * 1024) {
+      SDFT_HIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&k_density_split<T, N, RE, 2>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+      hipLaunchKernelGGL((k_density_split<T, N, RE, 2>), dim3(N, ngroups), dim3(kThreads), lds2,
+                         c.stream,
+                         reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M,
+                         reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
+    } else if (N % 4 == 0 && lds4 <= 160 * 1024) {
+      SDFT_HIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&k_density_split<T, N, RE, 4>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4));
+      hipLaunchKernelGGL((k_density_split<T, N, RE, 4>), dim3(N, ngroups), dim3(kThreads), lds4,
+                         c.stream,
+                         reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M,
+                         reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
+    } else {
       set_error("plane buffer exceeds LDS for this grid in this precision");
       return SDFT_EINVAL_CODE;
     }
-    SDFT_HIP_CHECK(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_density_split<T, N, RE>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-    hipLaunchKernelGGL((k_density_split<T, N, RE>), dim3(N, ngroups), dim3(kThreads), lds2,
-                       c.stream,
-                       reinterpret_cast<const Cx<T>*>(W), g.ncolp, c.t.rowcol, g.M,
-                       reinterpret_cast<const Cx<T>*>(tw), rho_part, npairs, ngroups);
     SDFT_HIP_CHECK(hipGetLastError());
     return 0;
   }

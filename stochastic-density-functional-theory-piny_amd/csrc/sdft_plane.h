@@ -196,15 +196,17 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
 // c-transform of the rows with the other half's entries zero is stored
 // (h = 0) or added (h = 1) to Wout: the forward transform is linear in its
 // input, and W (the input of both halves) is left intact.
-template <typename T, int N>
+// S column parts of width H = n / S (the plane kernels use halves; the
+// density kernel also quarters, for the density sphere's rows at 192^3)
+template <typename T, int N, int S = 2>
 struct SplitCfg {
-  static constexpr int H = N / 2, LDH = H + 1;
+  static constexpr int H = N / S, LDH = H + 1;
   static constexpr int QL = Cfg<N>::ML > PlaneCfg<T, N>::CB ? Cfg<N>::ML : PlaneCfg<T, N>::CB;
 };
 
-template <typename T, int N>
+template <typename T, int N, int S = 2>
 size_t plane_split_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N>::LDH + (size_t)SplitCfg<T, N>::QL *
+  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N, S>::LDH + (size_t)SplitCfg<T, N, S>::QL *
                           Cfg<N>::LD + N);
 }
 

@@ -570,3 +570,78 @@ def test_filter_fused_and_separate_expand(hip, si8_system, si8_ctx, prec, tol):
     si8_ctx.set_config(0, 512, 1)
     if prec == 0:
         assert rel(out[0], out[1]) < 1e-13
+
+
+def test_config1_si8_full_workload(hip, si8_ctx, si8_system):
+    """The CPU-runnable configuration in full: Si8 (32^3, ncoef 370, 8 atoms,
+    32 G-space projectors), 64 Rademacher orbitals from seed 0, Chebyshev
+    degree 256, kT = 0.01 Ha -- the moments pass, mu for N_e = 32 (relative
+    tolerance 1e-11, calc-chempot-chebyshev.c:78), the sqrt(f)
+    coefficients, the filter and the density, each on the device against the
+    oracle's chain: moments, filtered orbitals, T_{L-1} and density 1e-10
+    relative, mu 1e-9; the FP32 form's filtered orbitals 1e-4."""
+    from sdft_amd import systems
+    say("case: config 1 in full (Si8, 64 orbitals, L=256; oracle ~15 s)")
+    s = si8_system
+    B, L, beta = 64, 256, 100.0
+    emin, emax = bounds(s)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, B, seed=0)
+    ne = s.nelec
+    mom_r = O.cheby_moments(chi, s, L, scale, emean)
+    mu_r, n_r = O.chem_pot_cheby(mom_r, B, L, emin, emax, beta, ne, 0.3, 0.4)
+    co_r = O.cheby_coeffs(L, emin, emax, mu_r, beta, 2)
+    acc_r, tl_r = O.filter_cheby(chi, co_r, s, scale, emean)
+    rho_r = O.density_real(acc_r, s, 2.0 / B)
+    mom_g = si8_ctx.cheby_moments(chi, L, scale, emean)
+    mu_g, n_g = si8_ctx.chem_pot(L, emin, emax, beta, mom_g, B, ne, 0.3, 0.4)
+    co_g = si8_ctx.cheby_coeffs(L, emin, emax, mu_g, beta, 2)
+    acc_g, tl_g = si8_ctx.cheby_filter(chi, co_g, scale, emean)
+    rho_g = si8_ctx.density(acc_g, 2.0 / B)
+    acc32, _ = si8_ctx.cheby_filter(chi, co_g, scale, emean, prec=hip.PREC_FP32)
+    say("  config 1: moments %.3g, mu %.12f / %.12f, filter %.3g, T_L-1 %.3g, density %.3g, "
+        "N %.10f, FP32 filter %.3g" % (rel(mom_g, mom_r), mu_g, mu_r, rel(acc_g, acc_r),
+                                       rel(tl_g, tl_r), rel(rho_g, rho_r),
+                                       rho_g.sum() / s.ngrid, rel(acc32, acc_r)))
+    assert rel(mom_g, mom_r) < 1e-10
+    assert abs(mu_g - mu_r) < 1e-9 and abs(n_g - ne) <= 1e-11 * ne
+    assert rel(co_g, co_r) < 1e-12
+    assert rel(acc_g, acc_r) < 1e-10 and rel(tl_g, tl_r) < 1e-10
+    assert rel(rho_g, rho_r) < 1e-10
+    assert rel(acc32, acc_r) < 1e-4
+
+
+def test_hpsi_real_space_kb_dense_overlap(hip, si8_system):
+    """Per-atom real-space KB with 300 synthetic projectors crowded into the
+    first 2048 grid points (8 point blocks of the apply kernel, each touched
+    by more projectors than it stages in LDS, so the entries past the staged
+    set read Z directly), distinct point lists (no channel groups), random
+    values and energies of both signs: H.psi against the oracle, FP64 1e-12,
+    FP32 2e-5 relative, and the NL energy 1e-12."""
+    from sdft_amd import systems
+    say("case: real-space KB, 300 crowded synthetic projectors")
+    s = si8_system
+    rng = np.random.default_rng(5)
+    nproj, npts = 300, 300
+    offs, idx, vals = [0], [], []
+    for _ in range(nproj):
+        ix = np.sort(rng.choice(2048, npts, replace=False)).astype(np.int32)
+        idx.append(ix)
+        vals.append(0.05 * rng.standard_normal(npts))
+        offs.append(offs[-1] + npts)
+    energy = rng.uniform(0.5, 2.0, nproj) * np.where(rng.random(nproj) < 0.2, -1.0, 1.0)
+    rs = systems.RealSpaceProjectors(np.array(offs, dtype=np.int64), np.concatenate(idx),
+                                     np.concatenate(vals), energy, s.volume / s.ngrid,
+                                     np.zeros(nproj, dtype=np.int64))
+    ctx = hip.Context(s, with_nl=False)
+    ctx.set_projectors_real(rs)
+    C = systems.noise_orbitals(s, 7, seed=29)
+    ref = O.hpsi_raw(C, s, nl_real=rs)
+    e64 = rel(ctx.hpsi_raw(C), ref)
+    e32 = rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref)
+    _, enl = ctx.energies(C, 2.0, 7.0)
+    enl_ref = O.energy_nl_real(C, s, rs, 2.0, 7.0)
+    say("  crowded: hpsi fp64 %.3g fp32 %.3g, enl %.12g / %.12g" % (e64, e32, enl, enl_ref))
+    assert e64 < 1e-12 and e32 < 2e-5, (e64, e32)
+    assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref)
+    ctx.close()

@@ -1,0 +1,205 @@
+"""GPU parity on the large workloads themselves, not stand-ins of their shape.
+
+Si512: 4x4x4 diamond supercell (a = 41.04 bohr, 512 atoms), 12 Ry, 128^3 grid,
+ncoef 24245.  Si1728: 6x6x6 (a = 61.56 bohr, 1728 atoms displaced by
+N(0, 0.05 bohr) from seed 4), 192^3, ncoef 81662.  Their KB projectors are
+applied in real space (s + 3 p Gaussian channels per atom within 4 bohr:
+8217 grid points per atom, P = 2048 / 6912) in the library's per-atom form.
+
+Against the oracle on a few orbitals: H.psi (FP64 1e-12, FP32 2e-5 relative),
+a short filter (1e-10), the moments pass (1e-10) and the NL energy (1e-12).
+Si512's chemical potential comes from device moments of a 128-orbital block
+at the workload's full degree L = 2048 and must reach N(mu) = 2048 within
+1e-6 electrons (calc-chempot-chebyshev.c:212-251 with an absolute
+tolerance).  Si1728's Hellmann-Feynman forces (calc-nuclei-force.c:254,
+302-489): the NL force through the G-space table of a subset of atoms and
+the local force of a subset of atoms from the device density, 1e-10.  On
+full 128-orbital blocks (the size the workloads filter at once) the
+size-independent properties: Hermiticity of the device H (Gram matrix
+symmetric to 1e-12) and sum(rho)/N = occ/Nsto sum_s <psi_s|psi_s> (1e-12).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from sdft_amd import hip as H
+    H.load()
+    O.set_fft_workers(8)
+    return H
+
+
+_CACHE = {}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _close_contexts():
+    yield
+    for v in _CACHE.values():
+        v[3].close()
+    _CACHE.clear()
+
+
+def get_workload(hip, key):
+    """(key, System, RealSpaceProjectors, context with the per-atom
+    real-space KB, emin, emax), built once per module."""
+    if key not in _CACHE:
+        from sdft_amd import stodft, systems
+        say("setup: %s system + real-space projectors" % key)
+        s, rs = systems.workload_system(key)
+        ctx = hip.Context(s, with_nl=False)
+        ctx.set_projectors_real(rs)
+        emin, emax = stodft.spectral_bounds(s, rs)
+        say("  %s: ncoef %d, P %d, entries %d, bounds [%.4f, %.4f]" % (
+            key, s.ncoef, len(rs.energy), rs.offsets[-1], emin, emax))
+        _CACHE[key] = (key, s, rs, ctx, emin, emax)
+    return _CACHE[key]
+
+
+@pytest.fixture(params=["si512", "si1728"])
+def workload(request, hip):
+    return get_workload(hip, request.param)
+
+
+def test_workload_hpsi_and_energy(hip, workload):
+    from sdft_amd import systems
+    key, s, rs, ctx, _, _ = workload
+    say("case: %s H.psi (4 orbitals) + NL energy" % key)
+    C = systems.noise_orbitals(s, 4, seed=71)
+    ref = O.hpsi_raw(C, s, nl_real=rs)
+    e64 = rel(ctx.hpsi_raw(C), ref)
+    e32 = rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref)
+    _, enl = ctx.energies(C, 2.0, 4.0)
+    enl_ref = O.energy_nl_real(C, s, rs, 2.0, 4.0)
+    say("  %s: hpsi fp64 %.3g fp32 %.3g, enl %.12g / %.12g" % (key, e64, e32, enl, enl_ref))
+    assert e64 < 1e-12 and e32 < 2e-5, (e64, e32)
+    assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref)
+
+
+def test_workload_filter_and_moments(hip, workload):
+    """A short filter (L = 8) and moments pass (L = 8) of one orbital pair
+    against the oracle, FP64 1e-10 relative; the filter in FP32 1e-4."""
+    from sdft_amd import systems
+    key, s, rs, ctx, emin, emax = workload
+    say("case: %s filter L=8 + moments (oracle ~%d s)" % (key, 10 if key == "si512" else 40))
+    L = 8
+    beta = 100.0 if key == "si512" else 200.0
+    coeffs = O.cheby_coeffs(L, emin, emax, 0.25, beta, 2)
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, 2, seed=73)
+    acc_r, tl_r, mom_r = O.filter_cheby(chi, coeffs, s, scale, emean, moments=True, nl_real=rs)
+    acc_g, tl_g = ctx.cheby_filter(chi, coeffs, scale, emean)
+    acc32, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=hip.PREC_FP32)
+    mom_g = ctx.cheby_moments(chi, L, scale, emean)
+    mom_o = O.cheby_moments(chi, s, L, scale, emean, nl_real=rs)
+    say("  %s: filter fp64 %.3g (T_L-1 %.3g) fp32 %.3g, moments %.3g" % (
+        key, rel(acc_g, acc_r), rel(tl_g, tl_r), rel(acc32, acc_r), rel(mom_g, mom_o)))
+    assert rel(acc_g, acc_r) < 1e-10 and rel(tl_g, tl_r) < 1e-10
+    assert rel(acc32, acc_r) < 1e-4
+    assert rel(mom_g, mom_o) < 1e-10
+
+
+def test_workload_block_properties(hip, workload):
+    """A 128-orbital block (the workloads' block size): the Gram matrix
+    <c_i|H c_j> of the device H is symmetric to 1e-12 of its largest entry
+    (FP64), and the device density of the block integrates to
+    occ/Nsto sum_s <c_s|c_s> (Parseval) within 1e-12."""
+    from sdft_amd import systems
+    key, s, rs, ctx, _, _ = workload
+    say("case: %s 128-orbital block: Hermiticity + density sum" % key)
+    B = 128
+    C = ctx.noise(B, seed=2 if key == "si512" else 4)
+    f = ctx.hpsi_raw(C)
+    h = -0.25 * f
+    h[:, -1] *= 2.0
+    x = O.to_real_vec(C)
+    y = O.to_real_vec(h)
+    G = x @ y.T
+    asym = float(np.max(np.abs(G - G.T)) / np.max(np.abs(G)))
+    rho = ctx.density(C, 2.0 / B)
+    nsum = rho.sum() / s.ngrid
+    ref = 2.0 / B * float(O.gamma_dot(C, C).sum())
+    say("  %s: Gram asymmetry %.3g, sum(rho)/N %.12f vs %.12f" % (key, asym, nsum, ref))
+    assert asym < 1e-12, asym
+    assert abs(nsum - ref) < 1e-12 * ref
+    # the device noise is the workload's seeded Rademacher stream
+    assert rel(C[:2], systems.noise_orbitals(s, 2, seed=2 if key == "si512" else 4)) < 1e-12
+
+
+def test_si512_mu_full_degree(hip):
+    """Si512: moments of a 128-orbital block at the workload's degree
+    L = 2048 (kT = 0.01 Ha) on the device, then the chemical potential for
+    N_e = 2048 with an absolute tolerance of 1e-6 electrons; the search
+    agrees with the oracle's restatement run on the same moments."""
+    key, s, rs, ctx, emin, emax = get_workload(hip, "si512")
+    from sdft_amd import systems
+    say("case: si512 mu search, moments of 128 orbitals at L=2048")
+    L, beta, B = 2048, 100.0, 128
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = systems.noise_orbitals(s, B, seed=2)
+    mom = ctx.cheby_moments(chi, L, scale, emean)
+    assert abs(mom[0] - float(O.gamma_dot(chi, chi).sum())) < 1e-10 * abs(mom[0])
+    ne = s.nelec
+    mu, nmu = ctx.chem_pot(L, emin, emax, beta, mom, B, ne, 0.3, 0.4, tol=1e-6)
+    mu_r, n_r = O.chem_pot_cheby(mom, B, L, emin, emax, beta, ne, 0.3, 0.4, tol_abs=1e-6)
+    say("  si512: mu %.12f N(mu) %.9f (oracle mu %.12f N %.9f)" % (mu, nmu, mu_r, n_r))
+    assert ne == 2048.0
+    assert abs(nmu - ne) <= 1e-6
+    assert abs(O.num_elec_cheby(mu, mom / B, L, emin, emax, beta) - ne) <= 1e-6
+    assert abs(mu - mu_r) < 1e-9
+
+
+def test_si1728_forces(hip):
+    """Si1728 with displaced atoms: Hellmann-Feynman forces against the
+    oracle.  NL: the G-space table of 8 atoms' projectors on 4 orbitals
+    (sdft_hip_nl_forces vs the oracle's dE_NL/dR).  Local: the device density
+    of the same 4 orbitals turned into v_KS on the density sphere, then the
+    local-pseudopotential force on 16 atoms (sdft_hip_local_forces vs the
+    oracle on the same density).  Both 1e-10 relative to the largest
+    component."""
+    import dataclasses
+    key, s, rs, ctx, emin, emax = get_workload(hip, "si1728")
+    from sdft_amd import systems
+    say("case: si1728 NL forces (8 atoms) + local forces (16 atoms)")
+    sub = 8
+    proj, pe, pa, pl, pm = systems.kb_table(s.L, s.pos[:sub], s.k, s.rc_nl, s.h_kb)
+    s_nl = dataclasses.replace(s, proj=proj, proj_energy=pe, proj_atom=pa, proj_l=pl,
+                               proj_m=pm)
+    C = systems.noise_orbitals(s, 4, seed=79)
+    cnl = hip.Context(s_nl)
+    F = cnl.nl_forces(C, 2.0, 4.0)
+    F_ref = O.nl_forces(C, s_nl, 2.0, 4.0)
+    say("  NL forces: max |F| %.4g, rel %.3g" % (np.abs(F_ref).max(), rel(F, F_ref)))
+    assert np.abs(F_ref[:sub]).max() > 0
+    assert rel(F, F_ref) < 1e-10
+    cnl.close()
+    dens = systems.density_sphere(s)
+    sub2 = 16
+    s_loc = dataclasses.replace(s, pos=s.pos[:sub2])
+    vext = systems.vloc_density(s, dens, separable=True)
+    rho = ctx.density(C, 2.0 / 4.0)
+    _, en = O.ks_potential(rho, vext, dens, s.volume, 0)
+    ff = systems.vloc_form_factor(s, dens)
+    types = np.zeros(sub2, dtype=np.int32)
+    ref = O.local_forces(en["Q"], dens, s_loc.pos, ff, types)
+    ctx.ks_init(dens)
+    ctx.ks_potential(rho, vext, s.volume, xc=0)
+    Fl = ctx.local_forces(s_loc.pos, types, ff, dens.gvec)
+    say("  local forces: max |F| %.4g, rel %.3g" % (np.abs(ref).max(), rel(Fl, ref)))
+    assert rel(Fl, ref) < 1e-10
