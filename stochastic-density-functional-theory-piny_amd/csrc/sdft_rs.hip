@@ -31,6 +31,13 @@
 
 namespace sdft {
 
+constexpr int kRsThreads = 256;
+// the apply kernel's point blocks (one point per thread) and the largest
+// projector set it stages per block; pairs staged per pass
+constexpr int kRsApplyPoints = kRsThreads;
+constexpr int kRsMaxSlots = 256;
+constexpr int kRsStagePairs = 16;
+
 struct RsKB {
   int nproj = 0;
   int ngroup = 0;
@@ -45,9 +52,16 @@ struct RsKB {
   int* gnch = nullptr;        // [ngroup] channels
   // inverse lists: grid point -> (projector, E_p vol_elem beta_p(r))
   long long* ptoff = nullptr; // [N + 1]
-  int* ptp = nullptr;         // [nent]
+  int* ptslot = nullptr;      // [nent] slot of the entry's projector in its point block's
+                              // staged set, or -1 - p when the set is full
   double* ptw64 = nullptr;    // [nent]
   float* ptw32 = nullptr;
+  // per block of kRsApplyPoints consecutive grid points: the projectors its
+  // points touch (staged in LDS by the apply kernel)
+  int* bpoff = nullptr;       // [nblock + 1]
+  int* bpp = nullptr;         // projector ids
+  int nblock = 0;
+  int max_slots = 0;          // largest staged set
   double* escale = nullptr;   // [nproj] E_p vol_elem / N (energies)
   // per-call buffers
   void* F = nullptr;          // pair grids, complex in the form's type
@@ -99,8 +113,9 @@ static int rs_grow(void** p, size_t* have, size_t need) {
 }
 
 static void rs_release(RsKB* r) {
-  void* ptrs[] = {r->goff,  r->gidx,  r->gval64, r->gval32, r->gp0, r->gnch, r->ptoff,
-                  r->ptp,   r->ptw64, r->ptw32,  r->escale, r->F,   r->Z,    r->epart};
+  void* ptrs[] = {r->goff,   r->gidx,  r->gval64, r->gval32, r->gp0, r->gnch,
+                  r->ptoff,  r->ptslot, r->ptw64, r->ptw32, r->bpoff, r->bpp,
+                  r->escale, r->F,     r->Z,     r->epart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete r;
@@ -212,6 +227,35 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
       ptw[(size_t)d] = s * val[e];
     }
   }
+  // per point block: the distinct projectors (ascending), each entry's slot
+  const int nblock = (int)((N + kRsApplyPoints - 1) / kRsApplyPoints);
+  std::vector<int> bpoff{0}, bpp, ptslot((size_t)nent);
+  std::vector<int> mark(nproj, -1);
+  int max_slots = 0;
+  for (int b = 0; b < nblock; ++b) {
+    const long long r0 = (long long)b * kRsApplyPoints;
+    const long long r1 = std::min<long long>(N, r0 + kRsApplyPoints);
+    std::vector<int> set;
+    for (long long e = ptoff[(size_t)r0]; e < ptoff[(size_t)r1]; ++e) {
+      const int p = ptp[(size_t)e];
+      if (mark[p] != b) {
+        mark[p] = b;
+        set.push_back(p);
+      }
+    }
+    std::sort(set.begin(), set.end());
+    if ((int)set.size() > kRsMaxSlots) set.resize(kRsMaxSlots);
+    for (long long e = ptoff[(size_t)r0]; e < ptoff[(size_t)r1]; ++e) {
+      const int p = ptp[(size_t)e];
+      const auto it = std::lower_bound(set.begin(), set.end(), p);
+      ptslot[(size_t)e] = (it != set.end() && *it == p) ? (int)(it - set.begin()) : -1 - p;
+    }
+    max_slots = std::max(max_slots, (int)set.size());
+    bpp.insert(bpp.end(), set.begin(), set.end());
+    bpoff.push_back((int)bpp.size());
+  }
+  r->nblock = nblock;
+  r->max_slots = max_slots;
   std::vector<float> gval32(gval.begin(), gval.end());
   std::vector<float> ptw32(ptw.begin(), ptw.end());
   std::vector<double> esc(nproj);
@@ -224,7 +268,9 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
   if (!rc) rc = rs_upload(&r->gp0, gp0, c.stream);
   if (!rc) rc = rs_upload(&r->gnch, gnch, c.stream);
   if (!rc) rc = rs_upload(&r->ptoff, ptoff, c.stream);
-  if (!rc) rc = rs_upload(&r->ptp, ptp, c.stream);
+  if (!rc) rc = rs_upload(&r->ptslot, ptslot, c.stream);
+  if (!rc) rc = rs_upload(&r->bpoff, bpoff, c.stream);
+  if (!rc) rc = rs_upload(&r->bpp, bpp, c.stream);
   if (!rc) rc = rs_upload(&r->ptw64, ptw, c.stream);
   if (!rc) rc = rs_upload(&r->ptw32, ptw32, c.stream);
   if (!rc) rc = rs_upload(&r->escale, esc, c.stream);
@@ -258,8 +304,6 @@ __device__ inline T wave_sum(T v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-
-constexpr int kRsThreads = 256;
 
 // Z[p][q] = sum_k beta_p[k] z_q(r_k) for the channels p of group g = blockIdx.x
 // and the pairs q0 <= q < q0 + QC (q0 = blockIdx.y QC).  Each thread keeps
@@ -329,43 +373,75 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_project(
   }
 }
 
-// z_q(r) <- v(r) z_q(r) + sum_{e in point r} w_e Z[p_e][q] for the pairs
-// q0 <= q < q0 + QC, one grid point per thread.
-template <typename T, int QC>
+// z_q(r) <- v(r) z_q(r) + sum_{e in point r} w_e Z[p_e][q] over all pairs,
+// one grid point per thread, one block of consecutive points per workgroup.
+// The Z rows of the projectors the block's points touch are staged in LDS,
+// kRsStagePairs pairs at a time, so each row is read from memory once per
+// block instead of once per point and entry; an entry whose projector did not
+// fit the staged set (slot < 0) reads Z directly.
+template <typename T>
 __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
-    const long long* __restrict__ ptoff, const int* __restrict__ ptp, const T* __restrict__ ptw,
-    const T* __restrict__ v, const Cx<T>* __restrict__ Z, long long Ngrid, int npairs,
-    Cx<T>* __restrict__ F) {
-  const long long r = (long long)blockIdx.x * kRsThreads + threadIdx.x;
-  if (r >= Ngrid) return;
-  const int q0 = blockIdx.y * QC;
-  const int nq = npairs - q0 < QC ? npairs - q0 : QC;
-  const T vr = v[r];
-  Cx<T> acc[QC];
-  Cx<T>* Fq = F + (size_t)q0 * Ngrid + r;
-#pragma unroll
-  for (int qq = 0; qq < QC; ++qq) {
-    if (qq < nq) {
-      const Cx<T> z = Fq[(size_t)qq * Ngrid];
-      acc[qq] = cx(vr * z.x, vr * z.y);
+    const long long* __restrict__ ptoff, const int* __restrict__ ptslot, const T* __restrict__ ptw,
+    const int* __restrict__ bpoff, const int* __restrict__ bpp, const T* __restrict__ v,
+    const Cx<T>* __restrict__ Z, long long Ngrid, int npairs, Cx<T>* __restrict__ F) {
+  constexpr int QZ = kRsStagePairs;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* Zl = reinterpret_cast<Cx<T>*>(smem_raw);  // [slots][QZ]
+  const int b = blockIdx.x;
+  const long long r = (long long)b * kRsApplyPoints + threadIdx.x;
+  const bool live = r < Ngrid;
+  const int s0 = bpoff[b], ns = bpoff[b + 1] - s0;
+  const T vr = live ? v[r] : T(0);
+  const long long e0 = live ? ptoff[r] : 0, e1 = live ? ptoff[r + 1] : 0;
+  const Cx<T> zero = cx(T(0), T(0));
+  for (int q0 = 0; q0 < npairs; q0 += QZ) {
+    const int nq = npairs - q0 < QZ ? npairs - q0 : QZ;
+    __syncthreads();  // the previous pass's readers are done with Zl
+    for (int i = threadIdx.x; i < ns * QZ; i += kRsThreads) {
+      const int sl = i / QZ, qq = i - sl * QZ;
+      Zl[i] = qq < nq ? Z[(size_t)bpp[s0 + sl] * npairs + q0 + qq] : zero;
     }
-  }
-  const long long e1 = ptoff[r + 1];
-  for (long long e = ptoff[r]; e < e1; ++e) {
-    const T w = ptw[e];
-    const Cx<T>* zp = Z + (size_t)ptp[e] * npairs + q0;
+    __syncthreads();
+    if (live) {
+      Cx<T> acc[QZ];
+      Cx<T>* Fq = F + (size_t)q0 * Ngrid + r;
 #pragma unroll
-    for (int qq = 0; qq < QC; ++qq) {
-      if (qq < nq) {
-        const Cx<T> zz = zp[qq];
-        acc[qq].x += w * zz.x;
-        acc[qq].y += w * zz.y;
+      for (int qq = 0; qq < QZ; ++qq) {
+        if (qq < nq) {
+          const Cx<T> z = Fq[(size_t)qq * Ngrid];
+          acc[qq] = cx(vr * z.x, vr * z.y);
+        } else {
+          acc[qq] = zero;
+        }
       }
+      for (long long e = e0; e < e1; ++e) {
+        const T w = ptw[e];
+        const int sl = ptslot[e];
+        if (sl >= 0) {
+          const Cx<T>* zl = Zl + sl * QZ;
+#pragma unroll
+          for (int qq = 0; qq < QZ; ++qq) {
+            const Cx<T> zz = zl[qq];
+            acc[qq].x += w * zz.x;
+            acc[qq].y += w * zz.y;
+          }
+        } else {
+          const Cx<T>* zg = Z + (size_t)(-1 - sl) * npairs + q0;
+#pragma unroll
+          for (int qq = 0; qq < QZ; ++qq) {
+            if (qq < nq) {
+              const Cx<T> zz = zg[qq];
+              acc[qq].x += w * zz.x;
+              acc[qq].y += w * zz.y;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < QZ; ++qq)
+        if (qq < nq) Fq[(size_t)qq * Ngrid] = acc[qq];
     }
   }
-#pragma unroll
-  for (int qq = 0; qq < QC; ++qq)
-    if (qq < nq) Fq[(size_t)qq * Ngrid] = acc[qq];
 }
 
 // part[b] = sum over (p, q) of escale[p] (Re Z^2 + Im Z^2), the imaginary
@@ -400,7 +476,17 @@ static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
   a.field = F;
   const dim3 grid(N, npairs);
   const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
-  if (lds <= 160 * 1024) {
+  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
+  // the split form recomputes the c-transforms per column half; it pays where
+  // its smaller LDS buffer fits at least twice as many workgroups per CU
+  // (e.g. FP32 at 192^3: 117 KB -> 66 KB, one workgroup per CU -> two)
+  constexpr size_t kLds = 160 * 1024;
+  if (lds > kLds && lds2 > kLds) {
+    set_error("plane buffer exceeds LDS for this grid in this precision");
+    return SDFT_EINVAL_CODE;
+  }
+  const bool split_wins = lds2 <= kLds && (lds > kLds || kLds / lds2 >= 2 * (kLds / lds));
+  if (!split_wins) {
     if (plane_prunable(g.K, N)) {
       SDFT_HIP_CHECK(hipFuncSetAttribute(
           reinterpret_cast<const void*>(&k_plane_field<T, N, true, MODE>),
@@ -416,11 +502,6 @@ static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
     }
     SDFT_HIP_CHECK(hipGetLastError());
     return 0;
-  }
-  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
-  if (lds2 > 160 * 1024) {
-    set_error("plane buffer exceeds LDS for this grid in this precision");
-    return SDFT_EINVAL_CODE;
   }
   a.Wout = W;  // MODE 2 writes the result in place of its (unread) input
   SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_split_field<T, N, MODE>),
@@ -480,16 +561,17 @@ static int project_impl(Ctx& c, int npairs) {
 template <typename T>
 static int apply_impl(Ctx& c, int npairs) {
   RsKB& r = *c.rs;
-  constexpr int QC = rs_qc<T>();
   const long long N = c.g.N;
-  const dim3 grid((unsigned)((N + kRsThreads - 1) / kRsThreads), (npairs + QC - 1) / QC);
+  const size_t lds = sizeof(Cx<T>) * (size_t)std::max(1, r.max_slots) * kRsStagePairs;
   const T* ptw = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.ptw64)
                                 : reinterpret_cast<const T*>(r.ptw32);
   const T* v = sizeof(T) == 8 ? reinterpret_cast<const T*>(c.vks64)
                               : reinterpret_cast<const T*>(c.vks32);
-  hipLaunchKernelGGL((k_rs_apply<T, QC>), grid, dim3(kRsThreads), 0, c.stream, r.ptoff, r.ptp,
-                     ptw, v, reinterpret_cast<const Cx<T>*>(r.Z), N, npairs,
-                     reinterpret_cast<Cx<T>*>(r.F));
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rs_apply<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_rs_apply<T>), dim3(r.nblock), dim3(kRsThreads), lds, c.stream, r.ptoff,
+                     r.ptslot, ptw, r.bpoff, r.bpp, v, reinterpret_cast<const Cx<T>*>(r.Z), N,
+                     npairs, reinterpret_cast<Cx<T>*>(r.F));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
