@@ -425,13 +425,15 @@ def workload_bench(key, nsteps=24, block=128):
                                         "contract_a") if k in kt) / steps
         grid_b = float(N) * wc * npairs
         wbytes = float(wl.n) * ncol * wc * npairs
-        qc = 8 if prec == hip.PREC_FP32 else 4  # pairs per workgroup (product, apply)
+        qc = 8 if prec == hip.PREC_FP32 else 4  # pairs per product workgroup
         nch = -(-npairs // qc)
         wr = wc // 2  # one real value
+        # product: the grids at every (atom, point) entry, beta + index per
+        # pair chunk; apply: the grids read and written, v, the point offsets
+        # and the (slot, weight) lists once per point
         model = {"plane_bc_to_r": wbytes + grid_b,
                  "rs_project": float(nent_g) * wc * npairs + float(nent_g) * (4 + 4 * wr) * nch,
-                 "rs_apply_vpsi": 2.0 * grid_b + (float(N) * (wv + 8) + float(nent) * (4 + wr))
-                 * nch,
+                 "rs_apply_vpsi": 2.0 * grid_b + float(N) * (wv + 8) + float(nent) * (4 + wr),
                  "plane_r_to_bc": grid_b + wbytes}
         kern = {}
         for k, ms in per.items():

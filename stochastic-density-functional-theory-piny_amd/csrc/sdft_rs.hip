@@ -32,11 +32,28 @@
 namespace sdft {
 
 constexpr int kRsThreads = 256;
-// the apply kernel's point blocks (one point per thread) and the largest
-// projector set it stages per block; pairs staged per pass
+// The apply kernel's point blocks, one point per thread: a 16 x 16 tile of
+// (rb, rc) in one a-plane where n is a multiple of 16 (the projectors a tile
+// touches are those of the few atoms near it), else a run of 256 consecutive
+// grid points.  At most kRsMaxSlots projectors per block are staged in LDS,
+// and at most kRsStageBytes of their Z rows at a time; a point's first
+// kRsRegEntries list entries are held in registers across the pairs.
 constexpr int kRsApplyPoints = kRsThreads;
+constexpr int kRsTile = 16;
 constexpr int kRsMaxSlots = 256;
-constexpr int kRsStagePairs = 16;
+constexpr size_t kRsStageBytes = 64 * 1024;
+constexpr int kRsRegEntries = 16;
+constexpr int kRsAccPairs = 16;
+
+// grid point of thread t of apply block b (tiled: nt = n / 16 tiles per axis)
+__host__ __device__ inline long long rs_block_point(int b, int t, int n, bool tiled) {
+  if (!tiled) return (long long)b * kRsApplyPoints + t;
+  const int nt = n / kRsTile;
+  const int ra = b / (nt * nt), rem = b - ra * nt * nt;
+  const int tb = rem / nt, tc = rem - tb * nt;
+  const int rb = tb * kRsTile + t / kRsTile, rc = tc * kRsTile + t % kRsTile;
+  return ((long long)ra * n + rb) * n + rc;
+}
 
 struct RsKB {
   int nproj = 0;
@@ -62,6 +79,7 @@ struct RsKB {
   int* bpp = nullptr;         // projector ids
   int nblock = 0;
   int max_slots = 0;          // largest staged set
+  bool tiled = false;         // 16 x 16 (rb, rc) tiles (else runs of 256 points)
   double* escale = nullptr;   // [nproj] E_p vol_elem / N (energies)
   // per-call buffers
   void* F = nullptr;          // pair grids, complex in the form's type
@@ -228,32 +246,42 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
     }
   }
   // per point block: the distinct projectors (ascending), each entry's slot
-  const int nblock = (int)((N + kRsApplyPoints - 1) / kRsApplyPoints);
+  const int n = c.g.n;
+  const bool tiled = n % kRsTile == 0;
+  const int nblock = tiled ? n * (n / kRsTile) * (n / kRsTile)
+                           : (int)((N + kRsApplyPoints - 1) / kRsApplyPoints);
   std::vector<int> bpoff{0}, bpp, ptslot((size_t)nent);
   std::vector<int> mark(nproj, -1);
   int max_slots = 0;
   for (int b = 0; b < nblock; ++b) {
-    const long long r0 = (long long)b * kRsApplyPoints;
-    const long long r1 = std::min<long long>(N, r0 + kRsApplyPoints);
     std::vector<int> set;
-    for (long long e = ptoff[(size_t)r0]; e < ptoff[(size_t)r1]; ++e) {
-      const int p = ptp[(size_t)e];
-      if (mark[p] != b) {
-        mark[p] = b;
-        set.push_back(p);
+    for (int t = 0; t < kRsApplyPoints; ++t) {
+      const long long pt = rs_block_point(b, t, n, tiled);
+      if (pt >= N) break;
+      for (long long e = ptoff[(size_t)pt]; e < ptoff[(size_t)pt + 1]; ++e) {
+        const int p = ptp[(size_t)e];
+        if (mark[p] != b) {
+          mark[p] = b;
+          set.push_back(p);
+        }
       }
     }
     std::sort(set.begin(), set.end());
     if ((int)set.size() > kRsMaxSlots) set.resize(kRsMaxSlots);
-    for (long long e = ptoff[(size_t)r0]; e < ptoff[(size_t)r1]; ++e) {
-      const int p = ptp[(size_t)e];
-      const auto it = std::lower_bound(set.begin(), set.end(), p);
-      ptslot[(size_t)e] = (it != set.end() && *it == p) ? (int)(it - set.begin()) : -1 - p;
+    for (int t = 0; t < kRsApplyPoints; ++t) {
+      const long long pt = rs_block_point(b, t, n, tiled);
+      if (pt >= N) break;
+      for (long long e = ptoff[(size_t)pt]; e < ptoff[(size_t)pt + 1]; ++e) {
+        const int p = ptp[(size_t)e];
+        const auto it = std::lower_bound(set.begin(), set.end(), p);
+        ptslot[(size_t)e] = (it != set.end() && *it == p) ? (int)(it - set.begin()) : -1 - p;
+      }
     }
     max_slots = std::max(max_slots, (int)set.size());
     bpp.insert(bpp.end(), set.begin(), set.end());
     bpoff.push_back((int)bpp.size());
   }
+  r->tiled = tiled;
   r->nblock = nblock;
   r->max_slots = max_slots;
   std::vector<float> gval32(gval.begin(), gval.end());
@@ -309,14 +337,21 @@ __device__ inline T wave_sum(T v) {
 // and the pairs q0 <= q < q0 + QC (q0 = blockIdx.y QC).  Each thread keeps
 // QC x 4 complex partial sums over its share of the group's points; the wave
 // sums, then the workgroup's four waves, give Z.
+// Workgroup w runs group g's pair chunk j with w = ((g / 8) nchunk + j) 8 +
+// g % 8: workgroups are dealt to the 8 XCDs in turn (w mod 8), so all chunks
+// of a group run on one XCD at about the same time and re-read its beta
+// values and indices from that XCD's L2.
 template <typename T, int QC>
 __global__ __launch_bounds__(kRsThreads) void k_rs_project(
     const long long* __restrict__ goff, const int* __restrict__ gidx, const T* __restrict__ gval,
     const int* __restrict__ gp0, const int* __restrict__ gnch, const Cx<T>* __restrict__ F,
-    long long Ngrid, int npairs, Cx<T>* __restrict__ Z) {
+    long long Ngrid, int npairs, int ngroup, int nchunk, Cx<T>* __restrict__ Z) {
   __shared__ T red[kRsThreads / 64][QC * 8];
-  const int g = blockIdx.x;
-  const int q0 = blockIdx.y * QC;
+  const int w = blockIdx.x;
+  const int rest = w >> 3;
+  const int g = (rest / nchunk) * 8 + (w & 7);
+  if (g >= ngroup) return;
+  const int q0 = (rest % nchunk) * QC;
   const int nq = npairs - q0 < QC ? npairs - q0 : QC;
   const long long e0 = goff[g], e1 = goff[g + 1];
   T ar[QC][4], ai[QC][4];
@@ -374,39 +409,51 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_project(
 }
 
 // z_q(r) <- v(r) z_q(r) + sum_{e in point r} w_e Z[p_e][q] over all pairs,
-// one grid point per thread, one block of consecutive points per workgroup.
-// The Z rows of the projectors the block's points touch are staged in LDS,
-// kRsStagePairs pairs at a time, so each row is read from memory once per
-// block instead of once per point and entry; an entry whose projector did not
-// fit the staged set (slot < 0) reads Z directly.
+// one grid point per thread, one point block per workgroup.  The Z rows of
+// the projectors the block's points touch are staged in LDS, qz pairs at a
+// time, so each row is read from memory once per block instead of once per
+// point and entry; an entry whose projector did not fit the staged set
+// (slot < 0) reads Z directly.  A point's first kRsRegEntries entries stay in
+// registers across the pairs; the rest (crowded points) are re-read.
 template <typename T>
 __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
     const long long* __restrict__ ptoff, const int* __restrict__ ptslot, const T* __restrict__ ptw,
     const int* __restrict__ bpoff, const int* __restrict__ bpp, const T* __restrict__ v,
-    const Cx<T>* __restrict__ Z, long long Ngrid, int npairs, Cx<T>* __restrict__ F) {
-  constexpr int QZ = kRsStagePairs;
+    const Cx<T>* __restrict__ Z, long long Ngrid, int n, int tiled, int npairs, int qz,
+    Cx<T>* __restrict__ F) {
+  constexpr int QA = kRsAccPairs, ER = kRsRegEntries;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cx<T>* Zl = reinterpret_cast<Cx<T>*>(smem_raw);  // [slots][QZ]
+  Cx<T>* Zl = reinterpret_cast<Cx<T>*>(smem_raw);  // [slots][qz]
   const int b = blockIdx.x;
-  const long long r = (long long)b * kRsApplyPoints + threadIdx.x;
+  const long long r = rs_block_point(b, threadIdx.x, n, tiled != 0);
   const bool live = r < Ngrid;
   const int s0 = bpoff[b], ns = bpoff[b + 1] - s0;
   const T vr = live ? v[r] : T(0);
   const long long e0 = live ? ptoff[r] : 0, e1 = live ? ptoff[r + 1] : 0;
+  const int ne = (int)(e1 - e0);
+  int sl_r[ER];
+  T w_r[ER];
+#pragma unroll
+  for (int i = 0; i < ER; ++i) {
+    sl_r[i] = i < ne ? ptslot[e0 + i] : 0;
+    w_r[i] = i < ne ? ptw[e0 + i] : T(0);
+  }
   const Cx<T> zero = cx(T(0), T(0));
-  for (int q0 = 0; q0 < npairs; q0 += QZ) {
-    const int nq = npairs - q0 < QZ ? npairs - q0 : QZ;
+  for (int q0 = 0; q0 < npairs; q0 += qz) {
+    const int nqs = npairs - q0 < qz ? npairs - q0 : qz;
     __syncthreads();  // the previous pass's readers are done with Zl
-    for (int i = threadIdx.x; i < ns * QZ; i += kRsThreads) {
-      const int sl = i / QZ, qq = i - sl * QZ;
-      Zl[i] = qq < nq ? Z[(size_t)bpp[s0 + sl] * npairs + q0 + qq] : zero;
+    for (int i = threadIdx.x; i < ns * qz; i += kRsThreads) {
+      const int sl = i / qz, qq = i - sl * qz;
+      Zl[i] = qq < nqs ? Z[(size_t)bpp[s0 + sl] * npairs + q0 + qq] : zero;
     }
     __syncthreads();
-    if (live) {
-      Cx<T> acc[QZ];
-      Cx<T>* Fq = F + (size_t)q0 * Ngrid + r;
+    if (!live) continue;
+    for (int q1 = 0; q1 < nqs; q1 += QA) {
+      const int nq = nqs - q1 < QA ? nqs - q1 : QA;
+      Cx<T> acc[QA];
+      Cx<T>* Fq = F + (size_t)(q0 + q1) * Ngrid + r;
 #pragma unroll
-      for (int qq = 0; qq < QZ; ++qq) {
+      for (int qq = 0; qq < QA; ++qq) {
         if (qq < nq) {
           const Cx<T> z = Fq[(size_t)qq * Ngrid];
           acc[qq] = cx(vr * z.x, vr * z.y);
@@ -414,31 +461,37 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
           acc[qq] = zero;
         }
       }
-      for (long long e = e0; e < e1; ++e) {
-        const T w = ptw[e];
-        const int sl = ptslot[e];
-        if (sl >= 0) {
-          const Cx<T>* zl = Zl + sl * QZ;
+      // the LDS and the direct reads stay in separate branches with their
+      // own index expressions (one merged pointer would be a flat address)
+#define SDFT_RS_ADD(SL, W)                                                  \
+  do {                                                                      \
+    const int sl_ = (SL);                                                   \
+    const T w_ = (W);                                                       \
+    if (sl_ >= 0) {                                                         \
+      _Pragma("unroll") for (int qq = 0; qq < QA; ++qq) {                   \
+        if (qq < nq) {                                                      \
+          const Cx<T> zz = Zl[sl_ * qz + q1 + qq];                          \
+          acc[qq].x += w_ * zz.x;                                           \
+          acc[qq].y += w_ * zz.y;                                           \
+        }                                                                   \
+      }                                                                     \
+    } else {                                                                \
+      _Pragma("unroll") for (int qq = 0; qq < QA; ++qq) {                   \
+        if (qq < nq) {                                                      \
+          const Cx<T> zz = Z[(size_t)(-1 - sl_) * npairs + q0 + q1 + qq];   \
+          acc[qq].x += w_ * zz.x;                                           \
+          acc[qq].y += w_ * zz.y;                                           \
+        }                                                                   \
+      }                                                                     \
+    }                                                                       \
+  } while (0)
 #pragma unroll
-          for (int qq = 0; qq < QZ; ++qq) {
-            const Cx<T> zz = zl[qq];
-            acc[qq].x += w * zz.x;
-            acc[qq].y += w * zz.y;
-          }
-        } else {
-          const Cx<T>* zg = Z + (size_t)(-1 - sl) * npairs + q0;
+      for (int i = 0; i < ER; ++i)
+        if (i < ne) SDFT_RS_ADD(sl_r[i], w_r[i]);
+      for (long long e = e0 + ER; e < e1; ++e) SDFT_RS_ADD(ptslot[e], ptw[e]);
+#undef SDFT_RS_ADD
 #pragma unroll
-          for (int qq = 0; qq < QZ; ++qq) {
-            if (qq < nq) {
-              const Cx<T> zz = zg[qq];
-              acc[qq].x += w * zz.x;
-              acc[qq].y += w * zz.y;
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int qq = 0; qq < QZ; ++qq)
+      for (int qq = 0; qq < QA; ++qq)
         if (qq < nq) Fq[(size_t)qq * Ngrid] = acc[qq];
     }
   }
@@ -548,12 +601,13 @@ static int project_impl(Ctx& c, int npairs) {
   constexpr int QC = rs_qc<T>();
   int rc = rs_grow(&r.Z, &r.Z_bytes, sizeof(Cx<T>) * (size_t)r.nproj * npairs);
   if (rc) return rc;
-  const dim3 grid(r.ngroup, (npairs + QC - 1) / QC);
+  const int nchunk = (npairs + QC - 1) / QC;
+  const dim3 grid((unsigned)((r.ngroup + 7) / 8 * 8 * nchunk));
   const T* gval = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.gval64)
                                  : reinterpret_cast<const T*>(r.gval32);
   hipLaunchKernelGGL((k_rs_project<T, QC>), grid, dim3(kRsThreads), 0, c.stream, r.goff, r.gidx,
                      gval, r.gp0, r.gnch, reinterpret_cast<const Cx<T>*>(r.F), (long long)c.g.N,
-                     npairs, reinterpret_cast<Cx<T>*>(r.Z));
+                     npairs, r.ngroup, nchunk, reinterpret_cast<Cx<T>*>(r.Z));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -562,7 +616,13 @@ template <typename T>
 static int apply_impl(Ctx& c, int npairs) {
   RsKB& r = *c.rs;
   const long long N = c.g.N;
-  const size_t lds = sizeof(Cx<T>) * (size_t)std::max(1, r.max_slots) * kRsStagePairs;
+  const int slots = std::max(1, r.max_slots);
+  // pairs staged per pass: as many as kRsStageBytes hold for the largest
+  // set, a multiple of the accumulator width, at most all pairs
+  int qz = (int)(kRsStageBytes / (sizeof(Cx<T>) * slots));
+  qz = std::max(kRsAccPairs, qz / kRsAccPairs * kRsAccPairs);
+  qz = std::min(qz, (npairs + kRsAccPairs - 1) / kRsAccPairs * kRsAccPairs);
+  const size_t lds = sizeof(Cx<T>) * (size_t)slots * qz;
   const T* ptw = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.ptw64)
                                 : reinterpret_cast<const T*>(r.ptw32);
   const T* v = sizeof(T) == 8 ? reinterpret_cast<const T*>(c.vks64)
@@ -571,7 +631,7 @@ static int apply_impl(Ctx& c, int npairs) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL((k_rs_apply<T>), dim3(r.nblock), dim3(kRsThreads), lds, c.stream, r.ptoff,
                      r.ptslot, ptw, r.bpoff, r.bpp, v, reinterpret_cast<const Cx<T>*>(r.Z), N,
-                     npairs, reinterpret_cast<Cx<T>*>(r.F));
+                     c.g.n, r.tiled ? 1 : 0, npairs, qz, reinterpret_cast<Cx<T>*>(r.F));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
