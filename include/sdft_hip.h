@@ -93,6 +93,21 @@ int sdft_hip_set_projectors_real(sdft_ctx *ctx, int nproj, const long long *offs
                                  const int *grid_index, const double *values,
                                  const double *energy, double vol_elem);
 
+/* Gradient tables of the real-space projectors for the Hellmann-Feynman NL
+ * force in its per-atom form (nlppKBRealEnergyForce,
+ * energy/cp/cp_energy_eext_nonloc_real_force.c:36-240): dx/dy/dz hold
+ * d beta_p / d r_j at the same points and in the same order as the values
+ * of the last sdft_hip_set_projectors_real call (offsets[nproj] entries each;
+ * PINY's vnlPhiDx/Dy/DzAtomGridRe/Im rows, the derivative with respect to the
+ * grid point r - R_a).  With them set, sdft_hip_nl_forces computes
+ *   F_a = (occ/nsto) sum_s sum_{p in a} 2 energy_p vol_elem Z_ps Zd_ps / N
+ * with Z = sum_r beta_p psi_s and Zd = sum_r (d beta_p / d r) psi_s on the
+ * real-space grid (psi_s the unnormalised G->r transform), without the
+ * G-space table; its gvec argument is then unused.  Replaced by the next
+ * sdft_hip_set_projectors_real / sdft_hip_set_projectors call. */
+int sdft_hip_set_projector_gradients(sdft_ctx *ctx, int nproj, const double *dx,
+                                     const double *dy, const double *dz);
+
 /* How real-space projectors are applied (no reference counterpart: a choice
  * of this library).  form 0 (default): in their per-atom form, as the
  * reference does -- each H application transforms the pair grids to real
@@ -102,8 +117,8 @@ int sdft_hip_set_projectors_real(sdft_ctx *ctx, int nproj, const long long *offs
  * transforms back; the cost grows with the projectors' grid points, not with
  * P x ncoef.  form 1: through the equivalent G-space table (built on the
  * device when first needed), applied by the MFMA products of
- * sdft_hip_set_projectors.  sdft_hip_nl_forces always uses the table (it is
- * built on its first call); fragment batches need form 1. */
+ * sdft_hip_set_projectors.  sdft_hip_nl_forces uses the table (built on its
+ * first call) unless gradient tables are set; fragment batches need form 1. */
 int sdft_hip_set_nl_form(sdft_ctx *ctx, int form);
 
 /* Raw coefficient force f = -dE/dc* of H applied to nstate orbitals

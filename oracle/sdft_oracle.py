@@ -34,6 +34,8 @@ Parity status of each function (how it is pinned; see tests/):
   density_real                              : sum(rho)/N = occ <chi|chi>/Nsto
       identity and the brute-force real-space sum
   energy_ke / energy_nl / nl_forces         : closed forms and FD derivatives
+  nl_forces_real                            : central differences of energy_nl_real
+      with the projectors resampled at displaced atoms
   energy_nl_real                            : equals occ/Nsto sum_s <psi_s|V_NL|psi_s>
       with V_NL psi from nl_apply_real through hpsi_raw (Gamma dot of the
       raw-force difference)
@@ -637,6 +639,34 @@ def energy_nl_real(C, sysm, rs, occ, nsto):
             d = rs.vol_elem * (rs.values[sl] @ z[rs.index[sl]])
             e += (d.real ** 2 + d.imag ** 2) * rs.energy[p] / sysm.volume
     return occ * e / nsto
+
+
+def nl_forces_real(C, sysm, rs, grad, occ, nsto):
+    """Real-space KB Hellmann-Feynman forces, nlppKBRealEnergyForce
+    (energy/cp/cp_energy_eext_nonloc_real_force.c:36-240): per orbital the
+    grid values psi(r), per projector channel dot = volElem sum_r beta_p psi
+    and dotDev_j = volElem sum_r (d beta_p / d r_j) psi (:163-209, the gradient
+    tables vnlPhiD{x,y,z}AtomGrid of :483-494), and the force on the channel's
+    atom (d beta / d R_a = -d beta / d r) F_j += 2 vpsNorm_p / V dot dotDev_j
+    (the reference's 4 vpsNorm of the m > 0 rows is the caller's doubled
+    energy_p), scaled by occ / N_sto.  grad: (3, nent) aligned with
+    rs.values.  Returns (natm, 3)."""
+    nstate = C.shape[0]
+    shape = tuple(sysm.n)
+    zero = np.zeros(C.shape[1], dtype=np.complex128)
+    F = np.zeros((sysm.natm, 3))
+    for s in range(0, nstate, 2):
+        c2 = C[s + 1] if s + 1 < nstate else zero
+        z = to_r(pack_pair(C[s], c2, sysm.map, sysm.map_conj, shape)).ravel()
+        for p in range(len(rs.energy)):
+            sl = slice(rs.offsets[p], rs.offsets[p + 1])
+            zz = z[rs.index[sl]]
+            dot = rs.vol_elem * (rs.values[sl] @ zz)
+            for j in range(3):
+                dev = rs.vol_elem * (grad[j, sl] @ zz)
+                F[rs.atom[p], j] += 2.0 * rs.energy[p] / sysm.volume * (
+                    dot.real * dev.real + dot.imag * dev.imag)
+    return F * occ / nsto
 
 
 def nl_forces(C, sysm, occ, nsto):

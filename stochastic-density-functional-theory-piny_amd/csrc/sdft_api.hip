@@ -814,6 +814,44 @@ static int set_projectors_real_impl(Ctx& c, int nproj, const long long* off, con
   return rc;
 }
 
+// Real-space KB forces in the per-atom form (nlppKBRealEnergyForce): the
+// orbitals' grids, then per projector S_p,j = sum_s Z_ps Zd_ps,j with
+// Zd = sum_r (d beta_p / d r_j) psi_s; F_a,j = (occ/nsto) sum_{p in a}
+// 2 E_p dV S_p,j / N (d beta / d R_a = -d beta / d r).
+static int nl_forces_real(Ctx& c, int nstate, const double* re, const double* im,
+                          const int* proj_atom, int natm, double occ, double nsto,
+                          double* forces) {
+  int nproj;
+  const long long* off;
+  const int* idx;
+  const double *val, *energy;
+  double vol_elem;
+  rs_host_lists(c, &nproj, &off, &idx, &val, &energy, &vol_elem);
+  for (int p = 0; p < nproj; ++p)
+    if (proj_atom[p] < 0 || proj_atom[p] >= natm) {
+      set_error("projector atom index out of range");
+      return SDFT_EINVAL;
+    }
+  Block b;
+  int rc = block_alloc(c, nstate, 0, b);
+  if (!rc) rc = block_put(b, b.X0, re, im);
+  const int npairs = b.nstate_pad / 2;
+  if (!rc) rc = ensure_W(c, w_bytes(c, npairs, 0));
+  if (!rc) rc = launch_expand(c, 0, b.X0, npairs, c.W);
+  if (!rc) rc = rs_to_grid(c, 0, c.W, npairs);
+  std::vector<double> S((size_t)nproj * 3);
+  if (!rc) rc = rs_force_sums(c, npairs, S.data());
+  if (!rc) {
+    const double N = (double)c.g.N;
+    for (int p = 0; p < nproj; ++p)
+      for (int j = 0; j < 3; ++j)
+        forces[proj_atom[p] * 3 + j] +=
+            2.0 * energy[p] * vol_elem * S[(size_t)p * 3 + j] / N * occ / nsto;
+  }
+  block_free(b);
+  return rc;
+}
+
 }  // namespace sdft
 
 using namespace sdft;
@@ -917,6 +955,15 @@ int sdft_hip_set_projectors_real(sdft_ctx* h, int nproj, const long long* offset
   }
   return set_projectors_real_impl(h->c, nproj, offsets, grid_index, values, energy, vol_elem);
 }
+int sdft_hip_set_projector_gradients(sdft_ctx* h, int nproj, const double* dx, const double* dy,
+                                     const double* dz) {
+  if (!h || nproj < 1 || !dx || !dy || !dz) {
+    set_error("bad projector gradient arguments");
+    return SDFT_EINVAL;
+  }
+  return rs_set_gradients(h->c, nproj, dx, dy, dz);
+}
+
 int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* im,
                        const double* gvec, const int* proj_atom, int natm, double occ,
                        double nsto, double* forces) {
@@ -927,7 +974,10 @@ int sdft_hip_nl_forces(sdft_ctx* h, int nstate, const double* re, const double* 
   }
   Ctx& c = h->c;
   for (int a = 0; a < natm * 3; ++a) forces[a] = 0.0;
-  // real-space projectors: the forces use the G-space table they restrict to
+  if (rs_has_gradients(c)) return nl_forces_real(c, nstate, re, im, proj_atom, natm, occ, nsto,
+                                                 forces);
+  // real-space projectors without gradient tables: the forces use the
+  // G-space table they restrict to
   int rc0 = ensure_real_table(c);
   if (rc0) return rc0;
   if (c.nproj == 0) return 0;

@@ -410,6 +410,11 @@ int rs_project(Ctx& c, int prec, int npairs);
 int rs_apply(Ctx& c, int prec, int npairs);
 int rs_from_grid(Ctx& c, int prec, void* W, int npairs);
 int rs_energy(Ctx& c, int npairs, int nstate, double* out);
+// projector gradient tables (d beta_p / d r on the same point lists) and the
+// force sums S_p,j = sum_s Z_ps dZ_ps/d r_j over the grids of npairs FP64 pairs
+int rs_set_gradients(Ctx& c, int nproj, const double* dx, const double* dy, const double* dz);
+bool rs_has_gradients(const Ctx& c);
+int rs_force_sums(Ctx& c, int npairs, double* out);
 
 // ---- Chebyshev scalars (sdft_cheby.hip) ---------------------------------
 int cheby_coeffs_device(Ctx& c, int L, double emin, double emax, double mu, double beta,

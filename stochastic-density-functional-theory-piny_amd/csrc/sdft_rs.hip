@@ -88,6 +88,12 @@ struct RsKB {
   size_t Z_bytes = 0;
   double* epart = nullptr;    // energy partials
   size_t epart_bytes = 0;
+  // gradient tables d beta_p / d r (forces), per group entry [nent_g][4][3]
+  double* gdev = nullptr;
+  double* fpart = nullptr;    // force partials
+  size_t fpart_bytes = 0;
+  std::vector<long long> goff_h;  // the group structure (host)
+  std::vector<int> gp0_h, gnch_h;
   // the caller's lists (host), for the equivalent G-space table on demand
   std::vector<long long> off;
   std::vector<int> idx;
@@ -131,9 +137,9 @@ static int rs_grow(void** p, size_t* have, size_t need) {
 }
 
 static void rs_release(RsKB* r) {
-  void* ptrs[] = {r->goff,   r->gidx,  r->gval64, r->gval32, r->gp0, r->gnch,
+  void* ptrs[] = {r->goff,   r->gidx,  r->gval64, r->gval32, r->gp0,   r->gnch,
                   r->ptoff,  r->ptslot, r->ptw64, r->ptw32, r->bpoff, r->bpp,
-                  r->escale, r->F,     r->Z,     r->epart};
+                  r->escale, r->F,     r->Z,     r->epart,  r->gdev,  r->fpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete r;
@@ -158,6 +164,37 @@ void rs_host_lists(const Ctx& c, int* nproj, const long long** off, const int** 
   *val = r ? r->val.data() : nullptr;
   *energy = r ? r->energy.data() : nullptr;
   *vol_elem = r ? r->vol_elem : 0.0;
+}
+
+bool rs_has_gradients(const Ctx& c) { return c.rs != nullptr && c.rs->gdev != nullptr; }
+
+int rs_set_gradients(Ctx& c, int nproj, const double* dx, const double* dy, const double* dz) {
+  RsKB* r = c.rs;
+  if (!r || nproj != r->nproj) {
+    set_error("projector gradients: set the real-space projectors first, with the same count");
+    return SDFT_ESTATE;
+  }
+  const double* dd[3] = {dx, dy, dz};
+  for (int j = 0; j < 3; ++j)
+    for (long long e = 0; e < r->nent; ++e)
+      if (!std::isfinite(dd[j][e])) {
+        set_error("projector gradient value is not finite");
+        return SDFT_EINVAL;
+      }
+  std::vector<double> gd((size_t)r->nent_g * 12, 0.0);
+  for (int g = 0; g < r->ngroup; ++g) {
+    const int p0 = r->gp0_h[(size_t)g], nch = r->gnch_h[(size_t)g];
+    const long long k0 = r->goff_h[(size_t)g], kn = r->goff_h[(size_t)g + 1] - k0;
+    for (int ch = 0; ch < nch; ++ch)
+      for (long long k = 0; k < kn; ++k)
+        for (int j = 0; j < 3; ++j)
+          gd[(size_t)(k0 + k) * 12 + ch * 3 + j] = dd[j][r->off[(size_t)p0 + ch] + k];
+  }
+  if (r->gdev) SDFT_HIP_CHECK(hipFree(r->gdev));
+  r->gdev = nullptr;
+  int rc = rs_upload(&r->gdev, gd, c.stream);
+  if (!rc) SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return rc;
 }
 
 int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double* val,
@@ -230,6 +267,9 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
   }
   r->ngroup = (int)gp0.size();
   r->nent_g = goff.back();
+  r->goff_h = goff;
+  r->gp0_h = gp0;
+  r->gnch_h = gnch;
   // inverse lists by grid point, projectors ascending within a point
   std::vector<long long> ptoff((size_t)N + 1, 0);
   for (long long e = 0; e < nent; ++e) ++ptoff[(size_t)idx[e] + 1];
@@ -497,6 +537,78 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
   }
 }
 
+// Force sums of group g = blockIdx-mapped as in k_rs_project (FP64): over the
+// group's points Z_ch(q) = sum_k beta_ch z_q and Zd_ch,j(q) = sum_k (d beta_ch
+// / d r_j) z_q for the QC pairs of the chunk, then per channel and direction
+// S = sum_q (Re Z Re Zd + Im Z Im Zd) -- the two orbitals of each pair --
+// written to part[(g nchunk + chunk) 12 + 3 ch + j].
+template <int QC>
+__global__ __launch_bounds__(kRsThreads) void k_rs_force_sums(
+    const long long* __restrict__ goff, const int* __restrict__ gidx,
+    const double* __restrict__ gval, const double* __restrict__ gdev,
+    const Cx<double>* __restrict__ F, long long Ngrid, int npairs, int ngroup, int nchunk,
+    double* __restrict__ part) {
+  constexpr int NV = QC * 32;  // Z (4 x 2) + Zd (12 x 2) per pair
+  __shared__ double red[kRsThreads / 64][NV];
+  const int w = blockIdx.x;
+  const int rest = w >> 3;
+  const int g = (rest / nchunk) * 8 + (w & 7);
+  if (g >= ngroup) return;
+  const int chunk = rest % nchunk;
+  const int q0 = chunk * QC;
+  const int nq = npairs - q0 < QC ? npairs - q0 : QC;
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  const Cx<double>* Fq = F + (size_t)q0 * Ngrid;
+  for (long long e = goff[g] + threadIdx.x; e < goff[g + 1]; e += kRsThreads) {
+    const size_t r = (size_t)gidx[e];
+    double b[4], d[12];
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) b[ch] = gval[e * 4 + ch];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) d[i] = gdev[e * 12 + i];
+#pragma unroll
+    for (int qq = 0; qq < QC; ++qq) {
+      if (qq < nq) {
+        const Cx<double> z = Fq[(size_t)qq * Ngrid + r];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+          acc[qq * 32 + ch * 2 + 0] += b[ch] * z.x;
+          acc[qq * 32 + ch * 2 + 1] += b[ch] * z.y;
+        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+          acc[qq * 32 + 8 + i * 2 + 0] += d[i] * z.x;
+          acc[qq * 32 + 8 + i * 2 + 1] += d[i] * z.y;
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double v = wave_sum(acc[i]);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int ch = threadIdx.x / 3, j = threadIdx.x % 3;
+    double sum = 0.0;
+    for (int qq = 0; qq < nq; ++qq) {
+      double zr = 0.0, zi = 0.0, dr = 0.0, di = 0.0;
+      for (int wv = 0; wv < kRsThreads / 64; ++wv) {
+        zr += red[wv][qq * 32 + ch * 2 + 0];
+        zi += red[wv][qq * 32 + ch * 2 + 1];
+        dr += red[wv][qq * 32 + 8 + (ch * 3 + j) * 2 + 0];
+        di += red[wv][qq * 32 + 8 + (ch * 3 + j) * 2 + 1];
+      }
+      sum += zr * dr + zi * di;
+    }
+    part[((size_t)g * nchunk + chunk) * 12 + threadIdx.x] = sum;
+  }
+}
+
 // part[b] = sum over (p, q) of escale[p] (Re Z^2 + Im Z^2), the imaginary
 // part only where orbital 2q + 1 exists
 __global__ void k_rs_energy(const Cx<double>* __restrict__ Z, const double* __restrict__ esc,
@@ -642,6 +754,40 @@ int rs_project(Ctx& c, int prec, int npairs) {
 
 int rs_apply(Ctx& c, int prec, int npairs) {
   return prec == 0 ? apply_impl<double>(c, npairs) : apply_impl<float>(c, npairs);
+}
+
+// per projector and direction S_p,j = sum over the pairs' orbitals of
+// Z_p dZ_p/d r_j (real-space grids of npairs FP64 pairs in r.F); out
+// [nproj][3]
+int rs_force_sums(Ctx& c, int npairs, double* out) {
+  RsKB& r = *c.rs;
+  if (!r.gdev) {
+    set_error("internal error: no projector gradients");
+    return SDFT_EINVAL_CODE;
+  }
+  constexpr int QC = 2;
+  const int nchunk = (npairs + QC - 1) / QC;
+  const size_t nparts = (size_t)r.ngroup * nchunk * 12;
+  int rc = rs_grow(reinterpret_cast<void**>(&r.fpart), &r.fpart_bytes, sizeof(double) * nparts);
+  if (rc) return rc;
+  const dim3 grid((unsigned)((r.ngroup + 7) / 8 * 8 * nchunk));
+  hipLaunchKernelGGL((k_rs_force_sums<QC>), grid, dim3(kRsThreads), 0, c.stream, r.goff, r.gidx,
+                     r.gval64, r.gdev, reinterpret_cast<const Cx<double>*>(r.F),
+                     (long long)c.g.N, npairs, r.ngroup, nchunk, r.fpart);
+  SDFT_HIP_CHECK(hipGetLastError());
+  std::vector<double> h(nparts);
+  SDFT_HIP_CHECK(hipMemcpyAsync(h.data(), r.fpart, sizeof(double) * nparts,
+                                hipMemcpyDeviceToHost, c.stream));
+  SDFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (int p = 0; p < r.nproj * 3; ++p) out[p] = 0.0;
+  for (int g = 0; g < r.ngroup; ++g)
+    for (int ch = 0; ch < r.gnch_h[(size_t)g]; ++ch)
+      for (int j = 0; j < 3; ++j) {
+        double sum = 0.0;
+        for (int k = 0; k < nchunk; ++k) sum += h[((size_t)g * nchunk + k) * 12 + ch * 3 + j];
+        out[(size_t)(r.gp0_h[(size_t)g] + ch) * 3 + j] = sum;
+      }
+  return 0;
 }
 
 // after rs_project in FP64 on the same pairs

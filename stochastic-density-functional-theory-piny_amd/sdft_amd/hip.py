@@ -44,7 +44,7 @@ EXPORTS = [
     "sdft_hip_nl_forces", "sdft_hip_noise", "sdft_hip_block_noise",
     "sdft_hip_batch_create", "sdft_hip_batch_destroy", "sdft_hip_batch_filter",
     "sdft_hip_batch_autotune", "sdft_hip_batch_config", "sdft_hip_set_projectors_real",
-    "sdft_hip_set_nl_form",
+    "sdft_hip_set_nl_form", "sdft_hip_set_projector_gradients",
     "sdft_hip_spectral_bound", "sdft_hip_newton_filter", "sdft_hip_tune", "sdft_hip_set_config",
     "sdft_hip_ks_init", "sdft_hip_ks_potential", "sdft_hip_local_forces",
 ]
@@ -104,6 +104,7 @@ def load(path: str | None = None):
         "sdft_hip_set_projectors_real": ([vp, C.c_int, C.POINTER(C.c_longlong), _ip, _dp, _dp,
                                           C.c_double], C.c_int),
         "sdft_hip_set_nl_form": ([vp, C.c_int], C.c_int),
+        "sdft_hip_set_projector_gradients": ([vp, C.c_int, _dp, _dp, _dp], C.c_int),
         "sdft_hip_spectral_bound": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                                      _ip, C.c_int], C.c_int),
         "sdft_hip_newton_filter": ([vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
@@ -194,6 +195,7 @@ class Context:
         pr = np.ascontiguousarray(proj.real, dtype=np.float64)
         pi = np.ascontiguousarray(proj.imag, dtype=np.float64)
         e = np.ascontiguousarray(energy, dtype=np.float64)
+        self._rs_atom = None
         _check(self.lib.sdft_hip_set_projectors(self.h, pr.shape[0], _ptr(pr), _ptr(pi), _ptr(e)))
 
     def set_projectors_real(self, rs):
@@ -202,15 +204,26 @@ class Context:
         idx = np.ascontiguousarray(rs.index, dtype=np.int32)
         val = np.ascontiguousarray(rs.values, dtype=np.float64)
         e = np.ascontiguousarray(rs.energy, dtype=np.float64)
+        self._rs_energy = e
+        self._rs_atom = np.ascontiguousarray(rs.atom, dtype=np.int32)
         _check(self.lib.sdft_hip_set_projectors_real(
             self.h, len(e), off.ctypes.data_as(C.POINTER(C.c_longlong)),
             idx.ctypes.data_as(_ip), _ptr(val), _ptr(e), float(rs.vol_elem)))
+
+    def set_projector_gradients(self, grad):
+        """d beta / d r tables (3, nent) aligned with the real-space projector
+        values (systems.real_space_projector_gradients)."""
+        g = [np.ascontiguousarray(grad[j], dtype=np.float64) for j in range(3)]
+        self._grad = g
+        _check(self.lib.sdft_hip_set_projector_gradients(self.h, len(self._rs_energy),
+                                                         _ptr(g[0]), _ptr(g[1]), _ptr(g[2])))
 
     def set_nl_form(self, form):
         """Real-space projectors: 0 per-atom form on the grid, 1 G-space table."""
         _check(self.lib.sdft_hip_set_nl_form(self.h, int(form)))
 
     def clear_projectors(self):
+        self._rs_atom = None
         _check(self.lib.sdft_hip_set_projectors(self.h, 0, None, None, None))
 
     def hpsi_raw(self, Cz, prec=PREC_FP64):
@@ -372,7 +385,10 @@ class Context:
         s = self.sys
         re, im = _split(Cz)
         g = np.ascontiguousarray(s.gvec, dtype=np.float64)
-        pa = np.ascontiguousarray(s.proj_atom, dtype=np.int32)
+        # the atom of each projector the context holds: the real-space ones
+        # when set, else the system's G-space table
+        pa = self._rs_atom if getattr(self, "_rs_atom", None) is not None else \
+            np.ascontiguousarray(s.proj_atom, dtype=np.int32)
         out = np.zeros((s.natm, 3))
         _check(self.lib.sdft_hip_nl_forces(self.h, re.shape[0], _ptr(re), _ptr(im), _ptr(g),
                                            pa.ctypes.data_as(_ip), s.natm, occ, nsto, _ptr(out)))

@@ -342,6 +342,46 @@ def real_space_projectors(sysm, rcut: float = 4.0) -> RealSpaceProjectors:
                                np.array(atm))
 
 
+def real_space_projector_gradients(sysm, rs: RealSpaceProjectors) -> np.ndarray:
+    """The gradient tables of real_space_projectors' values (PINY's
+    vnlPhiDx/Dy/DzAtomGrid convention: d beta_p / d r at the grid point,
+    r - R_a the minimum-image offset from the atom; -d beta_p / d R_a), as
+    an array (3, nent) aligned with rs.values: s: -N0 g x_j / rc^2 Y_00,
+    p_m: N1 sqrt(3/(4 pi)) (delta_mj - x_m x_j / rc^2) g, g = exp(-r^2 / 2 rc^2)."""
+    n = sysm.n[0]
+    h = sysm.L / n
+    rc = sysm.rc_nl
+    out = np.zeros((3, int(rs.offsets[-1])))
+    nproj = len(rs.energy)
+    p = 0
+    while p < nproj:
+        a = int(rs.atom[p])
+        sl = slice(rs.offsets[p], rs.offsets[p + 1])
+        lin = rs.index[sl].astype(np.int64)
+        ijk = np.stack([lin // (n * n), (lin // n) % n, lin % n], axis=1).astype(np.float64)
+        d = ijk * h - sysm.pos[a]
+        d -= sysm.L * np.round(d / sysm.L)
+        r2 = (d * d).sum(axis=1)
+        g = np.exp(-r2 / (2.0 * rc * rc))
+        k = 0
+        for l, nm in ((0, 1), (1, 3)):
+            norm = 1.0 / math.sqrt(math.gamma(l + 1.5) * rc ** (2 * l + 3) / 2.0)
+            for m in range(nm):
+                if p + k >= nproj or int(rs.atom[p + k]) != a:
+                    break
+                slk = slice(rs.offsets[p + k], rs.offsets[p + k + 1])
+                for j in range(3):
+                    if l == 0:
+                        v = -norm / math.sqrt(4.0 * math.pi) * g * d[:, j] / (rc * rc)
+                    else:
+                        v = norm * math.sqrt(3.0 / (4.0 * math.pi)) * g * (
+                            (1.0 if m == j else 0.0) - d[:, m] * d[:, j] / (rc * rc))
+                    out[j, slk] = v
+                k += 1
+        p += k
+    return out
+
+
 def build_system(name: str, a0: float, rep: int, n: int, g2max: float = 12.0,
                  pos: Optional[np.ndarray] = None, with_nl: bool = True,
                  h_kb: tuple = (2.0, 1.0), cell: Optional[float] = None,

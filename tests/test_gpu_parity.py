@@ -645,3 +645,26 @@ def test_hpsi_real_space_kb_dense_overlap(hip, si8_system):
     assert e64 < 1e-12 and e32 < 2e-5, (e64, e32)
     assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref)
     ctx.close()
+
+
+def test_nl_forces_real_space_per_atom(hip, si8_system):
+    """Hellmann-Feynman NL forces in the per-atom real-space form
+    (sdft_hip_set_projector_gradients, then sdft_hip_nl_forces) against the
+    oracle's nlppKBRealEnergyForce restatement (itself pinned by finite
+    differences of the real-space energy): 1e-10 relative to the largest
+    component, for an odd orbital count (a zero-padded pair)."""
+    from sdft_amd import systems
+    say("case: per-atom real-space NL forces si8")
+    s = si8_system
+    rs = systems.real_space_projectors(s, 4.0)
+    grad = systems.real_space_projector_gradients(s, rs)
+    ctx = hip.Context(s, with_nl=False)
+    ctx.set_projectors_real(rs)
+    ctx.set_projector_gradients(grad)
+    C = systems.noise_orbitals(s, 5, seed=37)
+    F = ctx.nl_forces(C, 2.0, 5.0)
+    F_ref = O.nl_forces_real(C, s, rs, grad, 2.0, 5.0)
+    say("  per-atom NL forces: max |F| %.4g, rel %.3g" % (np.abs(F_ref).max(), rel(F, F_ref)))
+    assert np.abs(F_ref).max() > 0.1
+    assert rel(F, F_ref) < 1e-10
+    ctx.close()

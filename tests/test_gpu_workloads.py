@@ -168,7 +168,9 @@ def test_si512_mu_full_degree(hip):
 def test_si1728_forces(hip):
     """Si1728 with displaced atoms: Hellmann-Feynman forces against the
     oracle.  NL: the G-space table of 8 atoms' projectors on 4 orbitals
-    (sdft_hip_nl_forces vs the oracle's dE_NL/dR).  Local: the device density
+    (sdft_hip_nl_forces vs the oracle's dE_NL/dR), and the workload's own
+    real-space projectors on all 1728 atoms in the per-atom form (with their
+    gradient tables, vs nlppKBRealEnergyForce restated).  Local: the device density
     of the same 4 orbitals turned into v_KS on the density sphere, then the
     local-pseudopotential force on 16 atoms (sdft_hip_local_forces vs the
     oracle on the same density).  Both 1e-10 relative to the largest
@@ -203,3 +205,12 @@ def test_si1728_forces(hip):
     Fl = ctx.local_forces(s_loc.pos, types, ff, dens.gvec)
     say("  local forces: max |F| %.4g, rel %.3g" % (np.abs(ref).max(), rel(Fl, ref)))
     assert rel(Fl, ref) < 1e-10
+    # the NL force on all 1728 atoms in the per-atom real-space form, from the
+    # workload's own projectors and their gradient tables
+    grad = systems.real_space_projector_gradients(s, rs)
+    ctx.set_projector_gradients(grad)
+    Fa = ctx.nl_forces(C, 2.0, 4.0)
+    Fa_ref = O.nl_forces_real(C, s, rs, grad, 2.0, 4.0)
+    say("  all-atom per-atom NL forces: max |F| %.4g, rel %.3g" % (np.abs(Fa_ref).max(),
+                                                                  rel(Fa, Fa_ref)))
+    assert rel(Fa, Fa_ref) < 1e-10

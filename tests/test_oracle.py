@@ -437,6 +437,48 @@ def test_real_space_kb_energy_is_expectation_of_operator(si8_system):
     assert abs(e - expect) < 1e-11 * expect, (e, expect)
 
 
+def test_real_space_kb_forces_are_minus_energy_gradient(si8_system):
+    """nl_forces_real (nlppKBRealEnergyForce restated, with the builder's
+    analytic gradient tables) equals -dE_NL/dR_a from central differences of
+    energy_nl_real with the projectors resampled at the displaced atom (same
+    orbitals), for two atoms and all three directions; and the gradient
+    tables themselves match central differences of the sampled projectors."""
+    import dataclasses
+
+    from sdft_amd import systems
+    s = si8_system
+    rcut = 3.0
+    rs = systems.real_space_projectors(s, rcut)
+    grad = systems.real_space_projector_gradients(s, rs)
+    C = systems.noise_orbitals(s, 3, seed=33)
+    F = O.nl_forces_real(C, s, rs, grad, 2.0, 3.0)
+    h = 1.0e-4
+    for a in (0, 5):
+        for j in range(3):
+            e = []
+            for sign in (1.0, -1.0):
+                pos = s.pos.copy()
+                pos[a, j] += sign * h
+                sd = dataclasses.replace(s, pos=pos)
+                e.append(O.energy_nl_real(C, sd, systems.real_space_projectors(sd, rcut), 2.0,
+                                          3.0))
+            fd = -(e[0] - e[1]) / (2.0 * h)
+            assert abs(F[a, j] - fd) < 2e-6 * max(1.0, abs(fd)), (a, j, F[a, j], fd)
+    # the gradient table of atom 0's channels by central differences of beta
+    pos = s.pos.copy()
+    pos[0, 1] += h
+    rp = systems.real_space_projectors(dataclasses.replace(s, pos=pos), rcut)
+    pos[0, 1] -= 2 * h
+    rm = systems.real_space_projectors(dataclasses.replace(s, pos=pos), rcut)
+    for p in range(4):
+        sl = slice(rs.offsets[p], rs.offsets[p + 1])
+        if not (np.array_equal(rp.index[sl], rs.index[sl]) and
+                np.array_equal(rm.index[sl], rs.index[sl])):
+            continue
+        dbeta_dR = (rp.values[sl] - rm.values[sl]) / (2 * h)
+        assert np.max(np.abs(grad[1, sl] + dbeta_dR)) < 1e-6 * np.max(np.abs(grad[1, sl])) + 1e-9
+
+
 def test_real_space_spectral_bounds_contain_dense_spectrum(tiny_system):
     """stodft.spectral_bounds with real-space projectors (Gershgorin bound of
     the grid-space KB operator) contains the eigenvalues of the dense H built
