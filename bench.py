@@ -450,7 +450,58 @@ def workload_bench(key, nsteps=24, block=128):
         log("%s %s: %.4g H.psi/s, %.3f ms/step (NL %.3f, FFT %.3f ms/step)" % (
             key, tag, block * nsteps / dt, 1e3 * dt / nsteps, nl_ms, fft_ms))
     out["fp32_over_fp64"] = round(out["fp32"]["hpsi_per_s"] / out["fp64"]["hpsi_per_s"], 3)
+    out["end_to_end"] = workload_errors(ctx, sysm, rs, wl, emin, emax, block)
     ctx.close()
+    return out
+
+
+def workload_errors(ctx, sysm, rs, wl, emin, emax, block, degree=256):
+    """The workload's whole path on one 128-orbital block at a reduced degree
+    (the precision comparison, not the expansion's accuracy, is the point):
+    FP64 moments -> mu for N_e = 4 natm (tolerance 1e-6 electrons) -> sqrt(f)
+    coefficients -> the filter in both forms -> per form the density, the
+    total energy as calcTotEnergy assembles it without the ion-ion term
+    (E_kin + E_NL + E_H + E_ext + E_xc, stodft/calc-energy.c:329-333) and the
+    Hellmann-Feynman forces on every atom (NL in the per-atom real-space form
+    + local); FP32 against FP64."""
+    from sdft_amd import hip, systems
+    t0 = time.perf_counter()
+    beta = 1.0 / wl.kt
+    scale, emean = 2.0 / (emax - emin), 0.5 * (emin + emax)
+    chi = ctx.noise(block, wl.seed)
+    mom = ctx.cheby_moments(chi, degree, scale, emean)
+    mu, nmu = ctx.chem_pot(degree, emin, emax, beta, mom, block, sysm.nelec, 0.3, 0.4, tol=1e-6)
+    coeffs = ctx.cheby_coeffs(degree, emin, emax, mu, beta, 2)
+    dens = systems.density_sphere(sysm)
+    vext = systems.vloc_density(sysm, dens, separable=True)
+    vloc = systems.vloc_form_factor(sysm, dens)
+    types = np.zeros(sysm.natm, dtype=np.int32)
+    ctx.set_projector_gradients(systems.real_space_projector_gradients(sysm, rs))
+    ctx.ks_init(dens)
+    res = {}
+    for prec, tag in ((hip.PREC_FP64, "fp64"), (hip.PREC_FP32, "fp32")):
+        ctx.set_vks(sysm.vks)
+        C, _ = ctx.cheby_filter(chi, coeffs, scale, emean, prec=prec)
+        eke, enl = ctx.energies(C, 2.0, block)
+        rho = ctx.density(C, 2.0 / block)
+        _, en = ctx.ks_potential(rho, vext, sysm.volume, xc=0)
+        e = eke + enl + en["eh"] + en["eext"] + en["exc"]
+        f = ctx.nl_forces(C, 2.0, block) + ctx.local_forces(sysm.pos, types, vloc, dens.gvec)
+        res[tag] = (e, f, rho)
+    (e64, f64, r64), (e32, f32, r32) = res["fp64"], res["fp32"]
+    N = sysm.ngrid
+    ne64 = float(r64.sum()) / N
+    out = {"degree": degree, "mu": mu, "n_mu": nmu, "n_e": sysm.nelec,
+           "energy_fp64_ha": e64, "energy_per_atom_err_ha": abs(e32 - e64) / sysm.natm,
+           "force_max_ha_per_bohr": float(np.abs(f64).max()),
+           "force_err_max_ha_per_bohr": float(np.abs(f32 - f64).max()),
+           "density_l1_per_electron": float(np.abs(r32 - r64).sum() / N / ne64),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    log("%s end to end (L=%d): mu %.6f N %.7f, E %.6f Ha, FP32 err %.3g Ha/atom, forces %.3g "
+        "Ha/bohr (|F| max %.3g), density L1/e %.3g" % (
+            wl.name, degree, mu, nmu, e64, out["energy_per_atom_err_ha"],
+            out["force_err_max_ha_per_bohr"], out["force_max_ha_per_bohr"],
+            out["density_l1_per_electron"]))
     return out
 
 
