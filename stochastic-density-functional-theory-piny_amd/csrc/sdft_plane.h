@@ -90,9 +90,9 @@ size_t plane2_lds_bytes(int M) {
 // grid a.field[q][ra][rb][rc] (FFTW order).  MODE 2 starts there: the forward
 // b-lines are read from a.field (which already holds v.psi plus the real-space
 // KB term) and the stage continues as MODE 0.
-template <typename T, int N, int V, bool PR, int MODE = 0>
+template <typename T, int N, int V, bool PR, int MODE = 0, int NT = kThreads>
 __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N, V>::CB;
+  constexpr int LD = Cfg<N>::LD, ML = fft_max_lines(N, NT), CB = PlaneCfg<T, N, V>::CB;
   constexpr bool PAD = PlaneRows<T, N>::PAD;
   constexpr int LDC = PlaneRows<T, N>::LDC;
   constexpr int P = n_passes(N);
@@ -107,7 +107,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
   Cx<T>* tw = Q + CB * LD;
   const int tid = threadIdx.x;
   const int K = (M - 1) / 2;
-  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   __syncthreads();
   Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
   const Cx<T> zero = cx(T(0), T(0));
@@ -131,7 +131,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     };
     Cx<T>* buf = Pl + r0 * LDC;
     auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LDC + lds_slot<PAD>(pos)] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, kLineFast<T>, PR, false, PAD>(
+    fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, kLineFast<T>, PR, false, PAD>(
         buf, LDC, nl, tw, tid, lw, sb);
   }
 
@@ -155,19 +155,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       // inverse b-lines straight to the real-space grid (consecutive threads
       // on consecutive lines: runs of CB consecutive rc)
       auto sf = [&](int l, int pos, Cx<T> x) { F[(size_t)pos * N + rc0 + l] = x; };
-      fft_passes_io<T, N, +1, false, 0, P - 1, CB, kThreads, true, PR, false>(Q, LD, CB, tw, tid,
+      fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true, PR, false>(Q, LD, CB, tw, tid,
                                                                               lp, sf);
     } else if constexpr (MODE == 2) {
       auto lf = [&](int l, int pos) { return F[(size_t)pos * N + rc0 + l]; };
-      fft_passes_io<T, N, -1, true, 0, P - 1, CB, kThreads, true, false, PR>(Q, LD, CB, tw, tid,
+      fft_passes_io<T, N, -1, true, 0, P - 1, CB, NT, true, false, PR>(Q, LD, CB, tw, tid,
                                                                              lf, sp);
     } else {
       if constexpr (P >= 2)
-        fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true, PR, false>(Q, LD, CB, tw,
+        fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true, PR, false>(Q, LD, CB, tw,
                                                                                 tid, lp, sq);
-      fft_fused_vmul<T, N, CB, kThreads, true>(CB, tw, tid, lq, vf, sq);
+      fft_fused_vmul<T, N, CB, NT, true>(CB, tw, tid, lq, vf, sq);
       if constexpr (P >= 2)
-        fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true, false, PR>(Q, LD, CB, tw,
+        fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true, false, PR>(Q, LD, CB, tw,
                                                                                tid, lq, sp);
     }
   }
@@ -183,7 +183,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       const int d = rc[l * N + pos];
       if (d >= 0) Wp[d] = x;
     };
-    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, kThreads, kLineFast<T>, false, PR, PAD>(
+    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, NT, kLineFast<T>, false, PR, PAD>(
         buf, LDC, nl, tw, tid, lb, sw);
   }
 }
@@ -198,25 +198,27 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
 // input, and W (the input of both halves) is left intact.
 // S column parts of width H = n / S (the plane kernels use halves; the
 // density kernel also quarters, for the density sphere's rows at 192^3)
-template <typename T, int N, int S = 2>
+template <typename T, int N, int S = 2, int NT = kThreads>
 struct SplitCfg {
   static constexpr int H = N / S, LDH = H + 1;
-  static constexpr int QL = Cfg<N>::ML > PlaneCfg<T, N>::CB ? Cfg<N>::ML : PlaneCfg<T, N>::CB;
+  static constexpr int ML = fft_max_lines(N, NT);
+  static constexpr int QL = ML > PlaneCfg<T, N>::CB ? ML : PlaneCfg<T, N>::CB;
 };
 
-template <typename T, int N, int S = 2>
+template <typename T, int N, int S = 2, int NT = kThreads>
 size_t plane_split_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N, S>::LDH + (size_t)SplitCfg<T, N, S>::QL *
-                          Cfg<N>::LD + N);
+  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N, S, NT>::LDH +
+                          (size_t)SplitCfg<T, N, S, NT>::QL * Cfg<N>::LD + N);
 }
 
 // MODE 1 / 2 as in plane_body: per half, the inverse b-lines go to the
 // real-space grid a.field (no forward part), or the forward b-lines come from
 // it (no inverse part; W is not read, the output goes to Wout).
-template <typename T, int N, int MODE = 0>
+template <typename T, int N, int MODE = 0, int NT = kThreads>
 __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = Cfg<N>::ML, CB = PlaneCfg<T, N>::CB;
-  constexpr int H = SplitCfg<T, N>::H, LDH = SplitCfg<T, N>::LDH, QL = SplitCfg<T, N>::QL;
+  constexpr int LD = Cfg<N>::LD, ML = SplitCfg<T, N, 2, NT>::ML, CB = PlaneCfg<T, N>::CB;
+  constexpr int H = SplitCfg<T, N, 2, NT>::H, LDH = SplitCfg<T, N, 2, NT>::LDH;
+  constexpr int QL = SplitCfg<T, N, 2, NT>::QL;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
@@ -230,7 +232,7 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
   Cx<T>* tw = Q + QL * LD;
   const int tid = threadIdx.x;
   const int K = (M - 1) / 2;
-  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   __syncthreads();
   const Cx<T>* Wp = W + ((size_t)q * N + ra) * ncolp;
   Cx<T>* Wq = Wo + ((size_t)q * N + ra) * ncolp;
@@ -254,7 +256,7 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
         const int c = pos - c_lo;
         if (c >= 0 && c < H) Ph[(r0 + l) * LDH + c] = x;
       };
-      fft_passes_io<T, N, +1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lw, sh);
+      fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, false>(Q, LD, nl, tw, tid, lw, sh);
     }
     // along b, CB columns at a time
     for (int rc0 = 0; rc0 < H; rc0 += CB) {
@@ -272,17 +274,17 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
       };
       if constexpr (MODE == 1) {
         auto sf = [&](int l, int pos, Cx<T> x) { F[(size_t)pos * N + c_lo + rc0 + l] = x; };
-        fft_passes_io<T, N, +1, false, 0, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lp, sf);
+        fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lp, sf);
       } else if constexpr (MODE == 2) {
         auto lf = [&](int l, int pos) { return F[(size_t)pos * N + c_lo + rc0 + l]; };
-        fft_passes_io<T, N, -1, true, 0, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lf, sp);
+        fft_passes_io<T, N, -1, true, 0, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lf, sp);
       } else {
         if constexpr (P >= 2)
-          fft_passes_io<T, N, +1, false, 0, P - 2, CB, kThreads, true>(Q, LD, nb, tw, tid, lp,
+          fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true>(Q, LD, nb, tw, tid, lp,
                                                                        sq);
-        fft_fused_vmul<T, N, CB, kThreads, true>(nb, tw, tid, lq, vf, sq);
+        fft_fused_vmul<T, N, CB, NT, true>(nb, tw, tid, lq, vf, sq);
         if constexpr (P >= 2)
-          fft_passes_io<T, N, -1, true, 1, P - 1, CB, kThreads, true>(Q, LD, nb, tw, tid, lq,
+          fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lq,
                                                                       sp);
       }
     }
@@ -303,7 +305,7 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
         else
           Wq[d] = Wq[d] + x;
       };
-      fft_passes_io<T, N, -1, false, 0, P - 1, ML, kThreads, false>(Q, LD, nl, tw, tid, lb, sw);
+      fft_passes_io<T, N, -1, false, 0, P - 1, ML, NT, false>(Q, LD, nl, tw, tid, lb, sw);
     }
   }
 }

@@ -356,14 +356,16 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
 
 // ---- kernels ----------------------------------------------------------------
 
-template <typename T, int N, bool PR, int MODE>
-__global__ __launch_bounds__(kThreads) void k_plane_field(PlaneArgs a) {
-  plane_body<T, N, 0, PR, MODE>(a, blockIdx.x, blockIdx.y);
+// NT = 512 (with the doubled b-line tile, V = 1) puts eight waves behind one
+// workgroup's LDS buffer where only one workgroup fits a CU
+template <typename T, int N, bool PR, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_plane_field(PlaneArgs a) {
+  plane_body<T, N, NT == kThreads ? 0 : 1, PR, MODE, NT>(a, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int N, int MODE>
-__global__ __launch_bounds__(kThreads) void k_plane_split_field(PlaneArgs a) {
-  plane_split_body<T, N, MODE>(a, blockIdx.x, blockIdx.y);
+template <typename T, int N, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_plane_split_field(PlaneArgs a) {
+  plane_split_body<T, N, MODE, NT>(a, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
@@ -634,46 +636,60 @@ __global__ void k_rs_energy(const Cx<double>* __restrict__ Z, const double* __re
 
 // ---- launchers ----------------------------------------------------------------
 
+template <typename T, int N, bool PR, int MODE, int NT>
+static int launch_plane_field(Ctx& c, const PlaneArgs& a, size_t lds, int npairs) {
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_field<T, N, PR, MODE, NT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_plane_field<T, N, PR, MODE, NT>), dim3(N, npairs), dim3(NT), lds, c.stream,
+                     a);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int N, int MODE, int NT>
+static int launch_plane_split_field(Ctx& c, const PlaneArgs& a, size_t lds, int npairs) {
+  SDFT_HIP_CHECK(hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&k_plane_split_field<T, N, MODE, NT>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_plane_split_field<T, N, MODE, NT>), dim3(N, npairs), dim3(NT), lds,
+                     c.stream, a);
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// The plane stage's launch form, by waves per CU (LDS-limited): the whole
+// plane with 256 threads, or with 512 threads and the doubled b-line tile,
+// or -- where it at least doubles the waves, or where nothing else fits --
+// the split form (column halves, c-transforms recomputed per half) with 256
+// or 512 threads.
 template <typename T, int N, int MODE>
 static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
   const Geometry& g = c.g;
   PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
   a.field = F;
-  const dim3 grid(N, npairs);
-  const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
-  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
-  // the split form recomputes the c-transforms per column half; it pays where
-  // its smaller LDS buffer fits at least twice as many workgroups per CU
-  // (e.g. FP32 at 192^3: 117 KB -> 66 KB, one workgroup per CU -> two)
   constexpr size_t kLds = 160 * 1024;
-  if (lds > kLds && lds2 > kLds) {
+  auto waves = [&](size_t lds, int nt) { return lds <= kLds ? (int)(kLds / lds) * (nt / 64) : 0; };
+  const size_t l0 = plane2_lds_bytes<T, N, 0>(g.M), l1 = plane2_lds_bytes<T, N, 1>(g.M);
+  const size_t s0 = plane_split_lds_bytes<T, N, 2, kThreads>(g.M);
+  const size_t s1 = plane_split_lds_bytes<T, N, 2, 512>(g.M);
+  const int w0 = waves(l0, kThreads), w1 = waves(l1, 512);
+  const int ws0 = waves(s0, kThreads), ws1 = waves(s1, 512);
+  const int wun = std::max(w0, w1), wsp = std::max(ws0, ws1);
+  const bool pr = plane_prunable(g.K, N);
+  if (wun > 0 && wsp < 2 * wun) {
+    if (w0 >= w1)
+      return pr ? launch_plane_field<T, N, true, MODE, kThreads>(c, a, l0, npairs)
+                : launch_plane_field<T, N, false, MODE, kThreads>(c, a, l0, npairs);
+    return pr ? launch_plane_field<T, N, true, MODE, 512>(c, a, l1, npairs)
+              : launch_plane_field<T, N, false, MODE, 512>(c, a, l1, npairs);
+  }
+  if (wsp == 0) {
     set_error("plane buffer exceeds LDS for this grid in this precision");
     return SDFT_EINVAL_CODE;
   }
-  const bool split_wins = lds2 <= kLds && (lds > kLds || kLds / lds2 >= 2 * (kLds / lds));
-  if (!split_wins) {
-    if (plane_prunable(g.K, N)) {
-      SDFT_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void*>(&k_plane_field<T, N, true, MODE>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_plane_field<T, N, true, MODE>), grid, dim3(kThreads), lds, c.stream,
-                         a);
-    } else {
-      SDFT_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void*>(&k_plane_field<T, N, false, MODE>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_plane_field<T, N, false, MODE>), grid, dim3(kThreads), lds, c.stream,
-                         a);
-    }
-    SDFT_HIP_CHECK(hipGetLastError());
-    return 0;
-  }
   a.Wout = W;  // MODE 2 writes the result in place of its (unread) input
-  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_split_field<T, N, MODE>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-  hipLaunchKernelGGL((k_plane_split_field<T, N, MODE>), grid, dim3(kThreads), lds2, c.stream, a);
-  SDFT_HIP_CHECK(hipGetLastError());
-  return 0;
+  if (ws0 >= ws1) return launch_plane_split_field<T, N, MODE, kThreads>(c, a, s0, npairs);
+  return launch_plane_split_field<T, N, MODE, 512>(c, a, s1, npairs);
 }
 
 template <int MODE>
