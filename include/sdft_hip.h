@@ -298,15 +298,20 @@ int sdft_hip_batch_config(sdft_batch *bat, int split, int overlap, int plane_til
 /* Launch-configuration tuner for one context (no reference counterpart):
  * times `steps` Chebyshev filter steps of an nstate-orbital block (device
  * noise, zero coefficients) in precision prec for each candidate --
- * plane-kernel b-line tile (0 base sub-chunk, 1 twice as wide, where it fits
- * LDS) x KB projection workgroup target (256, 1024) x fused expand (0: an
- * expand kernel per step, 1: the contract kernel writes the next step's
- * a-expansion; offered only when every sphere column is one ka interval) --
- * and keeps the fastest for this context.  config_out (nullable, 3 ints):
- * tile, workgroup target, fused expand; cand_ms (nullable): per-candidate ms
- * per filter step of the block.  Returns the number of candidates (8, or 4
- * without the fused form).  sdft_hip_set_config sets the three knobs by
- * hand. */
+ * plane-kernel variant x KB projection workgroup target (256, 1024) x fused
+ * expand (0: an expand kernel per step, 1: the contract kernel writes the
+ * next step's a-expansion; offered only when every sphere column is one ka
+ * interval) -- and keeps the fastest for this context.  Plane variants: 0
+ * the base b-line tile, 1 twice as wide (where it fits LDS); at n = 64 also
+ * 2 radix-4 b-lines (4x4x4 instead of 8x8) on the base tile, 3-5 512-thread
+ * workgroups with radix-4 c-lines (3: wide tile, radix-8 b-lines; 4: wide
+ * tile, radix-4 b-lines; 5: base tile, radix-4 b-lines).  config_out
+ * (nullable, 3 ints): variant, workgroup target, fused expand; cand_ms
+ * (nullable): per-candidate ms per filter step of the block, loop order
+ * variant, target, fused expand.  Returns the number of candidates (2 x 2 x
+ * 2 per variant pair: 24 at n = 64, 8 elsewhere; half without the fused
+ * form).  sdft_hip_set_config sets the three knobs by hand (variant in
+ * [0, 6) at n = 64, [0, 2) elsewhere). */
 int sdft_hip_tune(sdft_ctx *ctx, int nstate, int prec, int steps, int *config_out, int max_cand,
                   double *cand_ms);
 int sdft_hip_set_config(sdft_ctx *ctx, int plane_tile, int kb_wg_target, int fused_expand);

@@ -10,16 +10,19 @@
 // transform; neither is normalised.
 //
 // Radix schedule: powers of two in radix-8 passes (one or two radix-4 passes
-// for the remainder), then radix-3 passes.  Supported N = 2^a 3^b.
+// for the remainder) or, by choice, in radix-4 passes; then radix-3 passes.
+// Supported N = 2^a 3^b.
 #pragma once
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SDFT_HD __host__ __device__
 #define SDFT_DEV __device__
+#define SDFT_DEVI __device__ __forceinline__
 #else
 #define SDFT_HD
 #define SDFT_DEV
+#define SDFT_DEVI inline
 #endif
 
 namespace sdft {
@@ -59,7 +62,24 @@ SDFT_HD inline Cx<T> mul_di(Cx<T> a) {
   return DIR > 0 ? cx(-a.y, a.x) : cx(a.y, -a.x);
 }
 
+// Index products below 2^24 (line offsets, twiddle indices) through the
+// full-rate 24-bit multiply; a 32-bit v_mul_lo_u32 issues at a quarter of the
+// VALU rate.
+SDFT_HD inline int mul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return static_cast<int>(__umul24(static_cast<unsigned>(a), static_cast<unsigned>(b)));
+#else
+  return a * b;
+#endif
+}
+
 // ---- radix schedule ------------------------------------------------------
+// Schedule S = 0 takes the powers of two in radix-8 passes (one or two
+// radix-4 passes for the remainder); S = 1 in radix-4 passes (one radix-8
+// pass for an odd power).  The two differ where 2^6 divides N (64, 192):
+// 8 x 8 against 4 x 4 x 4.  Radix-4 passes have twice the butterflies per
+// line, so a workgroup can keep every thread busy on half as many lines
+// (half the LDS for the lines in flight).  Radix-3 passes come last.
 constexpr int exp2_of(int n) { int e = 0; while (n % 2 == 0) { n /= 2; ++e; } return e; }
 constexpr int exp3_of(int n) { int e = 0; while (n % 3 == 0) { n /= 3; ++e; } return e; }
 constexpr bool fft_size_ok(int n) {
@@ -68,18 +88,23 @@ constexpr bool fft_size_ok(int n) {
   while (m % 3 == 0) m /= 3;
   return m == 1 && n > 1;
 }
-constexpr int n_pow2_passes(int N) {
+constexpr int n_pow2_passes(int N, int S = 0) {
   int e = exp2_of(N);
   if (e == 0) return 0;
+  if (S == 1) return e == 1 ? 1 : (e % 2 == 0 ? e / 2 : (e - 3) / 2 + 1);
   if (e % 3 == 0) return e / 3;
   if (e % 3 == 1) return e == 1 ? 1 : (e - 4) / 3 + 2;
   return e / 3 + 1;
 }
-constexpr int n_passes(int N) { return n_pow2_passes(N) + exp3_of(N); }
-constexpr int radix_at(int N, int i) {
+constexpr int n_passes(int N, int S = 0) { return n_pow2_passes(N, S) + exp3_of(N); }
+constexpr int radix_at(int N, int i, int S = 0) {
   int e = exp2_of(N);
-  int p2 = n_pow2_passes(N);
+  int p2 = n_pow2_passes(N, S);
   if (i >= p2) return 3;
+  if (S == 1) {
+    if (e == 1) return 2;
+    return (e % 2 == 1 && i == 0) ? 8 : 4;
+  }
   if (e % 3 == 0) return 8;
   if (e % 3 == 1) {
     if (e == 1) return 2;
@@ -87,20 +112,20 @@ constexpr int radix_at(int N, int i) {
   }
   return i < e / 3 ? 8 : 4;
 }
-constexpr int ns_before(int N, int i) {
+constexpr int ns_before(int N, int i, int S = 0) {
   int ns = 1;
-  for (int k = 0; k < i; ++k) ns *= radix_at(N, k);
+  for (int k = 0; k < i; ++k) ns *= radix_at(N, k, S);
   return ns;
 }
-constexpr int min_radix(int N) {
+constexpr int min_radix(int N, int S = 0) {
   int r = 8;
-  for (int i = 0; i < n_passes(N); ++i) r = radix_at(N, i) < r ? radix_at(N, i) : r;
+  for (int i = 0; i < n_passes(N, S); ++i) r = radix_at(N, i, S) < r ? radix_at(N, i, S) : r;
   return r;
 }
 // Lines a workgroup of NT threads transforms per call: enough that the
 // smallest-radix pass gives each thread at most two butterflies.
-constexpr int fft_max_lines(int N, int NT) {
-  int m = 2 * NT * min_radix(N) / N;
+constexpr int fft_max_lines(int N, int NT, int S = 0) {
+  int m = 2 * NT * min_radix(N, S) / N;
   return m < 1 ? 1 : m;
 }
 
@@ -205,7 +230,7 @@ constexpr bool mid_index(int r) {
 
 // tw[m] = exp(+2 pi i m / N), m < N (the table for the line length).
 template <typename T, int N, int DIR, int PASS, int MAXLINES, int NT>
-SDFT_DEV inline void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+SDFT_DEVI void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
   constexpr int R = radix_at(N, PASS);
   constexpr int NB = N / R;
   constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
@@ -235,7 +260,7 @@ SDFT_DEV inline void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, i
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const Cx<T> w = tw[r * k * STEP];
+          const Cx<T> w = tw[mul24(r, k * STEP)];
           v[b][r] = DIR > 0 ? cmul(v[b][r], w) : cmulc(v[b][r], w);
         }
       }
@@ -251,7 +276,7 @@ SDFT_DEV inline void fft_pass(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, i
 // All passes of a length-N transform over nlines <= MAXLINES lines (the
 // pass count is a compile-time constant; the if-constexpr chain unrolls it).
 template <typename T, int N, int DIR, int MAXLINES, int NT>
-SDFT_DEV inline void fft_lines(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+SDFT_DEVI void fft_lines(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
   static_assert(fft_size_ok(N), "FFT length must be 2^a 3^b");
   constexpr int P = n_passes(N);
   static_assert(P >= 1 && P <= 6, "unsupported pass count");
@@ -268,55 +293,83 @@ SDFT_DEV inline void fft_lines(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, 
 // transform in reversed order can start where an inverse transform in
 // standard order ends: the inverse's last pass writes positions j + r N/R,
 // exactly the inputs of the reversed forward's first butterfly j.
-constexpr int radix_sched(int N, int i, bool rev) {
-  return rev ? radix_at(N, n_passes(N) - 1 - i) : radix_at(N, i);
+constexpr int radix_sched(int N, int i, bool rev, int S = 0) {
+  return rev ? radix_at(N, n_passes(N, S) - 1 - i, S) : radix_at(N, i, S);
 }
-constexpr int ns_sched(int N, int i, bool rev) {
+constexpr int ns_sched(int N, int i, bool rev, int S = 0) {
   int ns = 1;
-  for (int k = 0; k < i; ++k) ns *= radix_sched(N, k, rev);
+  for (int k = 0; k < i; ++k) ns *= radix_sched(N, k, rev, S);
   return ns;
+}
+
+// Thread slot idx -> (line, butterfly j) of a pass over nlines lines.  For a
+// line count known at compile time the division folds; for a runtime count
+// it goes through a single-precision reciprocal instead of an integer
+// division: (idx + 1/2) / nlines lies at least 1/(2 nlines) from an integer,
+// and for idx < 2^16 the rounding of v_rcp_f32 (1 ulp) and of the product
+// stays below 2^-6 / nlines, so the truncation is the exact quotient.
+SDFT_DEVI float slot_rcp(int nlines) {
+  return __builtin_amdgcn_rcpf(static_cast<float>(nlines));
+}
+template <bool LINE_FAST, int NB>
+SDFT_DEVI void slot_map(int idx, int nlines, float rn, int& line, int& j) {
+  if constexpr (LINE_FAST) {
+    if (__builtin_constant_p(nlines)) {
+      line = idx % nlines;
+      j = idx / nlines;
+    } else {
+      j = static_cast<int>((static_cast<float>(idx) + 0.5f) * rn);
+      line = idx - mul24(j, nlines);
+    }
+  } else {
+    line = idx / NB;
+    j = idx - line * NB;
+  }
 }
 
 // One Stockham pass over nlines <= MAXLINES lines.  Inputs come from
 // lf(line, pos), outputs go to sf(line, pos, value); a barrier separates all
-// loads from all stores, and one follows the stores, so lf and sf may address
-// the same LDS buffer.  LINE_FAST maps consecutive threads to consecutive
-// lines (same butterfly), otherwise to consecutive butterflies of a line.
+// loads from all stores (SYNC_MID; needed only when lf and sf address the
+// same LDS buffer), and one follows the stores.  LINE_FAST maps consecutive
+// threads to consecutive lines (same butterfly), otherwise to consecutive
+// butterflies of a line.
 template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
-          bool ZIN = false, bool ZOUT = false, class LF, class SF>
-SDFT_DEV inline void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
-  constexpr int R = radix_sched(N, PASS, REV);
+          bool ZIN = false, bool ZOUT = false, int S = 0, bool SYNC_MID = true, class LF,
+          class SF>
+SDFT_DEVI void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
+  constexpr int R = radix_sched(N, PASS, REV, S);
   constexpr int NB = N / R;
   constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
-  constexpr int NS = ns_sched(N, PASS, REV);
+  constexpr int NS = ns_sched(N, PASS, REV, S);
   constexpr int STEP = N / (NS * R);
   static_assert(!ZIN || (NS == 1 && R % 4 == 0 && (R == 4 || R == 8)), "ZIN: first pass, radix 4/8");
   static_assert(!ZOUT || (NS * R == N && R % 4 == 0), "ZOUT: last pass, radix 4/8");
   const int total = nlines * NB;
+  const float rn = slot_rcp(nlines);
   Cx<T> v[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
     if (idx < total) {
-      const int line = LINE_FAST ? idx % nlines : idx / NB;
-      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      int line, j;
+      slot_map<LINE_FAST, NB>(idx, nlines, rn, line, j);
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (!(ZIN && mid_index<R>(r))) v[b][r] = lf(line, j + r * NB);
     }
   }
-  __syncthreads();
+  if constexpr (SYNC_MID) __syncthreads();
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
     if (idx < total) {
-      const int line = LINE_FAST ? idx % nlines : idx / NB;
-      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      int line, j;
+      slot_map<LINE_FAST, NB>(idx, nlines, rn, line, j);
       const int k = j % NS;
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const Cx<T> w = tw[r * k * STEP];
+          const Cx<T> w = tw[mul24(r, k * STEP)];
           v[b][r] = DIR > 0 ? cmul(v[b][r], w) : cmulc(v[b][r], w);
         }
       }
@@ -348,47 +401,55 @@ SDFT_HD inline int lds_slot(int pos) {
 // Passes FIRST..LAST (inclusive) of one transform: the first loads through
 // lf0, the last stores through sf1, the ones between go through the LDS lines
 // buf + line * ld (padded slots with PAD8).  ZIN / ZOUT prune the first /
-// last pass (see fft_pass_io).
+// last pass (see fft_pass_io).  SEP_IN: lf0 reads memory other than the lines
+// (global memory, another LDS buffer), so the first pass stores without a
+// barrier after its loads -- the caller guarantees that the lines are no
+// longer read (a barrier since their last use); SEP_OUT: sf1 writes memory
+// other than the lines, so the last pass does the same.
 template <typename T, int N, int DIR, bool REV, int FIRST, int LAST, int MAXLINES, int NT,
-          bool LINE_FAST, bool ZIN = false, bool ZOUT = false, bool PAD8 = false, class LF,
-          class SF>
-SDFT_DEV inline void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* tw, int tid,
+          bool LINE_FAST, bool ZIN = false, bool ZOUT = false, bool PAD8 = false, int S = 0,
+          bool SEP_IN = false, bool SEP_OUT = false, class LF, class SF>
+SDFT_DEVI void fft_passes_io(Cx<T>* buf, int ld, int nlines, const Cx<T>* tw, int tid,
                                    LF& lf0, SF& sf1) {
-  auto lb = [&](int line, int pos) { return buf[line * ld + lds_slot<PAD8>(pos)]; };
-  auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + lds_slot<PAD8>(pos)] = x; };
+  auto lb = [&](int line, int pos) { return buf[mul24(line, ld) + lds_slot<PAD8>(pos)]; };
+  auto sb = [&](int line, int pos, Cx<T> x) { buf[mul24(line, ld) + lds_slot<PAD8>(pos)] = x; };
   if constexpr (FIRST == LAST) {
-    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, ZOUT>(nlines, tw, tid, lf0,
-                                                                          sf1);
+    // one pass from lf0 to sf1: they cannot alias when exactly one of them
+    // addresses the lines
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, ZOUT, S,
+                !(SEP_IN != SEP_OUT)>(nlines, tw, tid, lf0, sf1);
   } else {
-    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, false>(nlines, tw, tid, lf0,
-                                                                           sb);
+    fft_pass_io<T, N, DIR, FIRST, REV, MAXLINES, NT, LINE_FAST, ZIN, false, S, !SEP_IN>(
+        nlines, tw, tid, lf0, sb);
     if constexpr (FIRST + 1 < LAST) {
       fft_passes_io<T, N, DIR, REV, FIRST + 1, LAST - 1, MAXLINES, NT, LINE_FAST, false, false,
-                    PAD8>(buf, ld, nlines, tw, tid, lb, sb);
+                    PAD8, S>(buf, ld, nlines, tw, tid, lb, sb);
     }
-    fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST, false, ZOUT>(nlines, tw, tid, lb,
-                                                                           sf1);
+    fft_pass_io<T, N, DIR, LAST, REV, MAXLINES, NT, LINE_FAST, false, ZOUT, S, !SEP_OUT>(
+        nlines, tw, tid, lb, sf1);
   }
 }
 
 // Last pass of an inverse transform (standard order), a real multiplier
 // vf(line, pos) at every output position, and the first pass of the forward
 // transform in reversed order, in registers.  Outputs land at j*R + r.
-template <typename T, int N, int MAXLINES, int NT, bool LINE_FAST, class LF, class VF, class SF>
-SDFT_DEV inline void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& vf,
+template <typename T, int N, int MAXLINES, int NT, bool LINE_FAST, int S = 0, class LF, class VF,
+          class SF>
+SDFT_DEVI void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& vf,
                                     SF& sf) {
-  constexpr int P = n_passes(N);
-  constexpr int R = radix_at(N, P - 1);
+  constexpr int P = n_passes(N, S);
+  constexpr int R = radix_at(N, P - 1, S);
   constexpr int NB = N / R;
   constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
   const int total = nlines * NB;
+  const float rn = slot_rcp(nlines);
   Cx<T> v[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
     if (idx < total) {
-      const int line = LINE_FAST ? idx % nlines : idx / NB;
-      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      int line, j;
+      slot_map<LINE_FAST, NB>(idx, nlines, rn, line, j);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = lf(line, j + r * NB);
     }
@@ -398,11 +459,11 @@ SDFT_DEV inline void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
     if (idx < total) {
-      const int line = LINE_FAST ? idx % nlines : idx / NB;
-      const int j = LINE_FAST ? idx / nlines : idx - line * NB;
+      int line, j;
+      slot_map<LINE_FAST, NB>(idx, nlines, rn, line, j);
       if (P > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * j]);
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[mul24(r, j)]);
       }
       Dft<T, R, +1>::run(v[b]);
 #pragma unroll
@@ -421,13 +482,13 @@ SDFT_DEV inline void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf
 // fft_lines with a choice of thread mapping (LINE_FAST: consecutive threads
 // on consecutive lines; otherwise on consecutive butterflies of one line).
 template <typename T, int N, int DIR, int MAXLINES, int NT, bool LINE_FAST>
-SDFT_DEV inline void fft_lines_map(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+SDFT_DEVI void fft_lines_map(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
   if constexpr (!LINE_FAST) {
     fft_lines<T, N, DIR, MAXLINES, NT>(buf, nlines, ld, tw, tid);
   } else {
     constexpr int P = n_passes(N);
-    auto lb = [&](int line, int pos) { return buf[line * ld + pos]; };
-    auto sb = [&](int line, int pos, Cx<T> x) { buf[line * ld + pos] = x; };
+    auto lb = [&](int line, int pos) { return buf[mul24(line, ld) + pos]; };
+    auto sb = [&](int line, int pos, Cx<T> x) { buf[mul24(line, ld) + pos] = x; };
     fft_passes_io<T, N, DIR, false, 0, P - 1, MAXLINES, NT, true>(buf, ld, nlines, tw, tid, lb,
                                                                   sb);
   }
@@ -435,7 +496,7 @@ SDFT_DEV inline void fft_lines_map(Cx<T>* buf, int nlines, int ld, const Cx<T>* 
 
 // Any number of lines: batches of MAXLINES.
 template <typename T, int N, int DIR, int MAXLINES, int NT>
-SDFT_DEV inline void fft_lines_batched(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
+SDFT_DEVI void fft_lines_batched(Cx<T>* buf, int nlines, int ld, const Cx<T>* tw, int tid) {
   for (int l0 = 0; l0 < nlines; l0 += MAXLINES) {
     const int nl = nlines - l0 < MAXLINES ? nlines - l0 : MAXLINES;
     fft_lines<T, N, DIR, MAXLINES, NT>(buf + l0 * ld, nl, ld, tw, tid);

@@ -1073,7 +1073,7 @@ int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out,
     set_error("hipEventCreate failed");
     rc = SDFT_EHIP;
   }
-  // candidates: plane tile x KB workgroup target x fused expand (the last
+  // candidates: plane variant x KB workgroup target x fused expand (the last
   // only where the columns allow it), each timed on filter steps k >= 3 of
   // a loop as block_filter_impl runs it (zero coefficients: the accumulator
   // is left as it is; a tiny scale keeps T_k bounded)
@@ -1095,7 +1095,8 @@ int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out,
     sp.step = k;
     return sp;
   };
-  for (int pv = 0; pv < 2 && !rc; ++pv)
+  const int npv = plane_variant_count(c.g.n);
+  for (int pv = 0; pv < npv && !rc; ++pv)
     for (int ti = 0; ti < 2 && !rc; ++ti)
       for (int fu = 0; fu < nfuse && !rc; ++fu) {
         c.plane_variant = pv;
@@ -1153,7 +1154,7 @@ int sdft_hip_tune(sdft_ctx* h, int nstate, int prec, int steps, int* config_out,
 }
 
 int sdft_hip_set_config(sdft_ctx* h, int plane_tile, int kb_wg_target, int fused_expand) {
-  if (!h || (plane_tile != 0 && plane_tile != 1) || kb_wg_target < 1 ||
+  if (!h || plane_tile < 0 || plane_tile >= plane_variant_count(h->c.g.n) || kb_wg_target < 1 ||
       (fused_expand != 0 && fused_expand != 1)) {
     set_error("bad configuration");
     return SDFT_EINVAL;

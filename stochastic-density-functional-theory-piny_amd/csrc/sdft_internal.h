@@ -130,7 +130,7 @@ struct Ctx {
   size_t red_bytes = 0;
   // launch configuration (the tuner's pick): plane-kernel tile variant (0
   // base sub-chunk, 1 double) and the KB projection's target workgroup count
-  int plane_variant = 0;
+  int plane_variant = 0;  // plane-kernel variant, 0 .. plane_variant_count(n) - 1
   int kb_wg_target = 512;
   // filter loops let the contract kernel write the next step's a-expansion
   // (when the columns allow it, Geometry::col_intervals); the tuner decides
@@ -334,6 +334,9 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W);
 // second buffer W2 when the plane runs in the split form (M x (n+1) plane
 // buffer above the LDS size)
 int launch_plane(Ctx& c, int prec, void* W, int npairs, void** Wout);
+// Plane-kernel variants offered at grid size n (b-line tile, threads per
+// workgroup, radix schedules): 6 at n = 64, 2 elsewhere.
+inline int plane_variant_count(int n) { return n == 64 ? 6 : 2; }
 int launch_contract(Ctx& c, int prec, int mode, const void* W, int npairs,
                     const double* Xcur, double* Xout, double* acc, const void* fnl,
                     const double* chi, const StepParams& sp, double* mom_part,

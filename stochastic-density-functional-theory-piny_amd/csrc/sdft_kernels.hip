@@ -138,9 +138,9 @@ __global__ __launch_bounds__(kThreads) void k_expand_batch(const ExpandArgs* __r
 }
 
 
-template <typename T, int N, int V, bool PR>
-__global__ __launch_bounds__(kThreads) void k_plane(PlaneArgs a) {
-  plane_body<T, N, V, PR>(a, blockIdx.x, blockIdx.y);
+template <typename T, int N, int V, bool PR, int NT = kThreads, int SC = 0, int SB = 0>
+__global__ __launch_bounds__(NT) void k_plane(PlaneArgs a) {
+  plane_body<T, N, V, PR, 0, NT, SC, SB>(a, blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int N, int V, bool PR>
@@ -815,13 +815,44 @@ int launch_expand(Ctx& c, int prec, const double* X, int npairs, void* W) {
   return 0;
 }
 
-template <typename T, int N, int V, bool PR>
+template <typename T, int N, int V, bool PR, int NT = kThreads, int SC = 0, int SB = 0>
 static int plane_launch(Ctx& c, const PlaneArgs& a, size_t lds, int npairs) {
-  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, V, PR>),
+  auto kern = &k_plane<T, N, V, PR, NT, SC, SB>;
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_plane<T, N, V, PR>), dim3(N, npairs), dim3(kThreads), lds, c.stream, a);
+  hipLaunchKernelGGL(kern, dim3(N, npairs), dim3(NT), lds, c.stream, a);
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+// Plane-kernel variants (the tuner's plane tile): b-line tile V, threads per
+// workgroup NT and the radix schedules of the c- and b-lines (SC, SB).  At
+// n = 64 every thread of a pass has a butterfly when (lines x butterflies
+// per line) reaches NT: with radix-4 b-lines (SB = 1) a 256-thread FP64
+// workgroup fills its threads from a 16-line b tile, and 512-thread ones keep
+// the plane's LDS buffer for twice the waves.
+//   0: V 0, 256, 8x8 / 8x8     1: V 1, 256, 8x8 / 8x8   2: V 0, 256, 8x8 / 4x4x4
+//   3: V 1, 512, 4x4x4 / 8x8   4: V 1, 512, 4x4x4 / 4x4x4
+//   5: V 0, 512, 4x4x4 / 4x4x4
+// Variants 2-5 differ from 0 / 1 only where 2^6 divides n and are built for
+// n = 64; other sizes take variant (pv & 1).
+template <typename T, int N, bool PR>
+static int plane_variant_launch(Ctx& c, const PlaneArgs& a, int pv, int npairs) {
+  const int M = c.g.M;
+  const size_t lds0 = plane2_lds_bytes<T, N, 0>(M), lds1 = plane2_lds_bytes<T, N, 1>(M);
+  if constexpr (N == 64) {
+    switch (pv) {
+      case 2: return plane_launch<T, N, 0, PR, kThreads, 0, 1>(c, a, lds0, npairs);
+      case 3: if (lds1 <= 160 * 1024) return plane_launch<T, N, 1, PR, 512, 1, 0>(c, a, lds1, npairs);
+              break;
+      case 4: if (lds1 <= 160 * 1024) return plane_launch<T, N, 1, PR, 512, 1, 1>(c, a, lds1, npairs);
+              break;
+      case 5: return plane_launch<T, N, 0, PR, 512, 1, 1>(c, a, lds0, npairs);
+      default: break;
+    }
+  }
+  if ((pv & 1) && lds1 <= 160 * 1024) return plane_launch<T, N, 1, PR>(c, a, lds1, npairs);
+  return plane_launch<T, N, 0, PR>(c, a, lds0, npairs);
 }
 
 template <typename T, int N>
@@ -830,17 +861,11 @@ static int plane_impl(Ctx& c, void* W, int npairs, void** Wout) {
   PlaneArgs a = make_plane_args(c, prec_of<T>(), W);
   dim3 grid(N, npairs);
   const bool pr = plane_prunable(g.K, N);
-  const size_t lds1 = plane2_lds_bytes<T, N, 1>(g.M);
-  if (c.plane_variant == 1 && lds1 <= 160 * 1024) {
-    *Wout = W;
-    return pr ? plane_launch<T, N, 1, true>(c, a, lds1, npairs)
-              : plane_launch<T, N, 1, false>(c, a, lds1, npairs);
-  }
   const size_t lds = plane2_lds_bytes<T, N, 0>(g.M);
   if (lds <= 160 * 1024) {
     *Wout = W;
-    return pr ? plane_launch<T, N, 0, true>(c, a, lds, npairs)
-              : plane_launch<T, N, 0, false>(c, a, lds, npairs);
+    return pr ? plane_variant_launch<T, N, true>(c, a, c.plane_variant, npairs)
+              : plane_variant_launch<T, N, false>(c, a, c.plane_variant, npairs);
   }
   const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
   if (lds2 > 160 * 1024) {

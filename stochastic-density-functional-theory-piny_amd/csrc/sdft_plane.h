@@ -70,13 +70,18 @@ struct PlaneCfg {
   static constexpr int CB = V == 0 ? CB0 : largest_divisor_le(N, 2 * CB0 < N ? 2 * CB0 : N);
 };
 
-// The c rows of the plane buffer in float at n = 64 use the padded slot
-// layout (lds_slot<true>, row stride 72): their first passes run line-major
-// (for coalesced W gathers) and store with a stride of 8 elements.
+// The c rows of the plane buffer at n = 64 use the padded slot layout
+// (lds_slot<true>, row stride 72): their passes run line-major (8 threads on
+// the butterflies of one row, so the W gathers and scatters of a wave touch a
+// few rows' consecutive sphere columns) and the radix-8 first pass stores
+// with a stride of 8 elements, which the padding spreads over distinct banks
+// for 8- and 16-byte elements alike.  Elsewhere the rows are unpadded and the
+// FP64 passes run line-fast.
 template <typename T, int N>
 struct PlaneRows {
-  static constexpr bool PAD = sizeof(T) == 4 && N == 64;
+  static constexpr bool PAD = N == 64;
   static constexpr int LDC = PAD ? N + N / 8 : Cfg<N>::LD;
+  static constexpr bool LINE_FAST = PAD ? false : kLineFast<T>;
 };
 
 template <typename T, int N, int V = 0>
@@ -90,12 +95,14 @@ size_t plane2_lds_bytes(int M) {
 // grid a.field[q][ra][rb][rc] (FFTW order).  MODE 2 starts there: the forward
 // b-lines are read from a.field (which already holds v.psi plus the real-space
 // KB term) and the stage continues as MODE 0.
-template <typename T, int N, int V, bool PR, int MODE = 0, int NT = kThreads>
+// SC / SB: the radix schedules (fft_lds.h) of the c-lines and the b-lines.
+template <typename T, int N, int V, bool PR, int MODE = 0, int NT = kThreads, int SC = 0,
+          int SB = 0>
 __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = fft_max_lines(N, NT), CB = PlaneCfg<T, N, V>::CB;
+  constexpr int LD = Cfg<N>::LD, ML = fft_max_lines(N, NT, SC), CB = PlaneCfg<T, N, V>::CB;
   constexpr bool PAD = PlaneRows<T, N>::PAD;
   constexpr int LDC = PlaneRows<T, N>::LDC;
-  constexpr int P = n_passes(N);
+  constexpr int PC = n_passes(N, SC), P = n_passes(N, SB);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* __restrict__ W = reinterpret_cast<Cx<T>*>(a.W);
   const int ncolp = a.ncolp, M = a.M;
@@ -131,8 +138,10 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
     };
     Cx<T>* buf = Pl + r0 * LDC;
     auto sb = [&](int l, int pos, Cx<T> x) { buf[l * LDC + lds_slot<PAD>(pos)] = x; };
-    fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, kLineFast<T>, PR, false, PAD>(
-        buf, LDC, nl, tw, tid, lw, sb);
+    // the rows are not read before this pass (W in, rows out: no barrier
+    // between the first pass's loads and stores)
+    fft_passes_io<T, N, +1, false, 0, PC - 1, ML, NT, PlaneRows<T, N>::LINE_FAST, PR, false,
+                  PAD, SC, true, false>(buf, LDC, nl, tw, tid, lw, sb);
   }
 
   // along b, CB columns rc at a time
@@ -155,20 +164,24 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       // inverse b-lines straight to the real-space grid (consecutive threads
       // on consecutive lines: runs of CB consecutive rc)
       auto sf = [&](int l, int pos, Cx<T> x) { F[(size_t)pos * N + rc0 + l] = x; };
-      fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true, PR, false>(Q, LD, CB, tw, tid,
-                                                                              lp, sf);
+      fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true, PR, false, false, SB, true, true>(
+          Q, LD, CB, tw, tid, lp, sf);
     } else if constexpr (MODE == 2) {
       auto lf = [&](int l, int pos) { return F[(size_t)pos * N + rc0 + l]; };
-      fft_passes_io<T, N, -1, true, 0, P - 1, CB, NT, true, false, PR>(Q, LD, CB, tw, tid,
-                                                                             lf, sp);
+      fft_passes_io<T, N, -1, true, 0, P - 1, CB, NT, true, false, PR, false, SB, true, true>(
+          Q, LD, CB, tw, tid, lf, sp);
     } else {
+      // Q is free here (the previous sub-chunk's last pass read it before
+      // its closing barrier) and the plane rows are read-only until the
+      // last forward pass: the first and last passes skip their middle
+      // barrier
       if constexpr (P >= 2)
-        fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true, PR, false>(Q, LD, CB, tw,
-                                                                                tid, lp, sq);
-      fft_fused_vmul<T, N, CB, NT, true>(CB, tw, tid, lq, vf, sq);
+        fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true, PR, false, false, SB, true,
+                      false>(Q, LD, CB, tw, tid, lp, sq);
+      fft_fused_vmul<T, N, CB, NT, true, SB>(CB, tw, tid, lq, vf, sq);
       if constexpr (P >= 2)
-        fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true, false, PR>(Q, LD, CB, tw,
-                                                                               tid, lq, sp);
+        fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true, false, PR, false, SB, false,
+                      true>(Q, LD, CB, tw, tid, lq, sp);
     }
   }
 
@@ -183,8 +196,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
       const int d = rc[l * N + pos];
       if (d >= 0) Wp[d] = x;
     };
-    fft_passes_io<T, N, -1, PR, 0, P - 1, ML, NT, kLineFast<T>, false, PR, PAD>(
-        buf, LDC, nl, tw, tid, lb, sw);
+    fft_passes_io<T, N, -1, PR, 0, PC - 1, ML, NT, PlaneRows<T, N>::LINE_FAST, false, PR,
+                  PAD, SC, false, true>(buf, LDC, nl, tw, tid, lb, sw);
   }
 }
 

@@ -238,12 +238,12 @@ class Context:
         """sdft_hip_tune: returns ((plane_tile, kb_wg_target, fused_expand),
         per-candidate ms per filter step)."""
         cfg = np.zeros(3, dtype=np.int32)
-        ms = np.zeros(16)
-        n = self.lib.sdft_hip_tune(self.h, nstate, prec, steps, cfg.ctypes.data_as(_ip), 16,
+        ms = np.zeros(64)
+        n = self.lib.sdft_hip_tune(self.h, nstate, prec, steps, cfg.ctypes.data_as(_ip), 64,
                                    _ptr(ms))
         if n < 0:
             _check(n)
-        return (int(cfg[0]), int(cfg[1]), int(cfg[2])), ms[:n].copy()
+        return (int(cfg[0]), int(cfg[1]), int(cfg[2])), ms[:min(n, 64)].copy()
 
     def ks_init(self, dens):
         """sdft_hip_ks_init: the density plane-wave sphere (systems.DensitySphere)."""

@@ -399,23 +399,29 @@ def si64_system():
 
 
 def test_plane_tiles_and_tuner_si64(hip, si64_system):
-    """Both plane-kernel b-line tiles (sdft_hip_set_config) at 64^3 against
-    the oracle (FP64 1e-12, FP32 2e-5), and the tuner returns one of the
-    candidates."""
+    """Every plane-kernel variant at 64^3 (sdft_hip_set_config: b-line tile,
+    256 / 512 threads, radix-8 / radix-4 schedules of the c- and b-lines)
+    against the oracle (FP64 1e-12, FP32 2e-5), with an odd orbital count,
+    and the tuner returns one of its 24 candidates."""
     from sdft_amd import systems
-    say("case: plane tiles 0/1 + tuner si64")
+    say("case: plane variants 0-5 + tuner si64")
     ctx = hip.Context(si64_system)
-    C = systems.noise_orbitals(si64_system, 4, seed=23)
-    ref = O.hpsi_raw(C, si64_system)
-    for tile in (0, 1):
-        for kt in (256, 2048):
-            ctx.set_config(tile, kt)
-            assert rel(ctx.hpsi_raw(C), ref) < 1e-12, (tile, kt)
-            assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5, (tile, kt)
+    for nst, seed in ((4, 23), (3, 29)):
+        C = systems.noise_orbitals(si64_system, nst, seed=seed)
+        ref = O.hpsi_raw(C, si64_system)
+        for tile in range(6):
+            for kt in (256, 2048):
+                ctx.set_config(tile, kt)
+                e64 = rel(ctx.hpsi_raw(C), ref)
+                e32 = rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref)
+                assert e64 < 1e-12 and e32 < 2e-5, (nst, tile, kt, e64, e32)
+    with pytest.raises(RuntimeError):
+        ctx.set_config(6, 256)
     (tile, kt, fu), ms = ctx.tune(8, hip.PREC_FP32, steps=2)
-    assert tile in (0, 1) and kt in (256, 1024) and fu in (0, 1)
-    assert len(ms) == 8 and np.all(ms > 0)
-    assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), ref) < 2e-5
+    assert tile in range(6) and kt in (256, 1024) and fu in (0, 1)
+    assert len(ms) == 24 and np.all(ms > 0)
+    C = systems.noise_orbitals(si64_system, 4, seed=23)
+    assert rel(ctx.hpsi_raw(C, hip.PREC_FP32), O.hpsi_raw(C, si64_system)) < 2e-5
 
 
 @pytest.fixture(scope="module")
