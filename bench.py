@@ -273,12 +273,11 @@ def main():
     if dom_name == "plane_bc_vpsi":
         gbs = bytes_ / avg_s / 1e9
         tfl = flops / avg_s / 1e12
+        kname = plane_kernel_name("float", wl.n, tuned["fp32"]["plane_tile"], pruned)
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic("k_plane<float, %d, %d, %s>" % (
-                        wl.n, tuned["fp32"]["plane_tile"], pruned), wl),
-                    "kernel": "k_plane<float,%d,%d,%s> (b/c transforms + v.psi)" % (
-                        wl.n, tuned["fp32"]["plane_tile"], pruned),
+                    "traffic": measured_traffic(kname, wl),
+                    "kernel": kname + " (b/c transforms + v.psi)",
                     "avg_launch_ms": round(avg_s * 1e3, 4),
                     "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
                     "achieved_tflops": round(tfl, 2),
@@ -660,6 +659,19 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
                               "density_l1_per_electron_max": float(max(dl1))},
         "mu": mu,
     }
+
+
+# plane-kernel variants (b-line tile V, threads per workgroup, radix
+# schedules of the c- and b-lines), as the library's tuner numbers them; the
+# variants above 1 exist at n = 64 only
+PLANE_VARIANTS = {0: (0, 256, 0, 0), 1: (1, 256, 0, 0), 2: (0, 256, 0, 1), 3: (1, 512, 1, 0),
+                  4: (1, 512, 1, 1), 5: (0, 512, 1, 1)}
+
+
+def plane_kernel_name(t, n, variant, pruned):
+    """The plane kernel's instantiation as rocprofv3 names it."""
+    v, nt, sc, sb = PLANE_VARIANTS[variant if n == 64 else variant & 1]
+    return "k_plane<%s, %d, %d, %s, %d, %d, %d>" % (t, n, v, pruned, nt, sc, sb)
 
 
 def measured_traffic(kernel, wl):
