@@ -228,15 +228,24 @@ __device__ __forceinline__ void kb_force_block(const T* __restrict__ Y, const T*
         for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::run(a[i][t], b[j][t], acc[i][j]);
   };
   const int nsteps = ppad / KS;
-  int st = 0;
-  load_raw(0, ra0, rb0);
-  while (st < nsteps) {
-    if (st + 1 < nsteps) load_raw(st + 1, ra1, rb1);
-    mma_raw(ra0, rb0);
-    if (++st >= nsteps) break;
-    if (st + 1 < nsteps) load_raw(st + 1, ra0, rb0);
-    mma_raw(ra1, rb1);
-    ++st;
+  if constexpr (sizeof(T) == 4) {
+    int st = 0;
+    load_raw(0, ra0, rb0);
+    while (st < nsteps) {
+      if (st + 1 < nsteps) load_raw(st + 1, ra1, rb1);
+      mma_raw(ra0, rb0);
+      if (++st >= nsteps) break;
+      if (st + 1 < nsteps) load_raw(st + 1, ra0, rb0);
+      mma_raw(ra1, rb1);
+      ++st;
+    }
+  } else {
+    // FP64: one operand set in flight (64 fewer VGPRs: 4 waves per SIMD
+    // instead of 2 hide the load latency)
+    for (int st = 0; st < nsteps; ++st) {
+      load_raw(st, ra0, rb0);
+      mma_raw(ra0, rb0);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)

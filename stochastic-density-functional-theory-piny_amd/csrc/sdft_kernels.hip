@@ -115,9 +115,10 @@ __device__ __forceinline__ void expand_body(const ExpandArgs& a, int chunk, int 
   __syncthreads();
   fft_lines_map<T, N, +1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
   Cx<T>* Wq = W + (size_t)q * N * ncolp;
+  const float rnc = slot_rcp(nc);
   for (int e = tid; e < nc * N; e += kThreads) {
-    const int ra = e / nc;
-    const int dl = e - ra * nc;
+    int dl, ra;
+    slot_map<true, 1>(e, nc, rnc, dl, ra);
     Wq[(size_t)ra * ncolp + c0 + dl] = buf[dl * LD + ra];
   }
 }
@@ -301,9 +302,10 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
     col_h[tid] = (a.ent_off[c0 + tid + 1] - a.ent_off[c0 + tid] - 1) >> 1;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
   const Cx<T>* Wq = W + (size_t)q * N * ncolp;
+  const float rnc = slot_rcp(nc);
   for (int e = tid; e < nc * N; e += kThreads) {
-    const int ra = e / nc;
-    const int dl = e - ra * nc;
+    int dl, ra;
+    slot_map<true, 1>(e, nc, rnc, dl, ra);
     buf[dl * LD + ra] = Wq[(size_t)ra * ncolp + c0 + dl];
   }
   __syncthreads();
@@ -462,8 +464,8 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, int chunk, 
     fft_lines_map<T, N, +1, ML, kThreads, kLineFastA<T>>(buf, nc, LD, tw, tid);
     Cx<T>* Wq2 = Wn + (size_t)q * N * ncolp;
     for (int e = tid; e < nc * N; e += kThreads) {
-      const int ra = e / nc;
-      const int dl = e - ra * nc;
+      int dl, ra;
+      slot_map<true, 1>(e, nc, rnc, dl, ra);
       Wq2[(size_t)ra * ncolp + c0 + dl] = buf[dl * LD + ra];
     }
   }
