@@ -424,7 +424,7 @@ def workload_bench(key, nsteps=24, block=128):
                                         "contract_a") if k in kt) / steps
         grid_b = float(N) * wc * npairs
         wbytes = float(wl.n) * ncol * wc * npairs
-        qc = 8 if prec == hip.PREC_FP32 else 4  # pairs per product workgroup
+        qc = 4  # pairs per product workgroup
         nch = -(-npairs // qc)
         wr = wc // 2  # one real value
         # product: the grids at every (atom, point) entry, beta + index per

@@ -37,21 +37,28 @@ constexpr int kRsThreads = 256;
 // touches are those of the few atoms near it), else a run of 256 consecutive
 // grid points.  At most kRsMaxSlots projectors per block are staged in LDS,
 // and at most kRsStageBytes of their Z rows at a time; a point's first
-// kRsRegEntries list entries are held in registers across the pairs.
+// kRsRegEntries list entries are held in registers across the pairs, and
+// kRsAccPairs pairs are accumulated at a time.  The apply kernel streams the
+// pair grids and is bound by memory latency: 8 + 8 (94 VGPRs in FP32, 124 in
+// FP64) keeps 4-5 waves per SIMD in flight where 16 + 16 (159 / 213 VGPRs)
+// kept 2-3, and measured 19 % (FP32) / 26 % (FP64) faster at 192^3.
 constexpr int kRsApplyPoints = kRsThreads;
 constexpr int kRsTile = 16;
 constexpr int kRsMaxSlots = 256;
 constexpr size_t kRsStageBytes = 64 * 1024;
-constexpr int kRsRegEntries = 16;
-constexpr int kRsAccPairs = 16;
+constexpr int kRsRegEntries = 8;
+constexpr int kRsAccPairs = 8;
 
-// grid point of thread t of apply block b (tiled: nt = n / 16 tiles per axis)
-__host__ __device__ inline long long rs_block_point(int b, int t, int n, bool tiled) {
-  if (!tiled) return (long long)b * kRsApplyPoints + t;
-  const int nt = n / kRsTile;
-  const int ra = b / (nt * nt), rem = b - ra * nt * nt;
-  const int tb = rem / nt, tc = rem - tb * nt;
-  const int rb = tb * kRsTile + t / kRsTile, rc = tc * kRsTile + t % kRsTile;
+// grid point of thread t of apply block b.  Tiled: blocks of kRsApplyPoints
+// points, tb x tc in (rb, rc) with tc the run along rc (tb = points / tc),
+// n / tb x n / tc tiles per a-plane; else runs of kRsApplyPoints points.
+__host__ __device__ inline long long rs_block_point(int b, int t, int n, int tc) {
+  if (tc == 0) return (long long)b * kRsApplyPoints + t;
+  const int tb = kRsApplyPoints / tc;
+  const int nb = n / tb, nc = n / tc;
+  const int ra = b / (nb * nc), rem = b - ra * nb * nc;
+  const int ib = rem / nc, ic = rem - ib * nc;
+  const int rb = ib * tb + t / tc, rc = ic * tc + t % tc;
   return ((long long)ra * n + rb) * n + rc;
 }
 
@@ -79,7 +86,8 @@ struct RsKB {
   int* bpp = nullptr;         // projector ids
   int nblock = 0;
   int max_slots = 0;          // largest staged set
-  bool tiled = false;         // 16 x 16 (rb, rc) tiles (else runs of 256 points)
+  bool tiled = false;         // (rb, rc) tiles of 256 points (else runs of 256 points)
+  int tile_c = 0;             // the tiles' extent along rc (0: runs)
   double* escale = nullptr;   // [nproj] E_p vol_elem / N (energies)
   // per-call buffers
   void* F = nullptr;          // pair grids, complex in the form's type
@@ -287,8 +295,11 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
   }
   // per point block: the distinct projectors (ascending), each entry's slot
   const int n = c.g.n;
-  const bool tiled = n % kRsTile == 0;
-  const int nblock = tiled ? n * (n / kRsTile) * (n / kRsTile)
+  const int tcw = kRsTile;
+  const int tbw = kRsApplyPoints / tcw;
+  const bool tiled = n % tcw == 0 && n % tbw == 0;
+  const int tc = tiled ? tcw : 0;
+  const int nblock = tiled ? n * (n / tbw) * (n / tcw)
                            : (int)((N + kRsApplyPoints - 1) / kRsApplyPoints);
   std::vector<int> bpoff{0}, bpp, ptslot((size_t)nent);
   std::vector<int> mark(nproj, -1);
@@ -296,7 +307,7 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
   for (int b = 0; b < nblock; ++b) {
     std::vector<int> set;
     for (int t = 0; t < kRsApplyPoints; ++t) {
-      const long long pt = rs_block_point(b, t, n, tiled);
+      const long long pt = rs_block_point(b, t, n, tc);
       if (pt >= N) break;
       for (long long e = ptoff[(size_t)pt]; e < ptoff[(size_t)pt + 1]; ++e) {
         const int p = ptp[(size_t)e];
@@ -309,7 +320,7 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
     std::sort(set.begin(), set.end());
     if ((int)set.size() > kRsMaxSlots) set.resize(kRsMaxSlots);
     for (int t = 0; t < kRsApplyPoints; ++t) {
-      const long long pt = rs_block_point(b, t, n, tiled);
+      const long long pt = rs_block_point(b, t, n, tc);
       if (pt >= N) break;
       for (long long e = ptoff[(size_t)pt]; e < ptoff[(size_t)pt + 1]; ++e) {
         const int p = ptp[(size_t)e];
@@ -322,6 +333,7 @@ int rs_set(Ctx& c, int nproj, const long long* off, const int* idx, const double
     bpoff.push_back((int)bpp.size());
   }
   r->tiled = tiled;
+  r->tile_c = tc;
   r->nblock = nblock;
   r->max_slots = max_slots;
   std::vector<float> gval32(gval.begin(), gval.end());
@@ -383,7 +395,7 @@ __device__ inline T wave_sum(T v) {
 // g % 8: workgroups are dealt to the 8 XCDs in turn (w mod 8), so all chunks
 // of a group run on one XCD at about the same time and re-read its beta
 // values and indices from that XCD's L2.
-template <typename T, int QC>
+template <typename T, int QC, int U = 1>
 __global__ __launch_bounds__(kRsThreads) void k_rs_project(
     const long long* __restrict__ goff, const int* __restrict__ gidx, const T* __restrict__ gval,
     const int* __restrict__ gp0, const int* __restrict__ gnch, const Cx<T>* __restrict__ F,
@@ -405,20 +417,33 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_project(
       ai[qq][ch] = T(0);
     }
   const Cx<T>* Fq = F + (size_t)q0 * Ngrid;
-  for (long long e = e0 + threadIdx.x; e < e1; e += kRsThreads) {
-    const size_t r = (size_t)gidx[e];
-    T b[4];
+  // U points per thread and trip: their index loads, then all their grid
+  // loads, are in flight together (a point past the list reads point 0 of
+  // the grid with zero weights)
+  for (long long e = e0 + threadIdx.x; e < e1; e += U * kRsThreads) {
+    size_t r[U];
+    T b[U][4];
 #pragma unroll
-    for (int ch = 0; ch < 4; ++ch) b[ch] = gval[e * 4 + ch];
+    for (int u = 0; u < U; ++u) {
+      const long long eu = e + u * kRsThreads;
+      const bool ok = eu < e1;
+      r[u] = ok ? (size_t)gidx[eu] : 0;
+#pragma unroll
+      for (int ch = 0; ch < 4; ++ch) b[u][ch] = ok ? gval[eu * 4 + ch] : T(0);
+    }
 #pragma unroll
     for (int qq = 0; qq < QC; ++qq) {
       if (qq < nq) {
-        const Cx<T> z = Fq[(size_t)qq * Ngrid + r];
+        Cx<T> z[U];
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) {
-          ar[qq][ch] += b[ch] * z.x;
-          ai[qq][ch] += b[ch] * z.y;
-        }
+        for (int u = 0; u < U; ++u) z[u] = Fq[(size_t)qq * Ngrid + r[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int ch = 0; ch < 4; ++ch) {
+            ar[qq][ch] += b[u][ch] * z[u].x;
+            ai[qq][ch] += b[u][ch] * z[u].y;
+          }
       }
     }
   }
@@ -457,17 +482,17 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_project(
 // point and entry; an entry whose projector did not fit the staged set
 // (slot < 0) reads Z directly.  A point's first kRsRegEntries entries stay in
 // registers across the pairs; the rest (crowded points) are re-read.
-template <typename T>
+template <typename T, int QA = kRsAccPairs, int ER = kRsRegEntries>
 __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
     const long long* __restrict__ ptoff, const int* __restrict__ ptslot, const T* __restrict__ ptw,
     const int* __restrict__ bpoff, const int* __restrict__ bpp, const T* __restrict__ v,
-    const Cx<T>* __restrict__ Z, long long Ngrid, int n, int tiled, int npairs, int qz,
+    const Cx<T>* __restrict__ Z, long long Ngrid, int n, int tc, int npairs, int qz,
     Cx<T>* __restrict__ F) {
-  constexpr int QA = kRsAccPairs, ER = kRsRegEntries;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Cx<T>* Zl = reinterpret_cast<Cx<T>*>(smem_raw);  // [slots][qz]
+  Cx<T>* Zl = reinterpret_cast<Cx<T>*>(smem_raw);  // [slots][qz + 1]
+  const int ldz = qz + 1;
   const int b = blockIdx.x;
-  const long long r = rs_block_point(b, threadIdx.x, n, tiled != 0);
+  const long long r = rs_block_point(b, threadIdx.x, n, tc);
   const bool live = r < Ngrid;
   const int s0 = bpoff[b], ns = bpoff[b + 1] - s0;
   const T vr = live ? v[r] : T(0);
@@ -486,7 +511,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
     __syncthreads();  // the previous pass's readers are done with Zl
     for (int i = threadIdx.x; i < ns * qz; i += kRsThreads) {
       const int sl = i / qz, qq = i - sl * qz;
-      Zl[i] = qq < nqs ? Z[(size_t)bpp[s0 + sl] * npairs + q0 + qq] : zero;
+      Zl[sl * ldz + qq] = qq < nqs ? Z[(size_t)bpp[s0 + sl] * npairs + q0 + qq] : zero;
     }
     __syncthreads();
     if (!live) continue;
@@ -512,7 +537,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_apply(
     if (sl_ >= 0) {                                                         \
       _Pragma("unroll") for (int qq = 0; qq < QA; ++qq) {                   \
         if (qq < nq) {                                                      \
-          const Cx<T> zz = Zl[sl_ * qz + q1 + qq];                          \
+          const Cx<T> zz = Zl[sl_ * ldz + q1 + qq];                         \
           acc[qq].x += w_ * zz.x;                                           \
           acc[qq].y += w_ * zz.y;                                           \
         }                                                                   \
@@ -718,50 +743,62 @@ int rs_from_grid(Ctx& c, int prec, void* W, int npairs) {
   return plane_field<2>(c, prec, W, npairs);
 }
 
-// pairs per workgroup of the product / apply kernels: register budget of the
-// partial sums (QC x 4 complex per thread in the product)
-template <typename T>
-constexpr int rs_qc() { return sizeof(T) == 8 ? 4 : 8; }
-
-template <typename T>
-static int project_impl(Ctx& c, int npairs) {
+template <typename T, int QC, int U>
+static int project_launch(Ctx& c, int npairs) {
   RsKB& r = *c.rs;
-  constexpr int QC = rs_qc<T>();
   int rc = rs_grow(&r.Z, &r.Z_bytes, sizeof(Cx<T>) * (size_t)r.nproj * npairs);
   if (rc) return rc;
   const int nchunk = (npairs + QC - 1) / QC;
   const dim3 grid((unsigned)((r.ngroup + 7) / 8 * 8 * nchunk));
   const T* gval = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.gval64)
                                  : reinterpret_cast<const T*>(r.gval32);
-  hipLaunchKernelGGL((k_rs_project<T, QC>), grid, dim3(kRsThreads), 0, c.stream, r.goff, r.gidx,
-                     gval, r.gp0, r.gnch, reinterpret_cast<const Cx<T>*>(r.F), (long long)c.g.N,
-                     npairs, r.ngroup, nchunk, reinterpret_cast<Cx<T>*>(r.Z));
+  hipLaunchKernelGGL((k_rs_project<T, QC, U>), grid, dim3(kRsThreads), 0, c.stream, r.goff,
+                     r.gidx, gval, r.gp0, r.gnch, reinterpret_cast<const Cx<T>*>(r.F),
+                     (long long)c.g.N, npairs, r.ngroup, nchunk, reinterpret_cast<Cx<T>*>(r.Z));
+  SDFT_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// 4 pairs per product workgroup; FP64 with two points per thread and trip.
+// Measured at 128^3 / 192^3 against 8 pairs (FP32) and one point per trip:
+// FP32 0.675 -> 0.591 / 2.00 -> 1.95 ms, FP64 1.035 -> 0.981 / 3.22 -> 3.19 ms;
+// 16 pairs (FP32) or 8 (FP64) were 1.2-1.5x slower (144-150 VGPRs).
+template <typename T>
+static int project_impl(Ctx& c, int npairs) {
+  return project_launch<T, 4, sizeof(T) == 8 ? 2 : 1>(c, npairs);
+}
+
+template <typename T, int QA, int ER>
+static int apply_launch(Ctx& c, int npairs, size_t stage_bytes) {
+  RsKB& r = *c.rs;
+  const long long N = c.g.N;
+  const int slots = std::max(1, r.max_slots);
+  // pairs staged per pass: as many as stage_bytes hold for the largest set,
+  // a multiple of the accumulator width, at most all pairs
+  int qz = (int)(stage_bytes / (sizeof(Cx<T>) * slots));
+  qz = std::max(QA, qz / QA * QA);
+  qz = std::min(qz, (npairs + QA - 1) / QA * QA);
+  // row stride qz + 1: the lanes of a wave read the rows of different
+  // projectors at one pair index, and with a stride of qz (a multiple of 16
+  // elements) every row would start in the same LDS bank
+  const size_t lds = sizeof(Cx<T>) * (size_t)slots * (qz + 1);
+  const T* ptw = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.ptw64)
+                                : reinterpret_cast<const T*>(r.ptw32);
+  const T* v = sizeof(T) == 8 ? reinterpret_cast<const T*>(c.vks64)
+                              : reinterpret_cast<const T*>(c.vks32);
+  auto kern = &k_rs_apply<T, QA, ER>;
+  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(r.nblock), dim3(kRsThreads), lds, c.stream, r.ptoff, r.ptslot,
+                     ptw, r.bpoff, r.bpp, v, reinterpret_cast<const Cx<T>*>(r.Z), N, c.g.n,
+                     r.tile_c, npairs, qz, reinterpret_cast<Cx<T>*>(r.F));
   SDFT_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
 template <typename T>
 static int apply_impl(Ctx& c, int npairs) {
-  RsKB& r = *c.rs;
-  const long long N = c.g.N;
-  const int slots = std::max(1, r.max_slots);
-  // pairs staged per pass: as many as kRsStageBytes hold for the largest
-  // set, a multiple of the accumulator width, at most all pairs
-  int qz = (int)(kRsStageBytes / (sizeof(Cx<T>) * slots));
-  qz = std::max(kRsAccPairs, qz / kRsAccPairs * kRsAccPairs);
-  qz = std::min(qz, (npairs + kRsAccPairs - 1) / kRsAccPairs * kRsAccPairs);
-  const size_t lds = sizeof(Cx<T>) * (size_t)slots * qz;
-  const T* ptw = sizeof(T) == 8 ? reinterpret_cast<const T*>(r.ptw64)
-                                : reinterpret_cast<const T*>(r.ptw32);
-  const T* v = sizeof(T) == 8 ? reinterpret_cast<const T*>(c.vks64)
-                              : reinterpret_cast<const T*>(c.vks32);
-  SDFT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rs_apply<T>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_rs_apply<T>), dim3(r.nblock), dim3(kRsThreads), lds, c.stream, r.ptoff,
-                     r.ptslot, ptw, r.bpoff, r.bpp, v, reinterpret_cast<const Cx<T>*>(r.Z), N,
-                     c.g.n, r.tiled ? 1 : 0, npairs, qz, reinterpret_cast<Cx<T>*>(r.F));
-  SDFT_HIP_CHECK(hipGetLastError());
-  return 0;
+  return apply_launch<T, kRsAccPairs, kRsRegEntries>(c, npairs, kRsStageBytes);
 }
 
 int rs_project(Ctx& c, int prec, int npairs) {
