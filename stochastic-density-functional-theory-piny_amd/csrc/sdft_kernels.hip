@@ -869,7 +869,7 @@ static int plane_impl(Ctx& c, void* W, int npairs, void** Wout) {
     return pr ? plane_variant_launch<T, N, true>(c, a, c.plane_variant, npairs)
               : plane_variant_launch<T, N, false>(c, a, c.plane_variant, npairs);
   }
-  const size_t lds2 = plane_split_lds_bytes<T, N>(g.M);
+  const size_t lds2 = plane_split_lds_bytes<T, N, 2, kThreads, true>(g.M);
   if (lds2 > 160 * 1024) {
     set_error("plane buffer exceeds LDS for this grid in this precision");
     return SDFT_EINVAL_CODE;

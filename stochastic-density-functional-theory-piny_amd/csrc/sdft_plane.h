@@ -70,17 +70,22 @@ struct PlaneCfg {
   static constexpr int CB = V == 0 ? CB0 : largest_divisor_le(N, 2 * CB0 < N ? 2 * CB0 : N);
 };
 
-// The c rows of the plane buffer at n = 64 use the padded slot layout
-// (lds_slot<true>, row stride 72): their passes run line-major (8 threads on
-// the butterflies of one row, so the W gathers and scatters of a wave touch a
-// few rows' consecutive sphere columns) and the radix-8 first pass stores
-// with a stride of 8 elements, which the padding spreads over distinct banks
-// for 8- and 16-byte elements alike.  Elsewhere the rows are unpadded and the
-// FP64 passes run line-fast.
+// Padded slot layout (lds_slot<true>: one unused slot after every 8
+// elements, line stride n + ceil(n / 8)).  Lines in it run line-major
+// (consecutive threads on the butterflies of one line: the W gathers and
+// scatters of a wave touch a few rows' consecutive sphere columns), and the
+// radix-8 first pass stores with a stride of 8 elements, which unpadded puts
+// the n / 8 butterflies of a line on 4 bank pairs and padded spreads over
+// distinct banks, for 8- and 16-byte elements.  The c rows of the plane
+// buffer use it at n = 64 (measured: FP32 74.7 -> 70.9 us, FP64 190.8 ->
+// 183.5 us per Si64 plane launch); at 128 its larger rows cost a workgroup
+// per CU, and at the fragments' 24-48 the unpadded rows (FP64 line-fast)
+// measured faster.  The split form's c-lines in Q use it at every size.
+constexpr int padded_ld(int n) { return n + (n + 7) / 8; }
 template <typename T, int N>
 struct PlaneRows {
   static constexpr bool PAD = N == 64;
-  static constexpr int LDC = PAD ? N + N / 8 : Cfg<N>::LD;
+  static constexpr int LDC = PAD ? padded_ld(N) : Cfg<N>::LD;
   static constexpr bool LINE_FAST = PAD ? false : kLineFast<T>;
 };
 
@@ -216,12 +221,19 @@ struct SplitCfg {
   static constexpr int H = N / S, LDH = H + 1;
   static constexpr int ML = fft_max_lines(N, NT);
   static constexpr int QL = ML > PlaneCfg<T, N>::CB ? ML : PlaneCfg<T, N>::CB;
+  // elements of Q: its QL lines of stride n + 1, or (plane split kernels)
+  // the c-lines' ML rows in the padded slot layout, whichever is larger
+  static constexpr int QE = QL * Cfg<N>::LD;
+  static constexpr int QEP = ML * padded_ld(N) > QE ? ML * padded_ld(N) : QE;
 };
 
-template <typename T, int N, int S = 2, int NT = kThreads>
+// PADQ: the plane split kernels, whose c-lines run padded in Q (the density
+// split kernel keeps stride n + 1)
+template <typename T, int N, int S = 2, int NT = kThreads, bool PADQ = false>
 size_t plane_split_lds_bytes(int M) {
-  return sizeof(Cx<T>) * ((size_t)M * SplitCfg<T, N, S, NT>::LDH +
-                          (size_t)SplitCfg<T, N, S, NT>::QL * Cfg<N>::LD + N);
+  return sizeof(Cx<T>) *
+         ((size_t)M * SplitCfg<T, N, S, NT>::LDH +
+          (size_t)(PADQ ? SplitCfg<T, N, S, NT>::QEP : SplitCfg<T, N, S, NT>::QE) + N);
 }
 
 // MODE 1 / 2 as in plane_body: per half, the inverse b-lines go to the
@@ -231,7 +243,6 @@ template <typename T, int N, int MODE = 0, int NT = kThreads>
 __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int q) {
   constexpr int LD = Cfg<N>::LD, ML = SplitCfg<T, N, 2, NT>::ML, CB = PlaneCfg<T, N>::CB;
   constexpr int H = SplitCfg<T, N, 2, NT>::H, LDH = SplitCfg<T, N, 2, NT>::LDH;
-  constexpr int QL = SplitCfg<T, N, 2, NT>::QL;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
@@ -241,8 +252,9 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
   const T* __restrict__ v = reinterpret_cast<const T*>(a.v);
   const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
   Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows x the half's columns
-  Cx<T>* Q = Ph + M * LDH;                         // QL lines of length N
-  Cx<T>* tw = Q + QL * LD;
+  Cx<T>* Q = Ph + M * LDH;  // QL b-lines of stride n + 1 / ML padded c-lines
+  Cx<T>* tw = Q + SplitCfg<T, N, 2, NT>::QEP;
+  constexpr int LDP = padded_ld(N);
   const int tid = threadIdx.x;
   const int K = (M - 1) / 2;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
@@ -269,7 +281,10 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
         const int c = pos - c_lo;
         if (c >= 0 && c < H) Ph[(r0 + l) * LDH + c] = x;
       };
-      fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, false>(Q, LD, nl, tw, tid, lw, sh);
+      // W in, the half's rows out: Q is free (closing barrier of its last
+      // use) and the rows are not read before the b-lines
+      fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, false, false, false, true, 0, true,
+                    true>(Q, LDP, nl, tw, tid, lw, sh);
     }
     // along b, CB columns at a time
     for (int rc0 = 0; rc0 < H; rc0 += CB) {
@@ -318,7 +333,8 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
         else
           Wq[d] = Wq[d] + x;
       };
-      fft_passes_io<T, N, -1, false, 0, P - 1, ML, NT, false>(Q, LD, nl, tw, tid, lb, sw);
+      fft_passes_io<T, N, -1, false, 0, P - 1, ML, NT, false, false, false, true, 0, true,
+                    true>(Q, LDP, nl, tw, tid, lb, sw);
     }
   }
 }

@@ -695,8 +695,8 @@ static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
   constexpr size_t kLds = 160 * 1024;
   auto waves = [&](size_t lds, int nt) { return lds <= kLds ? (int)(kLds / lds) * (nt / 64) : 0; };
   const size_t l0 = plane2_lds_bytes<T, N, 0>(g.M), l1 = plane2_lds_bytes<T, N, 1>(g.M);
-  const size_t s0 = plane_split_lds_bytes<T, N, 2, kThreads>(g.M);
-  const size_t s1 = plane_split_lds_bytes<T, N, 2, 512>(g.M);
+  const size_t s0 = plane_split_lds_bytes<T, N, 2, kThreads, true>(g.M);
+  const size_t s1 = plane_split_lds_bytes<T, N, 2, 512, true>(g.M);
   const int w0 = waves(l0, kThreads), w1 = waves(l1, 512);
   const int ws0 = waves(s0, kThreads), ws1 = waves(s1, 512);
   const int wun = std::max(w0, w1), wsp = std::max(ws0, ws1);
