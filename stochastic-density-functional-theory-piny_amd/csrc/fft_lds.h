@@ -76,8 +76,8 @@ SDFT_HD inline int mul24(int a, int b) {
 // ---- radix schedule ------------------------------------------------------
 // Schedule S = 0 takes the powers of two in radix-8 passes (one or two
 // radix-4 passes for the remainder); S = 1 in radix-4 passes (one radix-8
-// pass for an odd power).  The two differ where 2^6 divides N (64, 192):
-// 8 x 8 against 4 x 4 x 4.  Radix-4 passes have twice the butterflies per
+// pass for an odd power).  Among the supported sizes they differ where the
+// power of two in N is 2^6 (64, 192): 8 x 8 against 4 x 4 x 4.  Radix-4 passes have twice the butterflies per
 // line, so a workgroup can keep every thread busy on half as many lines
 // (half the LDS for the lines in flight).  Radix-3 passes come last.
 constexpr int exp2_of(int n) { int e = 0; while (n % 2 == 0) { n /= 2; ++e; } return e; }

@@ -836,7 +836,8 @@ static int plane_launch(Ctx& c, const PlaneArgs& a, size_t lds, int npairs) {
 //   0: V 0, 256, 8x8 / 8x8     1: V 1, 256, 8x8 / 8x8   2: V 0, 256, 8x8 / 4x4x4
 //   3: V 1, 512, 4x4x4 / 8x8   4: V 1, 512, 4x4x4 / 4x4x4
 //   5: V 0, 512, 4x4x4 / 4x4x4
-// Variants 2-5 differ from 0 / 1 only where 2^6 divides n and are built for
+// Variants 2-5 differ from 0 / 1 only where the power of two in n is 2^6
+// (64, 192) and are built for
 // n = 64; other sizes take variant (pv & 1).
 template <typename T, int N, bool PR>
 static int plane_variant_launch(Ctx& c, const PlaneArgs& a, int pv, int npairs) {
