@@ -596,6 +596,11 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
         t2 = time.perf_counter()
         bat.filter(coeffs, scale, emean, use_graph=False)
         dt_direct = time.perf_counter() - t2
+        # the whole filter as one graph launch (the first call captures it)
+        bat.filter(coeffs, scale, emean, use_graph=2)
+        t4 = time.perf_counter()
+        bat.filter(coeffs, scale, emean, use_graph=2)
+        dt_whole = time.perf_counter() - t4
         out[tag] = [blocks[i].get(0) for i in sample]
         bat.close()
         # the same fragments one after another (the reference's serial
@@ -609,11 +614,11 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
             b.filter(coeffs[f], scale[f], emean[f])
         dt_serial = time.perf_counter() - t3
         res[tag] = dict(s_per_filter=dt, hpsi_per_s=hpsi / dt, direct_s=dt_direct,
-                        serial_s=dt_serial, split=split, overlap=ov, plane_tile=pv,
-                        fused_expand=fu, cand=cand)
-        log("%s batch: %.3f s/filter (graph), %.3f s (direct), %.4g H.psi/s; serial "
-            "block filters %.3f s (batch/serial speed-up %.2f)" % (
-                tag, dt, dt_direct, hpsi / dt, dt_serial, dt_serial / dt))
+                        whole_graph_s=dt_whole, serial_s=dt_serial, split=split, overlap=ov,
+                        plane_tile=pv, fused_expand=fu, cand=cand)
+        log("%s batch: %.3f s/filter (graph), %.3f s (direct), %.3f s (whole filter as one "
+            "graph), %.4g H.psi/s; serial block filters %.3f s (batch/serial speed-up %.2f)" % (
+                tag, dt, dt_direct, dt_whole, hpsi / dt, dt_serial, dt_serial / dt))
         for b in blocks:
             b.close()
     # FP32 vs FP64 on the sampled fragments
@@ -642,6 +647,8 @@ def batch_bench(args, mu, degree=512, kt=0.01, seed=3):
         "value_fp32": round(r32["hpsi_per_s"], 1),
         "s_per_filter_fp32": round(r32["s_per_filter"], 4),
         "s_per_filter_fp32_direct_launches": round(r32["direct_s"], 4),
+        "s_per_filter_fp32_whole_filter_graph": round(r32["whole_graph_s"], 4),
+        "s_per_filter_fp64_whole_filter_graph": round(r64["whole_graph_s"], 4),
         "serial_block_filters_s_fp32": round(r32["serial_s"], 4),
         "serial_block_filters_s_fp64": round(r64["serial_s"], 4),
         "batch_over_serial_fp32": round(r32["serial_s"] / r32["s_per_filter"], 3),

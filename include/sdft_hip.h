@@ -270,8 +270,11 @@ int sdft_hip_batch_destroy(sdft_batch *bat);
  * noise copy chi is filtered with coeffs[f*L .. f*L+L) and its own scale /
  * emean (scale[f] = 2/(E_max - E_min), emean[f] = (E_min + E_max)/2); the
  * result is in each block's accumulator (sdft_hip_block_get, which = 0).
- * use_graph = 0 launches every step directly (same kernels and order).
- * L >= 2. */
+ * use_graph = 0 launches every step directly (same kernels and order); 1
+ * replays steps 2 .. L-1 from a captured graph of two steps (captured once per
+ * launch configuration); 2 captures the whole filter (chi copy, step counter,
+ * all L-1 steps) as one graph, once per configuration and L, and runs it as
+ * a single graph launch.  All three give identical bits.  L >= 2. */
 int sdft_hip_batch_filter(sdft_batch *bat, int L, const double *coeffs, const double *scale,
                           const double *emean, int use_graph);
 /* Autotuner: times `steps` Chebyshev steps (after one warm-up step) of each

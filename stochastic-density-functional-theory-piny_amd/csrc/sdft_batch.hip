@@ -146,6 +146,11 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
+  // the whole filter of full_L terms (chi copy, counter reset, steps 1 ..
+  // L-1) as one graph; full_L = 0 when absent or out of date
+  hipGraph_t full_graph = nullptr;
+  hipGraphExec_t full_exec = nullptr;
+  int full_L = 0;
   std::vector<int> split_of;  // per fragment slices at the current `split`
   std::vector<int> per_of;    // K steps per slice
   size_t max_zpart_slices = 8;
@@ -156,6 +161,8 @@ namespace {
 void free_batch(Batch& B) {
   if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
   if (B.graph) (void)hipGraphDestroy(B.graph);
+  if (B.full_exec) (void)hipGraphExecDestroy(B.full_exec);
+  if (B.full_graph) (void)hipGraphDestroy(B.full_graph);
   for (Frag& f : B.frags) {
     void* ptrs[] = {f.W, f.zpart, f.Y, f.Z};
     for (void* p : ptrs)
@@ -418,19 +425,58 @@ int copy_chi(Batch& B) {
   return 0;
 }
 
-// Steps 1 .. L-1 of the recurrence (filter.c:560-633) on every fragment.
-int run_filter(Batch& B, int L, bool use_graph) {
+// The whole filter of L terms: chi copy, step counter, steps 1 .. L-1.
+int enqueue_filter(Batch& B, int L) {
   int rc;
   if ((rc = copy_chi(B))) return rc;
   if ((rc = set_counter(B, 1))) return rc;
   if ((rc = enqueue_step(B, 0, 0))) return rc;  // k = 1
-  int k = 2;
-  if (use_graph && L - 2 >= 2) {
-    if (!B.graph_valid && (rc = capture_graph(B))) return rc;
-    for (; k + 1 < L; k += 2) SDFT_HIP_CHECK(hipGraphLaunch(B.graph_exec, B.stream));
-  }
-  for (; k < L; ++k)
+  for (int k = 2; k < L; ++k)
     if ((rc = enqueue_step(B, (k & 1) ? 2 : 1, k & 1))) return rc;
+  return 0;
+}
+
+int capture_full_graph(Batch& B, int L) {
+  if (B.full_exec) SDFT_HIP_CHECK(hipGraphExecDestroy(B.full_exec));
+  if (B.full_graph) SDFT_HIP_CHECK(hipGraphDestroy(B.full_graph));
+  B.full_exec = nullptr;
+  B.full_graph = nullptr;
+  B.full_L = 0;
+  SDFT_HIP_CHECK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeRelaxed));
+  const int rc = enqueue_filter(B, L);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(B.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  B.full_graph = g;
+  SDFT_HIP_CHECK(hipGraphInstantiate(&B.full_exec, g, nullptr, nullptr, 0));
+  B.full_L = L;
+  return 0;
+}
+
+// Steps 1 .. L-1 of the recurrence (filter.c:560-633) on every fragment.
+// use_graph 0: every kernel launched directly; 1: steps 2 .. replayed from
+// the captured pair of steps; 2: the whole filter replayed as one graph.
+int run_filter(Batch& B, int L, int use_graph) {
+  int rc;
+  if (use_graph == 2) {
+    if (B.full_L != L && (rc = capture_full_graph(B, L))) return rc;
+    SDFT_HIP_CHECK(hipGraphLaunch(B.full_exec, B.stream));
+  } else {
+    if ((rc = copy_chi(B))) return rc;
+    if ((rc = set_counter(B, 1))) return rc;
+    if ((rc = enqueue_step(B, 0, 0))) return rc;  // k = 1
+    int k = 2;
+    if (use_graph == 1 && L - 2 >= 2) {
+      if (!B.graph_valid && (rc = capture_graph(B))) return rc;
+      for (; k + 1 < L; k += 2) SDFT_HIP_CHECK(hipGraphLaunch(B.graph_exec, B.stream));
+    }
+    for (; k < L; ++k)
+      if ((rc = enqueue_step(B, (k & 1) ? 2 : 1, k & 1))) return rc;
+  }
   SDFT_HIP_CHECK(hipStreamSynchronize(B.stream));
   // T_{L-1} sits in X[(L-1) & 1]
   for (Frag& f : B.frags) f.b->cur = (L - 1) & 1;
@@ -622,6 +668,7 @@ int sdft_hip_batch_config(sdft_batch* h, int split, int overlap, int plane_tile,
   B.plane_variant = plane_tile;
   B.fused = fused_expand != 0;
   B.graph_valid = false;
+  B.full_L = 0;
   return build_kb_tables(B);
 }
 
@@ -638,7 +685,11 @@ int sdft_hip_batch_filter(sdft_batch* h, int L, const double* coeffs, const doub
   Batch& B = h->B;
   int rc = build_args(B, L, coeffs, scale, emean);
   if (rc) return rc;
-  return run_filter(B, L, use_graph != 0);
+  if (use_graph < 0 || use_graph > 2) {
+    set_error("use_graph must be 0, 1 or 2");
+    return SDFT_EINVAL;
+  }
+  return run_filter(B, L, use_graph);
 }
 
 int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_cand,
@@ -711,6 +762,7 @@ int sdft_hip_batch_autotune(sdft_batch* h, int steps, int* config_out, int max_c
   B.plane_variant = best_pv;
   B.fused = best_fu;
   B.graph_valid = false;
+  B.full_L = 0;
   int rc2 = build_kb_tables(B);
   if (rc) return rc;
   if (rc2) return rc2;

@@ -475,7 +475,9 @@ class Batch:
 
     def filter(self, coeffs, scale, emean, use_graph=True):
         """coeffs (nfrag, L); scale / emean (nfrag,).  Results in each block's
-        accumulator (Block.get(0))."""
+        accumulator (Block.get(0)).  use_graph: False / 0 direct launches,
+        True / 1 the captured pair of steps replayed, 2 the whole filter as
+        one graph launch."""
         coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
         nf, L = coeffs.shape
         assert nf == len(self.blocks)
@@ -483,7 +485,7 @@ class Batch:
         em = np.ascontiguousarray(emean, dtype=np.float64)
         assert sc.shape == (nf,) and em.shape == (nf,)
         _check(self.lib.sdft_hip_batch_filter(self.h, L, _ptr(coeffs), _ptr(sc), _ptr(em),
-                                              1 if use_graph else 0))
+                                              int(use_graph)))
 
     def autotune(self, steps=8):
         """Returns ((split, overlap, plane_tile, fused_expand), per-candidate

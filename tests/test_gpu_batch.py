@@ -107,6 +107,29 @@ def test_batch_graph_equals_direct_and_block(hip, frags):
     bat.filter(co, scale, emean, use_graph=True)
     for f in range(len(frags)):
         assert np.array_equal(blocks[f].get(0), g[f]), f
+    # the whole filter as one graph launch: identical bits, also after a
+    # filter of another length (recaptured) and a configuration change
+    bat.filter(co, scale, emean, use_graph=2)
+    for f in range(len(frags)):
+        assert np.array_equal(blocks[f].get(0), g[f]), f
+    co16, sc16, em16 = _coeffs(frags, 16)
+    bat.filter(co16, sc16, em16, use_graph=2)
+    h16 = [b.get(0) for b in blocks]
+    bat.filter(co16, sc16, em16, use_graph=False)
+    for f in range(len(frags)):
+        assert np.array_equal(blocks[f].get(0), h16[f]), f
+    bat.filter(co, scale, emean, use_graph=2)
+    for f in range(len(frags)):
+        assert np.array_equal(blocks[f].get(0), g[f]), f
+    bat.config(2, 0, 1, 1)
+    bat.filter(co, scale, emean, use_graph=2)
+    h2 = [b.get(0) for b in blocks]
+    bat.filter(co, scale, emean, use_graph=False)
+    for f in range(len(frags)):
+        assert np.array_equal(blocks[f].get(0), h2[f]), f
+        assert rel(h2[f], g[f]) < 1e-12, f
+    with pytest.raises(RuntimeError):
+        bat.filter(co, scale, emean, use_graph=3)
     bat.close()
 
 
