@@ -216,11 +216,14 @@ __device__ __forceinline__ void plane_body(const PlaneArgs& a, int ra, int q) {
 // input, and W (the input of both halves) is left intact.
 // S column parts of width H = n / S (the plane kernels use halves; the
 // density kernel also quarters, for the density sphere's rows at 192^3)
-template <typename T, int N, int S = 2, int NT = kThreads>
+// V: the b-line tile (PlaneCfg); the 512-thread split kernels take the
+// doubled tile, whose lines fit in the rows the padded c-lines reserve.
+template <typename T, int N, int S = 2, int NT = kThreads, int V = 0>
 struct SplitCfg {
   static constexpr int H = N / S, LDH = H + 1;
   static constexpr int ML = fft_max_lines(N, NT);
-  static constexpr int QL = ML > PlaneCfg<T, N>::CB ? ML : PlaneCfg<T, N>::CB;
+  static constexpr int CB = PlaneCfg<T, N, V>::CB;
+  static constexpr int QL = ML > CB ? ML : CB;
   // elements of Q: its QL lines of stride n + 1, or (plane split kernels)
   // the c-lines' ML rows in the padded slot layout, whichever is larger
   static constexpr int QE = QL * Cfg<N>::LD;
@@ -229,20 +232,20 @@ struct SplitCfg {
 
 // PADQ: the plane split kernels, whose c-lines run padded in Q (the density
 // split kernel keeps stride n + 1)
-template <typename T, int N, int S = 2, int NT = kThreads, bool PADQ = false>
+template <typename T, int N, int S = 2, int NT = kThreads, bool PADQ = false, int V = 0>
 size_t plane_split_lds_bytes(int M) {
-  return sizeof(Cx<T>) *
-         ((size_t)M * SplitCfg<T, N, S, NT>::LDH +
-          (size_t)(PADQ ? SplitCfg<T, N, S, NT>::QEP : SplitCfg<T, N, S, NT>::QE) + N);
+  using C = SplitCfg<T, N, S, NT, V>;
+  return sizeof(Cx<T>) * ((size_t)M * C::LDH + (size_t)(PADQ ? C::QEP : C::QE) + N);
 }
 
 // MODE 1 / 2 as in plane_body: per half, the inverse b-lines go to the
 // real-space grid a.field (no forward part), or the forward b-lines come from
 // it (no inverse part; W is not read, the output goes to Wout).
-template <typename T, int N, int MODE = 0, int NT = kThreads>
+template <typename T, int N, int MODE = 0, int NT = kThreads, int V = 0>
 __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int q) {
-  constexpr int LD = Cfg<N>::LD, ML = SplitCfg<T, N, 2, NT>::ML, CB = PlaneCfg<T, N>::CB;
-  constexpr int H = SplitCfg<T, N, 2, NT>::H, LDH = SplitCfg<T, N, 2, NT>::LDH;
+  using C = SplitCfg<T, N, 2, NT, V>;
+  constexpr int LD = Cfg<N>::LD, ML = C::ML, CB = C::CB;
+  constexpr int H = C::H, LDH = C::LDH;
   constexpr int P = n_passes(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const Cx<T>* __restrict__ W = reinterpret_cast<const Cx<T>*>(a.W);
@@ -253,7 +256,7 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
   const Cx<T>* __restrict__ twg = reinterpret_cast<const Cx<T>*>(a.tw);
   Cx<T>* Ph = reinterpret_cast<Cx<T>*>(smem_raw);  // M rows x the half's columns
   Cx<T>* Q = Ph + M * LDH;  // QL b-lines of stride n + 1 / ML padded c-lines
-  Cx<T>* tw = Q + SplitCfg<T, N, 2, NT>::QEP;
+  Cx<T>* tw = Q + C::QEP;
   constexpr int LDP = padded_ld(N);
   const int tid = threadIdx.x;
   const int K = (M - 1) / 2;

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(NT) void k_plane_field(PlaneArgs a) {
 
 template <typename T, int N, int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_plane_split_field(PlaneArgs a) {
-  plane_split_body<T, N, MODE, NT>(a, blockIdx.x, blockIdx.y);
+  plane_split_body<T, N, MODE, NT, NT == kThreads ? 0 : 1>(a, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
@@ -696,7 +696,7 @@ static int plane_field_impl(Ctx& c, void* W, void* F, int npairs) {
   auto waves = [&](size_t lds, int nt) { return lds <= kLds ? (int)(kLds / lds) * (nt / 64) : 0; };
   const size_t l0 = plane2_lds_bytes<T, N, 0>(g.M), l1 = plane2_lds_bytes<T, N, 1>(g.M);
   const size_t s0 = plane_split_lds_bytes<T, N, 2, kThreads, true>(g.M);
-  const size_t s1 = plane_split_lds_bytes<T, N, 2, 512, true>(g.M);
+  const size_t s1 = plane_split_lds_bytes<T, N, 2, 512, true, 1>(g.M);
   const int w0 = waves(l0, kThreads), w1 = waves(l1, 512);
   const int ws0 = waves(s0, kThreads), ws1 = waves(s1, 512);
   const int wun = std::max(w0, w1), wsp = std::max(ws0, ws1);
