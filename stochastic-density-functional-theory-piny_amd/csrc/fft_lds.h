@@ -333,20 +333,32 @@ SDFT_DEVI void slot_map(int idx, int nlines, float rn, int& line, int& j) {
 // same LDS buffer), and one follows the stores.  LINE_FAST maps consecutive
 // threads to consecutive lines (same butterfly), otherwise to consecutive
 // butterflies of a line.
-template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
-          bool ZIN = false, bool ZOUT = false, int S = 0, bool SYNC_MID = true, class LF,
-          class SF>
-SDFT_DEVI void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
-  constexpr int R = radix_sched(N, PASS, REV, S);
-  constexpr int NB = N / R;
-  constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
-  constexpr int NS = ns_sched(N, PASS, REV, S);
-  constexpr int STEP = N / (NS * R);
-  static_assert(!ZIN || (NS == 1 && R % 4 == 0 && (R == 4 || R == 8)), "ZIN: first pass, radix 4/8");
-  static_assert(!ZOUT || (NS * R == N && R % 4 == 0), "ZOUT: last pass, radix 4/8");
+// Shape of pass PASS: radix, butterflies per line, butterflies per thread
+// (MAXB, for MAXLINES lines over NT threads), stride of the outputs, and the
+// register type holding a thread's inputs.
+template <typename T, int N, int PASS, bool REV, int MAXLINES, int NT, int S = 0>
+struct PassShape {
+  static constexpr int R = radix_sched(N, PASS, REV, S);
+  static constexpr int NB = N / R;
+  static constexpr int MAXB = (MAXLINES * NB + NT - 1) / NT;
+  static constexpr int NS = ns_sched(N, PASS, REV, S);
+  static constexpr int STEP = N / (NS * R);
+  typedef Cx<T> Regs[MAXB][R];
+};
+
+// The load phase of a pass: each thread's butterfly inputs from lf into v.
+// A caller may issue it early (the loads stay in flight across barriers
+// until the inputs are first used) and finish the pass later.
+template <typename T, int N, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
+          bool ZIN = false, int S = 0, class LF>
+SDFT_DEVI void fft_pass_load(int nlines, int tid, LF& lf,
+                             typename PassShape<T, N, PASS, REV, MAXLINES, NT, S>::Regs& v) {
+  using Sh = PassShape<T, N, PASS, REV, MAXLINES, NT, S>;
+  constexpr int R = Sh::R, NB = Sh::NB, MAXB = Sh::MAXB;
+  static_assert(!ZIN || (Sh::NS == 1 && R % 4 == 0 && (R == 4 || R == 8)),
+                "ZIN: first pass, radix 4/8");
   const int total = nlines * NB;
   const float rn = slot_rcp(nlines);
-  Cx<T> v[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
@@ -358,7 +370,20 @@ SDFT_DEVI void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf)
         if (!(ZIN && mid_index<R>(r))) v[b][r] = lf(line, j + r * NB);
     }
   }
-  if constexpr (SYNC_MID) __syncthreads();
+}
+
+// The rest of a pass: twiddle, radix-R DFT, stores through sf, and the
+// closing barrier.
+template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
+          bool ZIN = false, bool ZOUT = false, int S = 0, class SF>
+SDFT_DEVI void fft_pass_finish(int nlines, const Cx<T>* tw, int tid,
+                               typename PassShape<T, N, PASS, REV, MAXLINES, NT, S>::Regs& v,
+                               SF& sf) {
+  using Sh = PassShape<T, N, PASS, REV, MAXLINES, NT, S>;
+  constexpr int R = Sh::R, NB = Sh::NB, MAXB = Sh::MAXB, NS = Sh::NS, STEP = Sh::STEP;
+  static_assert(!ZOUT || (NS * R == N && R % 4 == 0), "ZOUT: last pass, radix 4/8");
+  const int total = nlines * NB;
+  const float rn = slot_rcp(nlines);
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
@@ -384,6 +409,17 @@ SDFT_DEVI void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf)
     }
   }
   __syncthreads();
+}
+
+template <typename T, int N, int DIR, int PASS, bool REV, int MAXLINES, int NT, bool LINE_FAST,
+          bool ZIN = false, bool ZOUT = false, int S = 0, bool SYNC_MID = true, class LF,
+          class SF>
+SDFT_DEVI void fft_pass_io(int nlines, const Cx<T>* tw, int tid, LF& lf, SF& sf) {
+  typename PassShape<T, N, PASS, REV, MAXLINES, NT, S>::Regs v;
+  fft_pass_load<T, N, PASS, REV, MAXLINES, NT, LINE_FAST, ZIN, S>(nlines, tid, lf, v);
+  if constexpr (SYNC_MID) __syncthreads();
+  fft_pass_finish<T, N, DIR, PASS, REV, MAXLINES, NT, LINE_FAST, ZIN, ZOUT, S>(nlines, tw, tid,
+                                                                               v, sf);
 }
 
 // LDS slot of element pos of a line in the padded layout: one unused slot

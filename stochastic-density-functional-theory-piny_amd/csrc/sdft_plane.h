@@ -289,7 +289,16 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
       fft_passes_io<T, N, +1, false, 0, P - 1, ML, NT, false, false, false, true, 0, true,
                     true>(Q, LDP, nl, tw, tid, lw, sh);
     }
-    // along b, CB columns at a time
+    // along b, CB columns at a time.  MODE 2: the forward b-lines start from
+    // the grid F, and the first pass's inputs of the next sub-chunk are
+    // loaded while this sub-chunk's other passes run (they stay in flight
+    // across the LDS barriers)
+    using P0 = PassShape<T, N, 0, true, CB, NT>;
+    typename P0::Regs fpre;
+    int rcl = 0;
+    auto lfp = [&](int l, int pos) { return F[(size_t)pos * N + c_lo + rcl + l]; };
+    if constexpr (MODE == 2)
+      fft_pass_load<T, N, 0, true, CB, NT, true>(H < CB ? H : CB, tid, lfp, fpre);
     for (int rc0 = 0; rc0 < H; rc0 += CB) {
       const int nb = H - rc0 < CB ? H - rc0 : CB;
       auto lp = [&](int l, int pos) {
@@ -303,20 +312,31 @@ __device__ __forceinline__ void plane_split_body(const PlaneArgs& a, int ra, int
         const int kb = pos <= N / 2 ? pos : pos - N;
         if (kb >= -K && kb <= K) Ph[(kb + K) * LDH + rc0 + l] = x;
       };
+      // Q is free at each sub-chunk's first pass (closing barrier of its last
+      // use) and the half's rows are read by the first pass only (modes 0 /
+      // 1) or written by the last pass only (mode 2): the passes that read or
+      // write outside Q skip the barrier between their loads and stores
       if constexpr (MODE == 1) {
         auto sf = [&](int l, int pos, Cx<T> x) { F[(size_t)pos * N + c_lo + rc0 + l] = x; };
-        fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lp, sf);
+        fft_passes_io<T, N, +1, false, 0, P - 1, CB, NT, true, false, false, false, 0, true,
+                      true>(Q, LD, nb, tw, tid, lp, sf);
       } else if constexpr (MODE == 2) {
-        auto lf = [&](int l, int pos) { return F[(size_t)pos * N + c_lo + rc0 + l]; };
-        fft_passes_io<T, N, -1, true, 0, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lf, sp);
+        fft_pass_finish<T, N, -1, 0, true, CB, NT, true>(nb, tw, tid, fpre, sq);
+        if (rc0 + CB < H) {
+          rcl = rc0 + CB;
+          fft_pass_load<T, N, 0, true, CB, NT, true>(H - rcl < CB ? H - rcl : CB, tid, lfp,
+                                                     fpre);
+        }
+        fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true, false, false, false, 0, false,
+                      true>(Q, LD, nb, tw, tid, lq, sp);
       } else {
         if constexpr (P >= 2)
-          fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true>(Q, LD, nb, tw, tid, lp,
-                                                                       sq);
+          fft_passes_io<T, N, +1, false, 0, P - 2, CB, NT, true, false, false, false, 0, true,
+                        false>(Q, LD, nb, tw, tid, lp, sq);
         fft_fused_vmul<T, N, CB, NT, true>(nb, tw, tid, lq, vf, sq);
         if constexpr (P >= 2)
-          fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true>(Q, LD, nb, tw, tid, lq,
-                                                                      sp);
+          fft_passes_io<T, N, -1, true, 1, P - 1, CB, NT, true, false, false, false, 0, false,
+                        true>(Q, LD, nb, tw, tid, lq, sp);
       }
     }
     // forward along c of the rows with the other half zero: store / add

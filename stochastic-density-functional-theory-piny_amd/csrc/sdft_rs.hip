@@ -375,8 +375,12 @@ __global__ __launch_bounds__(NT) void k_plane_field(PlaneArgs a) {
   plane_body<T, N, NT == kThreads ? 0 : 1, PR, MODE, NT>(a, blockIdx.x, blockIdx.y);
 }
 
+// FP32 512-thread form: at most 128 VGPRs, so that two workgroups (16 waves)
+// share a CU where their LDS allows it
 template <typename T, int N, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_plane_split_field(PlaneArgs a) {
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NT == 512 ? 4 : 1))) void
+k_plane_split_field(PlaneArgs a) {
   plane_split_body<T, N, MODE, NT, NT == kThreads ? 0 : 1>(a, blockIdx.x, blockIdx.y);
 }
 
