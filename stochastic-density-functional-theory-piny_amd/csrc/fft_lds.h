@@ -480,6 +480,9 @@ SDFT_DEVI void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& 
   const int total = nlines * NB;
   const float rn = slot_rcp(nlines);
   Cx<T> v[MAXB][R];
+  // the multipliers (global memory) are loaded with the inputs, before the
+  // barrier, so that their latency is not exposed after it
+  T sv[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int idx = tid + b * NT;
@@ -487,7 +490,10 @@ SDFT_DEVI void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& 
       int line, j;
       slot_map<LINE_FAST, NB>(idx, nlines, rn, line, j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lf(line, j + r * NB);
+      for (int r = 0; r < R; ++r) {
+        v[b][r] = lf(line, j + r * NB);
+        sv[b][r] = vf(line, j + r * NB);
+      }
     }
   }
   __syncthreads();
@@ -503,10 +509,7 @@ SDFT_DEVI void fft_fused_vmul(int nlines, const Cx<T>* tw, int tid, LF& lf, VF& 
       }
       Dft<T, R, +1>::run(v[b]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const T s = vf(line, j + r * NB);
-        v[b][r] = cx(v[b][r].x * s, v[b][r].y * s);
-      }
+      for (int r = 0; r < R; ++r) v[b][r] = cx(v[b][r].x * sv[b][r], v[b][r].y * sv[b][r]);
       Dft<T, R, -1>::run(v[b]);
 #pragma unroll
       for (int r = 0; r < R; ++r) sf(line, j * R + r, v[b][r]);
