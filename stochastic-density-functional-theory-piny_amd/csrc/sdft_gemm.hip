@@ -50,10 +50,6 @@ struct Mfma<float> {
 
 constexpr int KS = 32;   // K per step: 4 lane groups x 8 consecutive k
 constexpr int KPL = 8;   // k per lane per step
-// pipe_ring depth of the two products per form.  FP64 force keeps one
-// operand set: 64 fewer VGPRs than two, 4 waves per SIMD instead of 2.
-constexpr int kProjectDepth32 = 2, kProjectDepth64 = 2;
-constexpr int kForceDepth32 = 2, kForceDepth64 = 1;
 
 // 8 consecutive values src[0..7] in 16-byte vector loads (src is 32-byte
 // aligned for float, 64-byte for double: row strides and run starts are
@@ -84,59 +80,10 @@ __device__ inline void load_run(const T* __restrict__ src, T (&dst)[KPL]) {
   }
 }
 
-// Software pipeline over the steps first, first + stride, ... < end with a
-// ring of D operand register sets: ld(slot, step) loads a step into a set,
-// mm(slot) runs its MFMAs.  D - 1 steps are in flight while the MFMAs of
-// the current one run.  The main loop is counted, unrolled by D (static
-// slots) and branch-free: every load is issued unconditionally (past the
-// last step it reloads the last one, in bounds and cached) ahead of the
-// MFMAs it overlaps, and a scheduling barrier keeps it there.  The last
-// n mod D steps are peeled.  (A load issued under a condition whose join
-// precedes the MFMAs, `if (next < end) load(next); mma(cur);`, makes the
-// compiler's wait before those MFMAs count the new loads too, so each step
-// waited for the next step's operands: this form measured 6 % faster on
-// both Si64 products.)  Rings of 3-6 measured slower than 2 (the products
-// stream their operands at a rate the memory system, not the latency,
-// limits; more requests in flight only contend), so the products use D = 2,
-// and D = 1 is the plain loop.
-template <int D, typename LD, typename MM>
-__device__ __forceinline__ void pipe_ring(int first, int end, int stride, LD&& ld, MM&& mm) {
-  if constexpr (D == 1) {
-    for (int st = first; st < end; st += stride) {
-      ld(0, st);
-      mm(0);
-    }
-    return;
-  }
-  if (first >= end) return;
-  const int n = (end - first + stride - 1) / stride;  // steps, uniform
-  auto step = [&](int i) { return first + (i < n ? i : n - 1) * stride; };
-#pragma unroll
-  for (int k = 0; k < D - 1; ++k) ld(k, step(k));
-  int i = 0;
-  for (; i + D <= n; i += D) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      ld((k + D - 1) % D, step(i + k + D - 1));
-      __builtin_amdgcn_sched_barrier(0);
-      mm(k);
-    }
-  }
-  // n - i < D steps left, in sets 0 .. n - i - 1
-  const int r = n - i;
-#pragma unroll
-  for (int k = 0; k < D - 1; ++k)
-    if (k < r) mm(k);
-}
-
 // ---- Z partials: zpart[z][p][s] = sum_{k in split z} A[p][k] X[s][k] -------
 // Tiles (ceil(P/32), ceil(ns/32), nsplit); 4 waves per workgroup share the
-// 32x32 output tile and interleave the split's K steps; waves 1-3 stage
-// their partial tiles in LDS in the MFMA type (12 KB in FP32, 24 KB in FP64)
-// and wave 0 adds them to its own in FP64, in wave order, and stores the
-// sum.  The small LDS footprint lets a projection workgroup share a CU with
-// the plane kernel's workgroups on the other stream (FP32: four of them
-// hold 140 KB) instead of displacing one.  Both operands must have their row count
+// 32x32 output tile and interleave the split's K steps; the four partial
+// tiles are summed through LDS.  Both operands must have their row count
 // padded to a multiple of 32 with readable rows (zeros, or values whose
 // products land only in the rows / columns the epilogue does not store):
 // the K loop loads without bounds tests.
@@ -145,7 +92,7 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
                                                 const TX* __restrict__ X, int P, int ns,
                                                 int K, int ksteps_per_split,
                                                 double* __restrict__ zpart, int bx, int by, int bz,
-                                                TI (*red)[32 * 32]) {
+                                                double (*red)[32 * 32]) {
   typedef typename Mfma<TI>::acc_t acc_t;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar K loop
@@ -169,59 +116,64 @@ __device__ __forceinline__ void kb_project_tile(const TA* __restrict__ A,
     xrow[i] = X + (size_t)(s0 + 16 * i + r16) * K + KPL * g;
   }
   // Operands are loaded in their storage types and converted to TI only at
-  // the MFMAs.
-  constexpr int D = sizeof(TI) == 4 ? kProjectDepth32 : kProjectDepth64;
-  TA ra[D][2][KPL];
-  TX rb[D][2][KPL];
-  auto ld = [&](int slot, int st) {
+  // the MFMAs.  The next step's loads are issued under a condition, and the
+  // compiler's wait before the current step's MFMAs then covers them too, so
+  // a wave does not overlap its loads with its MFMAs.  A counted, branch-free
+  // form that does (correct waits, rings of 2-6 steps in flight) ran the
+  // products 6 % faster alone at 2 steps and slower at more; beside the
+  // plane kernel on the other stream it left the Si64 step unchanged and
+  // the fragment batch 1 % slower.  These products are bound by how the
+  // memory system serves their operand stream, shared with the plane kernel.
+  TA ra0[2][KPL], ra1[2][KPL];
+  TX rb0[2][KPL], rb1[2][KPL];
+  auto load_raw = [&](int st, TA (&a)[2][KPL], TX (&b)[2][KPL]) {
     const int k0 = st * KS;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      load_run<TA>(arow[i] + k0, ra[slot][i]);
-      load_run<TX>(xrow[i] + k0, rb[slot][i]);
+      load_run<TA>(arow[i] + k0, a[i]);
+      load_run<TX>(xrow[i] + k0, b[i]);
     }
   };
-  auto mm = [&](int slot) {
+  auto mma_raw = [&](TA (&a)[2][KPL], TX (&b)[2][KPL]) {
 #pragma unroll
     for (int t = 0; t < KPL; ++t)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = Mfma<TI>::run(TI(ra[slot][i][t]), TI(rb[slot][j][t]), acc[i][j]);
+          acc[i][j] = Mfma<TI>::run(TI(a[i][t]), TI(b[j][t]), acc[i][j]);
   };
-  pipe_ring<D>(step0 + wave, step1, 4, ld, mm);
-  // sum the four waves' tiles: waves 1-3 through LDS into wave 0
-  if (wave > 0) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = 16 * i + Mfma<TI>::row(lane, r);
-          const int col = 16 * j + r16;
-          red[wave - 1][row * 32 + col] = acc[i][j][r];
-        }
+  int st = step0 + wave;
+  if (st < step1) load_raw(st, ra0, rb0);
+  while (st < step1) {
+    const int nx = st + 4;
+    if (nx < step1) load_raw(nx, ra1, rb1);
+    mma_raw(ra0, rb0);
+    st = nx;
+    if (st >= step1) break;
+    const int nx2 = st + 4;
+    if (nx2 < step1) load_raw(nx2, ra0, rb0);
+    mma_raw(ra1, rb1);
+    st = nx2;
   }
-  __syncthreads();
-  if (wave > 0) return;
-  double* zp = zpart + (size_t)bz * P * ns;
+  // sum the four waves' tiles
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = 16 * i + Mfma<TI>::row(lane, r);
-      const int p = p0 + row;
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int col = 16 * j + r16, e = row * 32 + col;
-        const int s = s0 + col;
-        if (p < P && s < ns)
-          zp[(size_t)p * ns + s] = double(acc[i][j][r]) + double(red[0][e]) +
-                                   double(red[1][e]) + double(red[2][e]);
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * i + Mfma<TI>::row(lane, r);
+        const int col = 16 * j + r16;
+        red[wave][row * 32 + col] = double(acc[i][j][r]);
       }
-    }
+  __syncthreads();
+  double* zp = zpart + (size_t)bz * P * ns;
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int row = e >> 5, col = e & 31;
+    const int p = p0 + row, s = s0 + col;
+    if (p < P && s < ns) zp[(size_t)p * ns + s] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+  }
 }
 
 // ---- F[s][k] = sum_p Y[s][p] At[k][p] ----------------------------------------
@@ -262,26 +214,43 @@ __device__ __forceinline__ void kb_force_block(const T* __restrict__ Y, const T*
   for (int i = 0; i < 2; ++i) arow[i] = Y + (size_t)(s0 + 16 * i + r16) * ppad + KPL * g;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) xrow[j] = At + (size_t)(k0 + 16 * j + r16) * ppad + KPL * g;
-  constexpr int D = sizeof(T) == 4 ? kForceDepth32 : kForceDepth64;
-  T ra[D][2][KPL];
-  T rb[D][NJ][KPL];
-  auto ld = [&](int slot, int st) {
+  T ra0[2][KPL], ra1[2][KPL];
+  T rb0[NJ][KPL], rb1[NJ][KPL];
+  auto load_raw = [&](int st, T (&a)[2][KPL], T (&b)[NJ][KPL]) {
     const int kk = st * KS;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) load_run<T>(arow[i] + kk, ra[slot][i]);
+    for (int i = 0; i < 2; ++i) load_run<T>(arow[i] + kk, a[i]);
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) load_run<T>(xrow[j] + kk, rb[slot][j]);
+    for (int j = 0; j < NJ; ++j) load_run<T>(xrow[j] + kk, b[j]);
   };
-  auto mm = [&](int slot) {
+  auto mma_raw = [&](T (&a)[2][KPL], T (&b)[NJ][KPL]) {
 #pragma unroll
     for (int t = 0; t < KPL; ++t)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j)
-          acc[i][j] = Mfma<T>::run(ra[slot][i][t], rb[slot][j][t], acc[i][j]);
+        for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::run(a[i][t], b[j][t], acc[i][j]);
   };
-  pipe_ring<D>(0, ppad / KS, 1, ld, mm);
+  const int nsteps = ppad / KS;
+  if constexpr (sizeof(T) == 4) {
+    int st = 0;
+    load_raw(0, ra0, rb0);
+    while (st < nsteps) {
+      if (st + 1 < nsteps) load_raw(st + 1, ra1, rb1);
+      mma_raw(ra0, rb0);
+      if (++st >= nsteps) break;
+      if (st + 1 < nsteps) load_raw(st + 1, ra0, rb0);
+      mma_raw(ra1, rb1);
+      ++st;
+    }
+  } else {
+    // FP64: one operand set in flight (64 fewer VGPRs: 4 waves per SIMD
+    // instead of 2 hide the load latency)
+    for (int st = 0; st < nsteps; ++st) {
+      load_raw(st, ra0, rb0);
+      mma_raw(ra0, rb0);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -313,7 +282,7 @@ __global__ __launch_bounds__(256) void k_kb_project(const TA* __restrict__ A,
                                                     int K, int ksteps_per_split,
                                                     double* __restrict__ zpart, int tp, int ts,
                                                     int total) {
-  __shared__ TI red[3][32 * 32];
+  __shared__ double red[4][32 * 32];
   const int L = xcd_swizzle(blockIdx.x, total);
   const int bx = L % tp, r = L / tp;
   kb_project_tile<TA, TX, TI>(A, X, P, ns, K, ksteps_per_split, zpart, bx, r % ts, r / ts, red);
@@ -326,7 +295,7 @@ template <typename TA, typename TI>
 __global__ __launch_bounds__(256) void k_kb_project_batch(const KbProjectArgs* __restrict__ args,
                                                           const int* __restrict__ prefix,
                                                           int nfrag) {
-  __shared__ TI red[3][32 * 32];
+  __shared__ double red[4][32 * 32];
   const int L = xcd_swizzle(blockIdx.x, prefix[nfrag]);
   const int f = find_frag(prefix, nfrag, L);
   const KbProjectArgs a = args[f];
