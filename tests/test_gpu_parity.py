@@ -269,6 +269,29 @@ def test_hpsi_real_space_kb(hip, si8_system, nproj):
     ctx.close()
 
 
+def test_hpsi_real_space_kb_state_counts_grow_and_shrink(hip, si8_system):
+    """One context with the per-atom real-space KB applied to blocks of 2, 9,
+    40, 3 and 33 states in turn: the pair grids, per-pair projections and
+    energy partials it keeps follow the pair count, both where they are
+    reallocated (9 -> 40) and where the earlier allocation is reused (2 -> 9
+    after a first call, 40 -> 3 -> 33).  H.psi FP64 1e-12, FP32 2e-5; NL
+    energy 1e-12 relative, each against the oracle's real-space KB."""
+    from sdft_amd import systems
+    rs = systems.real_space_projectors(si8_system, 4.0)
+    ctx = hip.Context(si8_system, with_nl=False)
+    ctx.set_projectors_real(rs)
+    for i, nstate in enumerate((2, 9, 40, 3, 33)):
+        say(f"case: hpsi real-space KB si8 state-count sequence nstate={nstate}")
+        chi = systems.noise_orbitals(si8_system, nstate, seed=60 + i)
+        ref = O.hpsi_raw(chi, si8_system, nl_real=rs)
+        assert rel(ctx.hpsi_raw(chi, hip.PREC_FP64), ref) < 1e-12, nstate
+        assert rel(ctx.hpsi_raw(chi, hip.PREC_FP32), ref) < 2e-5, nstate
+        _, enl = ctx.energies(chi, 2.0, float(nstate))
+        enl_ref = O.energy_nl_real(chi, si8_system, rs, 2.0, float(nstate))
+        assert abs(enl - enl_ref) < 1e-12 * abs(enl_ref), (nstate, enl, enl_ref)
+    ctx.close()
+
+
 def test_density_generic_layout_and_accumulate(hip, si8_ctx, si8_system):
     """SDFT_RHO_GENERIC: the density in PINY's generic-FFT order (a fastest),
     which the reference converts to and from the FFTW order by a pure
