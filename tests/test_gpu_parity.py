@@ -681,7 +681,8 @@ def test_nl_forces_real_space_per_atom(hip, si8_system):
     (sdft_hip_set_projector_gradients, then sdft_hip_nl_forces) against the
     oracle's nlppKBRealEnergyForce restatement (itself pinned by finite
     differences of the real-space energy): 1e-10 relative to the largest
-    component, for an odd orbital count (a zero-padded pair)."""
+    component, for an odd orbital count (a zero-padded pair), then on the
+    same context for 24 and 3 orbitals."""
     from sdft_amd import systems
     say("case: per-atom real-space NL forces si8")
     s = si8_system
@@ -696,4 +697,12 @@ def test_nl_forces_real_space_per_atom(hip, si8_system):
     say("  per-atom NL forces: max |F| %.4g, rel %.3g" % (np.abs(F_ref).max(), rel(F, F_ref)))
     assert np.abs(F_ref).max() > 0.1
     assert rel(F, F_ref) < 1e-10
+    # the same context with more, then fewer states: its force partials grow
+    # (5 -> 24) and are reused (24 -> 3)
+    for nstate, seed in ((24, 38), (3, 39)):
+        say("case: per-atom real-space NL forces si8 nstate=%d (same context)" % nstate)
+        C = systems.noise_orbitals(s, nstate, seed=seed)
+        F = ctx.nl_forces(C, 2.0, float(nstate))
+        F_ref = O.nl_forces_real(C, s, rs, grad, 2.0, float(nstate))
+        assert rel(F, F_ref) < 1e-10, nstate
     ctx.close()
